@@ -1,0 +1,265 @@
+"""ctypes bindings to the CPU oracle (liboracle.so) and, when it was built, to
+the reference's own translation units (_ref/libarecref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+CPU-baseline leg of bench.py.  The product never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libarecref.so")
+
+METHODS = {"linear": 1, "kb": 2, "sq": 3}
+
+_F = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_D = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+class Hist(C.Structure):
+    _fields_ = [("iter", C.c_int), ("event", C.c_int), ("relnrm", C.c_double), ("rnorm", C.c_double)]
+
+
+_lib = None
+_ref = None
+
+
+def build():
+    """Compile liboracle.so (and _ref when the reference sources are present)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if os.path.isdir("/root/reference/AREC3D/AREC3D"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_cyl_geometry.argtypes = [C.c_double, C.c_int, C.c_void_p]
+        L.orc_cyl_geometry.restype = C.c_int
+        for sfx, arr in (("f", _F), ("d", _D)):
+            getattr(L, f"orc_project_{sfx}").argtypes = [C.c_int, _I, C.c_int, C.c_double, C.c_int, C.c_int, arr, _F, C.c_int, arr]
+            getattr(L, f"orc_backproject_{sfx}").argtypes = [C.c_int, _I, C.c_int, C.c_double, C.c_int, C.c_int, arr, _F, C.c_int, arr]
+        L.orc_sirt_f.argtypes = [C.c_int, _F, C.c_int, C.c_int, C.c_int, _F, C.c_float, C.c_int, C.c_float, _F,
+                                 C.POINTER(Hist), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]
+        L.orc_sirt_d.argtypes = [C.c_int, _D, C.c_int, C.c_int, C.c_int, _F, C.c_double, C.c_int, C.c_double, _D,
+                                 C.POINTER(Hist), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.orc_cgls_f.argtypes = [C.c_int, _F, C.c_int, C.c_int, C.c_int, _F, C.c_int, C.c_float, C.c_int, _F,
+                                 C.POINTER(Hist), C.c_int, C.POINTER(C.c_int)]
+        L.orc_cgls_d.argtypes = [C.c_int, _D, C.c_int, C.c_int, C.c_int, _F, C.c_int, C.c_double, C.c_int, _D,
+                                 C.POINTER(Hist), C.c_int, C.POINTER(C.c_int)]
+        L.orc_cgls_stat_f.argtypes = L.orc_cgls_f.argtypes
+        L.orc_init_weights_f.argtypes = [_F, C.c_int, C.c_int, C.c_int, _F]
+        L.orc_ccorr2d.argtypes = [C.c_int, C.c_int, _F, _F, _D]
+        L.orc_quad2dfit.argtypes = [_F, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_speak.argtypes = [C.c_int, C.c_int, _F, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_fshift2d.argtypes = [C.c_int, C.c_int, _F, _F, C.c_float, C.c_float]
+        L.orc_cc_images.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, _F, _F]
+        L.orc_shift_images.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, _F]
+        L.orc_soft_noneg.argtypes = [_F, C.c_long]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF_PATH)
+        R.ref_cyl_geometry.argtypes = [C.c_double, C.c_int, C.POINTER(C.c_int), C.c_void_p]
+        R.ref_project.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, _F, _F, C.c_int, _F]
+        R.ref_backproject.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, _F, _F, C.c_int, _F]
+        R.ref_solve.argtypes = [C.c_int, C.c_int, _F, C.c_int, C.c_int, C.c_int, _F, C.c_float, C.c_int,
+                                C.c_float, C.c_int, _F, C.POINTER(C.c_int)]
+        _ref = R
+    return _ref
+
+
+def _m(method):
+    return METHODS[method] if isinstance(method, str) else int(method)
+
+
+def cyl_geometry(radius, height=1):
+    L = lib()
+    n = L.orc_cyl_geometry(float(radius), int(height), None)
+    cord = np.zeros(2 * n, np.int32)
+    L.orc_cyl_geometry(float(radius), int(height), cord.ctypes.data)
+    return cord.reshape(n, 2)
+
+
+def project(method, x, angles, nx, ny, dtype=np.float32):
+    """x: [nrays, ny] -> p: [na, ny, nx]."""
+    radius = 0.5 * (nx - 1)
+    cord = np.ascontiguousarray(cyl_geometry(radius).reshape(-1))
+    angles = np.ascontiguousarray(angles, np.float32)
+    x = np.ascontiguousarray(x, dtype)
+    p = np.zeros((len(angles), ny, nx), dtype)
+    f = lib().orc_project_f if dtype == np.float32 else lib().orc_project_d
+    st = f(_m(method), cord, len(cord) // 2, radius, nx, ny, x, angles, len(angles), p)
+    if st:
+        raise RuntimeError(f"orc_project status {st}")
+    return p
+
+
+def backproject(method, p, angles, nx, ny, dtype=np.float32):
+    radius = 0.5 * (nx - 1)
+    cord = np.ascontiguousarray(cyl_geometry(radius).reshape(-1))
+    angles = np.ascontiguousarray(angles, np.float32)
+    p = np.ascontiguousarray(p, dtype)
+    x = np.zeros((len(cord) // 2, ny), dtype)
+    f = lib().orc_backproject_f if dtype == np.float32 else lib().orc_backproject_d
+    st = f(_m(method), cord, len(cord) // 2, radius, nx, ny, p, angles, len(angles), x)
+    if st:
+        raise RuntimeError(f"orc_backproject status {st}")
+    return x
+
+
+def _hist(h, n):
+    return [(h[i].iter, h[i].event, h[i].relnrm, h[i].rnorm) for i in range(min(n, len(h)))]
+
+
+def sirt(method, b, angles, lam, maxit, tol, dtype=np.float32):
+    """Returns (status, x [nrays, ny], history, final lam)."""
+    na, ny, nx = b.shape
+    nrays = len(cyl_geometry(0.5 * (nx - 1)))
+    x = np.zeros((nrays, ny), dtype)
+    H = (Hist * (maxit + 64))()
+    nh = C.c_int(0)
+    if dtype == np.float32:
+        lo = C.c_float(0)
+        st = lib().orc_sirt_f(_m(method), np.ascontiguousarray(b, np.float32), nx, ny, na,
+                              np.ascontiguousarray(angles, np.float32), lam, maxit, tol, x, H, len(H),
+                              C.byref(nh), C.byref(lo))
+    else:
+        lo = C.c_double(0)
+        st = lib().orc_sirt_d(_m(method), np.ascontiguousarray(b, np.float64), nx, ny, na,
+                              np.ascontiguousarray(angles, np.float32), lam, maxit, tol, x, H, len(H),
+                              C.byref(nh), C.byref(lo))
+    return st, x, _hist(H, nh.value), lo.value
+
+
+def cgls(method, b, angles, maxit, tol, x0=None, dtype=np.float32, weighted=False):
+    na, ny, nx = b.shape
+    radius = (nx // 2) if (method in (2, "kb") or weighted) else 0.5 * (nx - 1)
+    nrays = len(cyl_geometry(radius))
+    x = np.zeros((nrays, ny), dtype) if x0 is None else np.array(x0, dtype, copy=True)
+    H = (Hist * (maxit + 4))()
+    nh = C.c_int(0)
+    if weighted:
+        f, bb = lib().orc_cgls_stat_f, np.ascontiguousarray(b, np.float32)
+    elif dtype == np.float32:
+        f, bb = lib().orc_cgls_f, np.ascontiguousarray(b, np.float32)
+    else:
+        f, bb = lib().orc_cgls_d, np.ascontiguousarray(b, np.float64)
+    st = f(_m(method), bb, nx, ny, na, np.ascontiguousarray(angles, np.float32), maxit, tol,
+           0 if x0 is None else 1, x, H, len(H), C.byref(nh))
+    return st, x, _hist(H, nh.value)
+
+
+def init_weights(b):
+    na, ny, nx = b.shape
+    w = np.zeros_like(b, dtype=np.float32)
+    lib().orc_init_weights_f(np.ascontiguousarray(b, np.float32), nx, ny, na, w)
+    return w
+
+
+def ccorr2d(x, y):
+    ny, nx = x.shape
+    c = np.zeros((ny, nx), np.float64)
+    lib().orc_ccorr2d(nx, ny, np.ascontiguousarray(x, np.float32), np.ascontiguousarray(y, np.float32), c)
+    return c
+
+
+def quad2dfit(z3x3):
+    xs, ys, pv = C.c_float(0), C.c_float(0), C.c_float(0)
+    lib().orc_quad2dfit(np.ascontiguousarray(z3x3, np.float32).reshape(9), C.byref(xs), C.byref(ys), C.byref(pv))
+    return xs.value, ys.value, pv.value
+
+
+def speak(data):
+    ny, nx = data.shape
+    xt, yt = C.c_float(0), C.c_float(0)
+    lib().orc_speak(nx, ny, np.ascontiguousarray(data, np.float32), C.byref(xt), C.byref(yt))
+    return xt.value, yt.value
+
+
+def fshift2d(img, sx, sy):
+    ny, nx = img.shape
+    out = np.zeros((ny, nx), np.float32)
+    lib().orc_fshift2d(nx, ny, np.ascontiguousarray(img, np.float32), out, sx, sy)
+    return out
+
+
+def cc_images(proj, meas):
+    nz, ny, nx = proj.shape
+    sx = np.zeros(nz, np.float32)
+    sy = np.zeros(nz, np.float32)
+    lib().orc_cc_images(nx, ny, nz, np.ascontiguousarray(proj, np.float32), np.ascontiguousarray(meas, np.float32), sx, sy)
+    return sx, sy
+
+
+def shift_images(stack, sx, sy):
+    nz, ny, nx = stack.shape
+    out = np.array(stack, np.float32, copy=True)
+    lib().orc_shift_images(nx, ny, nz, out, np.ascontiguousarray(sx, np.float32), np.ascontiguousarray(sy, np.float32))
+    return out
+
+
+def soft_noneg(x):
+    out = np.array(x, np.float32, copy=True)
+    lib().orc_soft_noneg(out.reshape(-1), out.size)
+    return out
+
+
+# ---- reference build wrappers --------------------------------------------
+
+def ref_cyl_geometry(radius, height=1):
+    R = ref()
+    n = C.c_int(0)
+    R.ref_cyl_geometry(float(radius), height, C.byref(n), None)
+    cord = np.zeros(2 * n.value, np.int32)
+    R.ref_cyl_geometry(float(radius), height, C.byref(n), cord.ctypes.data)
+    return cord.reshape(-1, 2)
+
+
+def ref_project(method, x, angles, nx, ny):
+    na = len(angles)
+    p = np.zeros((na, ny, nx), np.float32)
+    st = ref().ref_project(_m(method), 0.5 * (nx - 1), nx, ny, np.ascontiguousarray(x, np.float32),
+                           np.ascontiguousarray(angles, np.float32), na, p)
+    if st:
+        raise RuntimeError(f"ref_project status {st}")
+    return p
+
+
+def ref_backproject(method, p, angles, nx, ny):
+    na = len(angles)
+    nrays = len(ref_cyl_geometry(0.5 * (nx - 1)))
+    x = np.zeros((nrays, ny), np.float32)
+    st = ref().ref_backproject(_m(method), 0.5 * (nx - 1), nx, ny, np.ascontiguousarray(p, np.float32),
+                               np.ascontiguousarray(angles, np.float32), na, x)
+    if st:
+        raise RuntimeError(f"ref_backproject status {st}")
+    return x
+
+
+def ref_solve(solver, method, b, angles, lam, maxit, tol, x0=None):
+    na, ny, nx = b.shape
+    nrays = len(ref_cyl_geometry(0.5 * (nx - 1)))
+    x = np.zeros((nrays + 8 * nx, ny), np.float32) if x0 is None else np.array(x0, np.float32, copy=True)
+    nnz = C.c_int(0)
+    st = ref().ref_solve(1 if solver == "sirt" else 2, _m(method), np.ascontiguousarray(b, np.float32), nx, ny, na,
+                         np.ascontiguousarray(angles, np.float32), lam, maxit, tol, 0 if x0 is None else 1, x,
+                         C.byref(nnz))
+    return st, x.reshape(-1)[: nnz.value].reshape(-1, ny)
