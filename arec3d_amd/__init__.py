@@ -1,0 +1,448 @@
+"""arec3d_amd: MI355X-native reconstruction hot path of AREC3D.
+
+Python view of libarec3d_amd.so (ctypes).  Two layers, mirroring the C ABI in
+include/arec3d_amd.h:
+
+* reference-style functions on host arrays with the reference's names,
+  arguments and status codes: ``arecProject2D``, ``arecProject2D_KB``,
+  ``arecProject2D_SQ``, the matching ``arecBackProject2D*``, ``cyl_sirt*``,
+  ``cyl_cgls*`` and ``cyl_cgls*_stat`` (AREC3D/AREC3D/arecproject.h:55-62,
+  arecsirt.h, areccgls.h, areccgls_stat.h), over :class:`ArecImage`;
+* a device-resident API (:class:`Context`, :class:`Geometry`,
+  :class:`DeviceArray`) used by the solvers, tests and benchmarks.
+
+There is no CPU fallback: every call runs the HIP kernels, and importing the
+package raises if the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libarec3d_amd.so")
+
+LINEAR, KB, SQ = 1, 2, 3
+METHODS = {"linear": LINEAR, "kb": KB, "sq": SQ}
+
+
+class ArecError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__(f"arec3d_amd error {rc}: {msg}")
+        self.rc = rc
+
+
+class Hist(C.Structure):
+    _fields_ = [("iter", C.c_int32), ("event", C.c_int32), ("relnrm", C.c_double), ("rnorm", C.c_double)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m arec3d_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, dbl, flt = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_float
+    P = C.POINTER
+    sig = {
+        "arec_last_error": ([], C.c_char_p),
+        "arec_version": ([], C.c_char_p),
+        "arec_create": ([P(vp), C.c_int], C.c_int),
+        "arec_destroy": ([vp], C.c_int),
+        "arec_synchronize": ([vp], C.c_int),
+        "arec_dev_alloc": ([vp, P(vp), C.c_size_t], C.c_int),
+        "arec_dev_free": ([vp, vp], C.c_int),
+        "arec_memcpy_h2d": ([vp, vp, vp, C.c_size_t], C.c_int),
+        "arec_memcpy_d2h": ([vp, vp, vp, C.c_size_t], C.c_int),
+        "arec_memset": ([vp, vp, C.c_int, C.c_size_t], C.c_int),
+        "arec_cyl_nrays": ([dbl, vp], i64),
+        "arec_geom_create": ([vp, P(vp), C.c_int, i64, i64, dbl, vp, i64], C.c_int),
+        "arec_geom_destroy": ([vp], C.c_int),
+        "arec_geom_info": ([vp, P(i64), P(i64), P(i64)], C.c_int),
+        "arec_geom_device_bytes": ([vp], i64),
+        "arec_project": ([vp, vp, vp, vp], C.c_int),
+        "arec_backproject": ([vp, vp, vp, vp], C.c_int),
+        "arec_sirt": ([vp, vp, vp, vp, flt, i32, flt, P(Hist), i32, P(i32), P(flt)], C.c_int),
+        "arec_cgls": ([vp, vp, vp, vp, vp, i32, i32, flt, P(Hist), i32, P(i32)], C.c_int),
+        "arec_init_weights": ([vp, vp, vp, i64, i64, i64], C.c_int),
+        "arec_soft_noneg": ([vp, vp, i64], C.c_int),
+        "arec_dot": ([vp, vp, vp, i64, P(dbl)], C.c_int),
+        "arec_max": ([vp, vp, i64, P(flt)], C.c_int),
+        "arec_time_operator": ([vp, vp, C.c_int, vp, vp, i32, P(flt)], C.c_int),
+        "arec_phantom": ([vp, dbl, i64, dbl, vp, i32, vp], C.c_int),
+        "arec_add_noise": ([vp, vp, i64, C.c_uint64, i32, dbl, dbl, dbl], C.c_int),
+        "arec_h_project": ([C.c_int, dbl, i32, i32, vp, vp, i32, vp], C.c_int),
+        "arec_h_backproject": ([C.c_int, dbl, i32, i32, i32, vp, vp, i32, vp], C.c_int),
+        "arec_h_solve": ([C.c_int, C.c_int, vp, i32, i32, i32, vp, flt, i32, flt, i32, vp, P(Hist), i32, P(i32), P(flt)], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    return L
+
+
+lib = _load()
+
+EXPORTED = [
+    "arec_last_error", "arec_version", "arec_create", "arec_destroy", "arec_synchronize",
+    "arec_dev_alloc", "arec_dev_free", "arec_memcpy_h2d", "arec_memcpy_d2h", "arec_memset",
+    "arec_cyl_nrays", "arec_geom_create", "arec_geom_destroy", "arec_geom_info",
+    "arec_geom_device_bytes", "arec_project", "arec_backproject", "arec_sirt", "arec_cgls",
+    "arec_init_weights", "arec_soft_noneg", "arec_dot", "arec_max", "arec_time_operator", "arec_phantom",
+    "arec_add_noise", "arec_h_project", "arec_h_backproject",
+    "arec_h_solve",
+]
+
+
+def _check(rc):
+    if rc != 0:
+        raise ArecError(rc, lib.arec_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _method(m):
+    return METHODS[m] if isinstance(m, str) else int(m)
+
+
+def cyl_cord(radius):
+    """(ix, iz) of every voxel column of arecAllocateCylImage (arecImage.cpp:29-86)."""
+    n = lib.arec_cyl_nrays(float(radius), None)
+    cord = np.zeros(2 * n, np.int32)
+    lib.arec_cyl_nrays(float(radius), _ptr(cord))
+    return cord.reshape(n, 2)
+
+
+def cyl_nrays(radius):
+    return int(lib.arec_cyl_nrays(float(radius), None))
+
+
+# --------------------------------------------------------------------------
+# Device-resident API
+# --------------------------------------------------------------------------
+
+class Context:
+    """A device and a HIP stream (one per process and GPU is the usual use)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib.arec_create(C.byref(self.h), device))
+
+    def synchronize(self):
+        _check(lib.arec_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib.arec_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def empty(self, shape, dtype=np.float32):
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+
+class DeviceArray:
+    """A device buffer owned by the library's allocator."""
+
+    def __init__(self, ctx, shape, dtype=np.float32):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = C.c_void_p()
+        _check(lib.arec_dev_alloc(ctx.h, C.byref(self.ptr), self.nbytes))
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape))
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        _check(lib.arec_memcpy_h2d(self.ctx.h, self.ptr, _ptr(a), self.nbytes))
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib.arec_memcpy_d2h(self.ctx.h, _ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        _check(lib.arec_memset(self.ctx.h, self.ptr, 0, self.nbytes))
+        return self
+
+    def free(self):
+        if self.ptr:
+            lib.arec_dev_free(self.ctx.h, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Geometry:
+    """Projector geometry: detector nx x ny, cylinder radius, FP32 angles.
+
+    ``method`` is 'sq', 'kb' or 'linear' (reference pmethod 3, 2, 1).
+    """
+
+    def __init__(self, ctx, method, nx, ny, angles, radius=None):
+        self.ctx = ctx
+        self.method = _method(method)
+        self.nx, self.ny = int(nx), int(ny)
+        self.angles = np.ascontiguousarray(angles, np.float32)
+        self.na = len(self.angles)
+        self.radius = 0.5 * (self.nx - 1) if radius is None else float(radius)
+        self.h = C.c_void_p()
+        _check(lib.arec_geom_create(ctx.h, C.byref(self.h), self.method, self.nx, self.ny, self.radius,
+                                    _ptr(self.angles), self.na))
+        nr, nnz, npj = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib.arec_geom_info(self.h, C.byref(nr), C.byref(nnz), C.byref(npj)))
+        self.nrays, self.nnz, self.nproj = nr.value, nnz.value, npj.value
+
+    @property
+    def vol_shape(self):
+        return (self.nrays, self.ny)
+
+    @property
+    def proj_shape(self):
+        return (self.na, self.ny, self.nx)
+
+    def device_bytes(self):
+        return int(lib.arec_geom_device_bytes(self.h))
+
+    def project(self, x, p):
+        _check(lib.arec_project(self.ctx.h, self.h, x.ptr, p.ptr))
+
+    def backproject(self, p, x):
+        _check(lib.arec_backproject(self.ctx.h, self.h, p.ptr, x.ptr))
+
+    def time_operator(self, which, src, dst, reps):
+        """ms per launch of the forward (which=0) or back (1) projector."""
+        ms = C.c_float()
+        _check(lib.arec_time_operator(self.ctx.h, self.h, which, src.ptr, dst.ptr, reps, C.byref(ms)))
+        return ms.value
+
+    def sirt(self, b, x, lam, maxit, tol, hist_cap=None):
+        cap = hist_cap or (maxit + 64)
+        H = (Hist * cap)()
+        nh, lo = C.c_int32(), C.c_float()
+        _check(lib.arec_sirt(self.ctx.h, self.h, b.ptr, x.ptr, lam, maxit, tol, H, cap, C.byref(nh), C.byref(lo)))
+        return _hist(H, nh.value), lo.value
+
+    def cgls(self, b, x, maxit, tol, warm=False, w=None, hist_cap=None):
+        cap = hist_cap or (maxit + 4)
+        H = (Hist * cap)()
+        nh = C.c_int32()
+        _check(lib.arec_cgls(self.ctx.h, self.h, b.ptr, None if w is None else w.ptr, x.ptr, int(bool(warm)),
+                             maxit, tol, H, cap, C.byref(nh)))
+        return _hist(H, nh.value)
+
+    def close(self):
+        if self.h:
+            lib.arec_geom_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _hist(H, n):
+    return [(H[i].iter, H[i].event, H[i].relnrm, H[i].rnorm) for i in range(min(n, len(H)))]
+
+
+def dot(ctx, a, b):
+    out = C.c_double()
+    _check(lib.arec_dot(ctx.h, a.ptr, b.ptr, a.size, C.byref(out)))
+    return out.value
+
+
+def reduce_max(ctx, a):
+    out = C.c_float()
+    _check(lib.arec_max(ctx.h, a.ptr, a.size, C.byref(out)))
+    return out.value
+
+
+def init_weights(ctx, b, w, nx, ny, nz):
+    _check(lib.arec_init_weights(ctx.h, b.ptr, w.ptr, nx, ny, nz))
+
+
+def soft_noneg(ctx, x):
+    _check(lib.arec_soft_noneg(ctx.h, x.ptr, x.size))
+
+
+# --------------------------------------------------------------------------
+# Reference-style interface on host arrays
+# --------------------------------------------------------------------------
+
+class ArecImage:
+    """Host mirror of the reference's arecImage (arecImage.h:5-22): a cube
+    ``data[nz][ny][nx]`` or a compact cylinder ``data[nrays][height]``."""
+
+    def __init__(self, data=None, nx=0, ny=0, nz=0, radius=0.0, height=0, cord=None, is_cyl=False):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.radius, self.height = radius, height
+        self.cord = cord
+        self.is_cyl = is_cyl
+        self.data = data
+
+    @property
+    def nrays(self):
+        return 0 if self.cord is None else len(self.cord)
+
+    @property
+    def nnz(self):
+        return self.nrays * self.height
+
+
+def arecAllocateCBImage(nx, ny, nz):
+    """arecAllocateCBImage (arecImage.cpp:15-27)."""
+    return ArecImage(np.zeros((nz, ny, nx), np.float32), nx, ny, nz)
+
+
+def arecAllocateCylImage(radius, height):
+    """arecAllocateCylImage (arecImage.cpp:29-86)."""
+    cord = cyl_cord(radius)
+    n = int(round(2 * radius + 1))
+    return ArecImage(np.zeros((len(cord), height), np.float32), n, height, n, float(radius), height, cord, True)
+
+
+def _project(method, cylvol, angles, projstack):
+    if not cylvol.is_cyl:
+        return 1
+    if projstack.nx != cylvol.nx or projstack.ny != cylvol.ny:
+        return 2
+    ang = np.ascontiguousarray(angles, np.float32)
+    out = np.zeros((len(ang), cylvol.ny, cylvol.nx), np.float32)
+    x = np.ascontiguousarray(cylvol.data, np.float32)
+    rc = lib.arec_h_project(method, cylvol.radius, cylvol.nx, cylvol.ny, _ptr(x), _ptr(ang), len(ang), _ptr(out))
+    if rc < 0 and rc <= -100:
+        raise ArecError(rc, lib.arec_last_error().decode())
+    if rc == 0:
+        projstack.data[: len(ang)] = out
+    return rc
+
+
+def _backproject(method, images, angles, bvol):
+    ang = np.ascontiguousarray(angles, np.float32)
+    if images.nz != len(ang):
+        return 1
+    if bvol.nx != images.nx or bvol.ny != images.ny or not bvol.is_cyl:
+        return -1
+    out = np.zeros((bvol.nrays, images.ny), np.float32)
+    p = np.ascontiguousarray(images.data, np.float32)
+    rc = lib.arec_h_backproject(method, bvol.radius, images.nx, images.ny, images.nz, _ptr(p), _ptr(ang),
+                                len(ang), _ptr(out))
+    if rc <= -100:
+        raise ArecError(rc, lib.arec_last_error().decode())
+    if rc == 0:
+        bvol.data[...] = out
+    return rc
+
+
+def arecProject2D(cylvol, angles, nangles, projstack):
+    return _project(LINEAR, cylvol, np.asarray(angles)[:nangles], projstack)
+
+
+def arecProject2D_KB(cylvol, angles, nangles, projstack):
+    return _project(KB, cylvol, np.asarray(angles)[:nangles], projstack)
+
+
+def arecProject2D_SQ(cylvol, angles, nangles, projstack):
+    return _project(SQ, cylvol, np.asarray(angles)[:nangles], projstack)
+
+
+def arecBackProject2D(images, angles, nangles, bvol):
+    return _backproject(LINEAR, images, np.asarray(angles)[:nangles], bvol)
+
+
+def arecBackProject2D_KB(images, angles, nangles, bvol):
+    return _backproject(KB, images, np.asarray(angles)[:nangles], bvol)
+
+
+def arecBackProject2D_SQ(images, angles, nangles, bvol):
+    return _backproject(SQ, images, np.asarray(angles)[:nangles], bvol)
+
+
+def _solve(solver, method, images, angles, xvol, lam, maxit, tol):
+    """Shared body of cyl_sirt*/cyl_cgls*: returns the status; fills xvol
+    (allocating it like the reference when xvol.data is None) and stores the
+    printed history in xvol.history."""
+    nx, ny, nz = images.nx, images.ny, images.nz
+    radius = 0.5 * (nx - 1)
+    if (solver == 2 and method == KB) or solver == 3:
+        radius = float(nx // 2)
+    warm = xvol.data is not None and solver != 1
+    if xvol.data is None:
+        fresh = arecAllocateCylImage(radius, ny)
+        xvol.__dict__.update(fresh.__dict__)
+    b = np.ascontiguousarray(images.data, np.float32)
+    ang = np.ascontiguousarray(angles, np.float32)[:nz]
+    x = np.ascontiguousarray(xvol.data, np.float32).copy()
+    cap = maxit * 22 + 64
+    H = (Hist * cap)()
+    nh, lo = C.c_int32(), C.c_float()
+    rc = lib.arec_h_solve(solver, method, _ptr(b), nx, ny, nz, _ptr(ang), lam, maxit, tol, int(warm), _ptr(x),
+                          H, cap, C.byref(nh), C.byref(lo))
+    if rc <= -100:
+        raise ArecError(rc, lib.arec_last_error().decode())
+    if rc == 0:
+        xvol.data[...] = x
+    xvol.history = _hist(H, nh.value)
+    xvol.lam = lo.value
+    return rc
+
+
+def cyl_sirt(comm, images, angles, xvol, lam, maxit, tol):
+    return _solve(1, LINEAR, images, angles, xvol, lam, maxit, tol)
+
+
+def cyl_sirt_KB(comm, images, angles, xvol, lam, maxit, tol):
+    return _solve(1, KB, images, angles, xvol, lam, maxit, tol)
+
+
+def cyl_sirt_SQ(comm, images, angles, xvol, lam, maxit, tol):
+    return _solve(1, SQ, images, angles, xvol, lam, maxit, tol)
+
+
+def cyl_cgls(comm, images, angles, xvol, lam, maxit, tol):
+    return _solve(2, LINEAR, images, angles, xvol, lam, maxit, tol)
+
+
+def cyl_cgls_KB(comm, images, angles, xvol, lam, maxit, tol):
+    return _solve(2, KB, images, angles, xvol, lam, maxit, tol)
+
+
+def cyl_cgls_SQ(comm, images, angles, xvol, lam, maxit, tol):
+    return _solve(2, SQ, images, angles, xvol, lam, maxit, tol)
+
+
+def cyl_cgls_stat(comm, images, angles, xvol, lam, maxit, tol):
+    return _solve(3, LINEAR, images, angles, xvol, lam, maxit, tol)
+
+
+def cyl_cgls_KB_stat(comm, images, angles, xvol, lam, maxit, tol):
+    return _solve(3, KB, images, angles, xvol, lam, maxit, tol)
+
+
+def cyl_cgls_SQ_stat(comm, images, angles, xvol, lam, maxit, tol):
+    return _solve(3, SQ, images, angles, xvol, lam, maxit, tol)

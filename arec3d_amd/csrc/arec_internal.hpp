@@ -1,0 +1,109 @@
+// Internal declarations shared by the translation units of libarec3d_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "arec3d_amd.h"
+
+namespace arec {
+
+// Projector methods, numbered like the reference's "pmethod" input key
+// (AREC3D/AREC3D/AREC3D.cpp:145-169).
+enum Method { kLinear = 1, kKB = 2, kSQ = 3 };
+
+// Per-angle constants.  c and s are the FP32 cos/sin of the FP32 angle,
+// widened to FP64: the reference evaluates cos(angles[ia]) on a float
+// argument, which selects the float overload (arecproject.cpp:71-72,
+// 393-394, 599-600).  a, b and lmax are sqpoints (arecproject.h:13-25),
+// computed from FP64 cos/sin because sqpoints takes a double.
+struct AngleConst {
+    double c, s;
+    double a, b, lmax;
+};
+
+// Columns of one back-projection tile: up to 64 voxel columns whose
+// coordinates lie in one 8x8 box.
+constexpr int kTileCols = 64;
+constexpr int kTileBox = 8;
+
+// Entry of the forward table: voxel columns of one angle in order of their
+// first detector tap.  x = column index (low 22 bits) | increase of the first
+// tap over the previous entry (high 10 bits); y,z,w = tap weights.
+constexpr int kColBits = 22;
+constexpr int kMaxCols = 1 << kColBits;
+constexpr int kMaxTapStep = 1023;
+constexpr int kTapBase = -4;  // first tap of the virtual entry before entry 0
+
+struct Geometry {
+    int method = kSQ;
+    int nx = 0, ny = 0, na = 0;  // detector width, rows (height), angles
+    double radius = 0;
+    int ncyl = 0;     // cylinder width round(2r+1)
+    int nrays = 0;    // voxel columns
+    int nvalid = 0;   // columns that contribute (linear: nrays-2)
+    int xcent = 0;    // linear pair's integer centre (int)r + 1
+    int64_t nnz = 0, nproj = 0;
+    std::vector<int> cord;        // (ix, iz) per column, iz-major
+    std::vector<int> coord;       // (X, Z) used by the weight function
+    std::vector<float> angles;
+    std::vector<AngleConst> ang;
+    int ntiles = 0;
+    std::vector<int> tile_cols;   // ntiles * 64, -1 = empty
+    std::vector<int> tile_org;    // ntiles * 2 box origin
+    // device copies
+    AngleConst *d_ang = nullptr;
+    int *d_coord = nullptr;
+    int *d_tile_cols = nullptr;
+    int *d_tile_org = nullptr;
+    uint4 *d_fwd = nullptr;       // na * nvalid entries
+    int *d_err = nullptr;         // geometry self-check flag
+    bool device_ready = false;
+};
+
+// Host geometry (geometry.cpp).
+int cyl_mask(double radius, std::vector<int> *cord);
+int geometry_build_host(Geometry &g, int method, int nx, int ny, double radius,
+                        const float *angles, int na, std::string *err);
+// First detector tap of column j at angle a, the same arithmetic as the
+// device weight function (used to order the forward table).
+int host_first_tap(const Geometry &g, int a, int j);
+void geometry_sort_angles(const Geometry &g, int a0, int a1, std::vector<int> &order);
+
+// Device setup and operators (projector.hip).
+int geometry_upload(Geometry &g, hipStream_t st, std::string *err);
+void geometry_free_device(Geometry &g);
+// Modes of the back projector's epilogue.
+enum BpMode { kBpStore = 0, kBpSirtGrad = 1, kBpStoreNorm = 2 };
+int launch_forward(const Geometry &g, const float *x, float *p, double *partials, hipStream_t st,
+                   const int *skip);
+int launch_backward(const Geometry &g, const float *p, float *x, int mode, const float *bvec,
+                    double *partials, hipStream_t st, const int *skip);
+int64_t forward_partial_count(const Geometry &g);
+int64_t backward_partial_count(const Geometry &g);
+
+// Fill with a constant by a kernel (graph-capture safe); skip as above.
+int fill(float *x, float v, int64_t n, hipStream_t st, const int *skip);
+
+// Error plumbing (capi.cpp).
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+#define AREC_HIP(call)                                   \
+    do {                                                 \
+        hipError_t e__ = (call);                         \
+        if (e__ != hipSuccess) return arec::hip_fail(e__, #call); \
+    } while (0)
+
+}  // namespace arec
+
+struct arec_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct arec_geom {
+    arec::Geometry g;
+};

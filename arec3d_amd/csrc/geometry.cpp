@@ -1,0 +1,164 @@
+// Host-side geometry: cylinder mask, per-angle constants, back-projection
+// tiles and the ordering of the forward table.  Compiled with
+// -ffp-contract=off so that every FP64 product and sum here is a separate
+// rounding, exactly like the device weight function that uses the same
+// formulas.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <thread>
+
+#include "arec_internal.hpp"
+
+namespace arec {
+
+// arecAllocateCylImage, AREC3D/AREC3D/arecImage.cpp:29-86.  The mask keeps
+// (ix,iz) with (int)(ix-r)^2 + (int)(iz-r)^2 <= (int)(r^2), every conversion
+// truncating toward zero, rays listed iz-major.
+int cyl_mask(double radius, std::vector<int> *cord) {
+    int n = (int)std::lround(2 * radius + 1);
+    int r2 = (int)(radius * radius);
+    int nrays = 0;
+    if (cord) cord->clear();
+    for (int iz = 0; iz < n; iz++) {
+        int zs = (int)(iz - radius);
+        for (int ix = 0; ix < n; ix++) {
+            int xs = (int)(ix - radius);
+            if (zs * zs + xs * xs <= r2) {
+                if (cord) {
+                    cord->push_back(ix);
+                    cord->push_back(iz);
+                }
+                nrays++;
+            }
+        }
+    }
+    return nrays;
+}
+
+static AngleConst angle_const(float angle) {
+    AngleConst a;
+    a.c = (double)cosf(angle);
+    a.s = (double)sinf(angle);
+    double cd = std::cos((double)angle), sd = std::sin((double)angle);
+    double cms = std::fabs(0.5 * (cd - sd));
+    double cps = std::fabs(0.5 * (cd + sd));
+    a.a = std::min(cms, cps);
+    a.b = std::max(cms, cps) + 1e-6;
+    a.lmax = 1 / ((a.b - a.a) + 2 * a.a);
+    return a;
+}
+
+int geometry_build_host(Geometry &g, int method, int nx, int ny, double radius,
+                        const float *angles, int na, std::string *err) {
+    if (method < 1 || method > 3) {
+        *err = "method must be 1 (linear), 2 (KB) or 3 (SQ)";
+        return AREC_EINVAL;
+    }
+    if (nx <= 0 || ny <= 0 || na <= 0 || !(radius > 0)) {
+        *err = "sizes and radius must be positive";
+        return AREC_EINVAL;
+    }
+    g.method = method;
+    g.nx = nx;
+    g.ny = ny;
+    g.na = na;
+    g.radius = radius;
+    g.ncyl = (int)std::lround(2 * radius + 1);
+    g.nrays = cyl_mask(radius, &g.cord);
+    g.nnz = (int64_t)g.nrays * ny;
+    g.nproj = (int64_t)nx * ny * na;
+    g.xcent = (int)radius + 1;
+    if (g.nrays >= kMaxCols) {
+        *err = "more voxel columns than the forward table encodes (2^22)";
+        return AREC_ESIZE;
+    }
+    g.angles.assign(angles, angles + na);
+    g.ang.resize(na);
+    for (int a = 0; a < na; a++) g.ang[a] = angle_const(angles[a]);
+
+    // Coordinates fed to the weight function.  SQ and KB use the column's
+    // own (ix, iz).  The linear pair indexes the 0-based cord table with a
+    // 1-based loop (arecproject.cpp:15,75-77), so column j is placed at
+    // X = iz of column j+1, Z = ix of column j+2; the last two columns would
+    // read past the table and contribute nothing here.
+    g.coord.assign(2 * (size_t)g.nrays, 0);
+    g.nvalid = g.nrays;
+    if (method == kLinear) {
+        g.nvalid = std::max(0, g.nrays - 2);
+        for (int j = 0; j < g.nvalid; j++) {
+            g.coord[2 * j] = g.cord[2 * j + 3];
+            g.coord[2 * j + 1] = g.cord[2 * j + 4];
+        }
+    } else {
+        g.coord = g.cord;
+    }
+
+    // Back-projection tiles: columns bucketed by 8x8 boxes of their
+    // coordinates, buckets split into tiles of at most 64 columns.
+    std::map<std::pair<int, int>, std::vector<int>> boxes;
+    for (int j = 0; j < g.nvalid; j++)
+        boxes[{g.coord[2 * j + 1] / kTileBox, g.coord[2 * j] / kTileBox}].push_back(j);
+    g.tile_cols.clear();
+    g.tile_org.clear();
+    for (auto &kv : boxes) {
+        const std::vector<int> &cols = kv.second;
+        for (size_t s = 0; s < cols.size(); s += kTileCols) {
+            for (int c = 0; c < kTileCols; c++)
+                g.tile_cols.push_back(s + c < cols.size() ? cols[s + c] : -1);
+            g.tile_org.push_back(kv.first.second * kTileBox);
+            g.tile_org.push_back(kv.first.first * kTileBox);
+        }
+    }
+    g.ntiles = (int)(g.tile_org.size() / 2);
+    return 0;
+}
+
+// Same arithmetic as tap_weights<> in projector.hip (first tap only).
+int host_first_tap(const Geometry &g, int a, int j) {
+    const AngleConst &A = g.ang[a];
+    int X = g.coord[2 * j], Z = g.coord[2 * j + 1];
+    if (g.method == kLinear) {
+        int xc = X - g.xcent, zc = Z - g.xcent;
+        double t1 = (double)xc * A.c;
+        double t2 = (double)zc * A.s;
+        float xb = (float)((t1 + t2) + (double)g.xcent);
+        return (int)std::floor(xb) - 1;
+    }
+    double xc = (double)X - g.radius, zc = (double)Z - g.radius;
+    double t1 = xc * A.c;
+    double t2 = zc * A.s;
+    double xb = (t1 + t2) + g.radius;
+    return (int)std::nearbyint(xb) - 1;
+}
+
+// Stable counting sort of the contributing columns of angles [a0, a1) by
+// their first tap; order receives (a1-a0)*nvalid column indices.
+void geometry_sort_angles(const Geometry &g, int a0, int a1, std::vector<int> &order) {
+    const int nv = g.nvalid;
+    order.resize((size_t)(a1 - a0) * nv);
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    nth = std::min<unsigned>(nth, (unsigned)(a1 - a0));
+    auto work = [&](unsigned t) {
+        std::vector<int> key(nv);
+        std::vector<int> count;
+        for (int a = a0 + (int)t; a < a1; a += (int)nth) {
+            int kmin = 1 << 30, kmax = -(1 << 30);
+            for (int j = 0; j < nv; j++) {
+                key[j] = host_first_tap(g, a, j);
+                kmin = std::min(kmin, key[j]);
+                kmax = std::max(kmax, key[j]);
+            }
+            count.assign((size_t)(kmax - kmin + 2), 0);
+            for (int j = 0; j < nv; j++) count[key[j] - kmin + 1]++;
+            for (size_t k = 1; k < count.size(); k++) count[k] += count[k - 1];
+            int *out = order.data() + (size_t)(a - a0) * nv;
+            for (int j = 0; j < nv; j++) out[count[key[j] - kmin]++] = j;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; t++) pool.emplace_back(work, t);
+    for (auto &th : pool) th.join();
+}
+
+}  // namespace arec

@@ -1,0 +1,542 @@
+// Parallel-beam projector pair over the compact cylinder, gfx950.
+//
+// Geometry (AREC3D/AREC3D/arecproject.cpp): every row y of the volume is an
+// independent 2-D problem with the same weights, so one operator application
+// is a sparse matrix with 3 taps per (angle, voxel column) applied to the ny
+// columns of the volume.  Both kernels call tap_weights<>() below, so the
+// forward and back projections use bit-identical weights and are exact
+// adjoints up to FP32 summation order.
+//
+//  * Forward (ray-driven gather, no atomics): one wavefront owns one angle and
+//    a chunk of 64*RPL detector rows (lanes = rows).  It walks that angle's
+//    voxel columns in order of their first detector tap (the forward table),
+//    reads each column's rows as one coalesced 256-byte segment and keeps the
+//    three detector pixels the column can touch in registers.  A pixel is
+//    complete once the walk has passed it and is stored then; every output
+//    pixel is written exactly once.
+//  * Back (voxel-driven gather, no atomics): one workgroup owns a tile of 64
+//    voxel columns inside an 8x8 box times 64 rows.  Per block of angles it
+//    stages the tile's detector footprint (<= 20 pixels x 64 rows per angle)
+//    in LDS, computes the 64 columns' taps once for the 4 wavefronts, and
+//    accumulates 16 rows per thread in registers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "arec_internal.hpp"
+
+namespace arec {
+
+__constant__ float c_kb_lut[1000] = {
+#include "kb_lut_f.inc"
+};
+
+struct Taps {
+    int p0;
+    float w0, w1, w2;
+};
+
+// piece_wise_integrated (arecproject.h:27-53), the CDF of the unit-area
+// trapezoid; the plateau formula applies on |x| < a + 1e-3.
+__device__ __forceinline__ double trap_cdf(double x, double a, double b, double ym) {
+#pragma clang fp contract(off)
+    if (x < -b) return 0.0;
+    if (x > b) return 1.0;
+    double a_eps = a + 1e-3;
+    if (x > -a_eps && x < a_eps) return 0.5 * ym * (b + a) + ym * x;
+    if (x < -a) return 0.5 * ym * (x + b) * (x + b) / (b - a);
+    if (x > a) return 0.5 * ym * (b - x) * (b - x) / (a - b) + 1.0;
+    return 0.0;
+}
+
+// interp1_R_MOD (arecproject.cpp:340-346).
+__device__ __forceinline__ float kb_interp(float x) {
+#pragma clang fp contract(off)
+    const float DR = (float)(3.0 / 999);
+    x = fabsf(x);
+    x = x / DR;
+    int ind = (int)x;
+    x = x - (float)ind;
+    float second = c_kb_lut[ind + 1] * x;
+    return (float)((double)c_kb_lut[ind] * (1.0 - (double)x) + (double)second);
+}
+
+// Detector taps of the column at coordinates (X, Z) for one angle.
+//  SQ  (arecproject.cpp:603-620): xb = (X-r)c + (Z-r)s + r in FP64,
+//      iqx = round-half-even(xb), taps iqx-1..iqx+1 weighted by differences
+//      of the trapezoid CDF.
+//  KB  (:396-403): same xb and iqx, LUT weights at (float)(xb - tap).
+//  linear (:73-80): xb rounded to FP32 around the integer centre (int)r+1,
+//      taps floor(xb)-1 and floor(xb) (0-based), third weight 0.
+template <int M>
+__device__ __forceinline__ Taps tap_weights(int X, int Z, const AngleConst &A, double radius,
+                                            int xcent) {
+#pragma clang fp contract(off)
+    Taps t;
+    if (M == kLinear) {
+        double t1 = (double)(X - xcent) * A.c;
+        double t2 = (double)(Z - xcent) * A.s;
+        float xb = (float)((t1 + t2) + (double)xcent);
+        int iqx = (int)floorf(xb);
+        float dipx = xb - (float)iqx;
+        t.p0 = iqx - 1;
+        t.w0 = (float)(1.0 - (double)dipx);
+        t.w1 = dipx;
+        t.w2 = 0.0f;
+        return t;
+    }
+    double t1 = ((double)X - radius) * A.c;
+    double t2 = ((double)Z - radius) * A.s;
+    double xb = (t1 + t2) + radius;
+    int iqx = __double2int_rn(xb);
+    t.p0 = iqx - 1;
+    if (M == kKB) {
+        t.w0 = kb_interp((float)(xb - (double)(iqx - 1)));
+        t.w1 = kb_interp((float)(xb - (double)iqx));
+        t.w2 = kb_interp((float)(xb - (double)(iqx + 1)));
+        return t;
+    }
+    double xv = (double)iqx - xb;
+    double v0 = trap_cdf(xv - 1.5, A.a, A.b, A.lmax);
+    double v1 = trap_cdf(xv - 0.5, A.a, A.b, A.lmax);
+    double v2 = trap_cdf(xv + 0.5, A.a, A.b, A.lmax);
+    double v3 = trap_cdf(xv + 1.5, A.a, A.b, A.lmax);
+    t.w0 = (float)(v1 - v0);
+    t.w1 = (float)(v2 - v1);
+    t.w2 = (float)(v3 - v2);
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// Forward table: entries of angle a in order of first tap.
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ void build_fwd_table(const int *__restrict__ order, int a0, int na_batch, int nvalid,
+                                const int *__restrict__ coord, const AngleConst *__restrict__ ang,
+                                double radius, int xcent, uint4 *__restrict__ table,
+                                int *__restrict__ err) {
+    int ab = blockIdx.y;
+    if (ab >= na_batch) return;
+    int a = a0 + ab;
+    AngleConst A = ang[a];
+    const int *ord = order + (size_t)ab * nvalid;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nvalid; i += gridDim.x * blockDim.x) {
+        int j = ord[i];
+        Taps t = tap_weights<M>(coord[2 * j], coord[2 * j + 1], A, radius, xcent);
+        int prev = kTapBase;
+        if (i > 0) {
+            int jp = ord[i - 1];
+            prev = tap_weights<M>(coord[2 * jp], coord[2 * jp + 1], A, radius, xcent).p0;
+        }
+        int step = t.p0 - prev;
+        if (step < 0 || step > kMaxTapStep) {
+            atomicOr(err, 1);
+            step = 0;
+        }
+        uint4 e;
+        e.x = (unsigned)j | ((unsigned)step << kColBits);
+        e.y = __float_as_uint(t.w0);
+        e.z = __float_as_uint(t.w1);
+        e.w = __float_as_uint(t.w2);
+        table[(size_t)a * nvalid + i] = e;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward projector
+// ---------------------------------------------------------------------------
+constexpr int kFwdWaves = 4;  // angles per workgroup
+
+// Store pixel q of the wave's rows (q outside the detector is dropped).
+template <int RPL>
+__device__ __forceinline__ void fwd_store(float *__restrict__ pa, const float *v, int q, int nx,
+                                          int ny, int y0, double &sq) {
+    if (q < 0 || q >= nx) return;
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+        int y = y0 + r;
+        if (y < ny) {
+            pa[(size_t)y * nx + q] = v[r];
+            float s = v[r] * v[r];
+            sq += (double)s;
+        }
+    }
+}
+
+// p[a][y][:] for the wave's angle and rows.  partials (optional) receives one
+// FP64 sum of FP32 squares of the written values per wave.
+template <int RPL>
+__global__ __launch_bounds__(64 * kFwdWaves) void forward_kernel(
+    const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ table, int nvalid,
+    int nx, int ny, int na, int nchunks, double *__restrict__ partials, const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    // Workgroups with consecutive ids run on different XCDs; give each XCD a
+    // contiguous range of (row chunk, angle block) pairs, angle fastest, so
+    // that the workgroups resident on one XCD read the same rows of nearly
+    // the same columns (neighbouring angles sweep the disk almost together)
+    // and share its L2.
+    const int nblk = gridDim.x;
+    int b = blockIdx.x;
+    if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+    const int nablk = nblk / nchunks;
+    const int chunk = b / nablk;
+    const int a = (b % nablk) * kFwdWaves + wave;
+    if (a >= na) return;
+    const int y0 = chunk * 64 * RPL + lane * RPL;
+    float *__restrict__ pa = p + (size_t)a * nx * ny;
+    const uint4 *__restrict__ tab = table + (size_t)a * nvalid;
+
+    float acc0[RPL], acc1[RPL], acc2[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; r++) acc0[r] = acc1[r] = acc2[r] = 0.0f;
+    double sq = 0.0;
+    int base = kTapBase;  // window covers pixels base, base+1, base+2
+
+    for (int i = 0; i < nvalid; i++) {
+        uint4 e = tab[i];  // wave-uniform
+        int j = (int)(e.x & (kMaxCols - 1));
+        int step = (int)(e.x >> kColBits);
+        if (step) {
+            // pixels base .. base+step-1 are complete
+            if (step >= 3) {
+                fwd_store<RPL>(pa, acc0, base, nx, ny, y0, sq);
+                fwd_store<RPL>(pa, acc1, base + 1, nx, ny, y0, sq);
+                fwd_store<RPL>(pa, acc2, base + 2, nx, ny, y0, sq);
+                float z[RPL];
+#pragma unroll
+                for (int r = 0; r < RPL; r++) z[r] = 0.0f;
+                for (int q = base + 3; q < base + step; q++) fwd_store<RPL>(pa, z, q, nx, ny, y0, sq);
+#pragma unroll
+                for (int r = 0; r < RPL; r++) acc0[r] = acc1[r] = acc2[r] = 0.0f;
+            } else if (step == 1) {
+                fwd_store<RPL>(pa, acc0, base, nx, ny, y0, sq);
+#pragma unroll
+                for (int r = 0; r < RPL; r++) {
+                    acc0[r] = acc1[r];
+                    acc1[r] = acc2[r];
+                    acc2[r] = 0.0f;
+                }
+            } else {
+                fwd_store<RPL>(pa, acc0, base, nx, ny, y0, sq);
+                fwd_store<RPL>(pa, acc1, base + 1, nx, ny, y0, sq);
+#pragma unroll
+                for (int r = 0; r < RPL; r++) {
+                    acc0[r] = acc2[r];
+                    acc1[r] = 0.0f;
+                    acc2[r] = 0.0f;
+                }
+            }
+            base += step;
+        }
+        const float w0 = __uint_as_float(e.y), w1 = __uint_as_float(e.z), w2 = __uint_as_float(e.w);
+        const float *__restrict__ xj = x + (size_t)j * ny + y0;
+        float v[RPL];
+        if (y0 + RPL <= ny) {
+#pragma unroll
+            for (int r = 0; r < RPL; r++) v[r] = xj[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < RPL; r++) v[r] = (y0 + r < ny) ? xj[r] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            acc0[r] = fmaf(w0, v[r], acc0[r]);
+            acc1[r] = fmaf(w1, v[r], acc1[r]);
+            acc2[r] = fmaf(w2, v[r], acc2[r]);
+        }
+    }
+    fwd_store<RPL>(pa, acc0, base, nx, ny, y0, sq);
+    fwd_store<RPL>(pa, acc1, base + 1, nx, ny, y0, sq);
+    fwd_store<RPL>(pa, acc2, base + 2, nx, ny, y0, sq);
+    {
+        float z[RPL];
+#pragma unroll
+        for (int r = 0; r < RPL; r++) z[r] = 0.0f;
+        for (int q = std::max(base + 3, 0); q < nx; q++) fwd_store<RPL>(pa, z, q, nx, ny, y0, sq);
+    }
+    if (partials) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+        if (lane == 0) partials[(size_t)a * nchunks + chunk] = sq;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Back projector
+// ---------------------------------------------------------------------------
+constexpr int kBpThreads = 256;
+constexpr int kBpRows = 64;        // rows per workgroup
+constexpr int kBpRowsPerThr = 16;  // rows per thread (4 row groups)
+constexpr int kBpAngBlk = 8;       // angles staged per LDS pass
+constexpr int kBpWin = 20;         // staged pixels per (row, angle)
+
+template <int M>
+__device__ __forceinline__ double corner_xb(int X, int Z, const AngleConst &A, double radius,
+                                            int xcent) {
+#pragma clang fp contract(off)
+    if (M == kLinear) return ((double)(X - xcent) * A.c + (double)(Z - xcent) * A.s) + (double)xcent;
+    return (((double)X - radius) * A.c + ((double)Z - radius) * A.s) + radius;
+}
+
+// x[j][y] = sum_a sum_k w_k(a,j) p[a][y][p0+k].  Epilogue by mode:
+//  kBpStore      out = acc
+//  kBpSirtGrad   out = bvec - acc (FP32), partial += sum out*(double)out
+//  kBpStoreNorm  out = acc, partial += sum (double)(acc*acc)
+template <int M, int MODE>
+__global__ __launch_bounds__(kBpThreads) void backward_kernel(
+    const float *__restrict__ p, float *__restrict__ out, const float *__restrict__ bvec,
+    const int *__restrict__ tile_cols, const int *__restrict__ tile_org, const int *__restrict__ coord,
+    const AngleConst *__restrict__ ang, double radius, int xcent, int nx, int ny, int na, int nrowblk,
+    double *__restrict__ partials, int *__restrict__ err, const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ float s_p[kBpAngBlk][kBpRows][kBpWin];
+    __shared__ int s_p0[kBpAngBlk][kTileCols];
+    __shared__ float s_w[kBpAngBlk][kTileCols][3];
+    __shared__ int s_u0[kBpAngBlk];
+    __shared__ double s_red[kBpThreads / 64];
+
+    const int tid = threadIdx.x;
+    const int col = tid & 63, rg = tid >> 6;
+    const int tile = blockIdx.x / nrowblk;
+    const int rb = blockIdx.x % nrowblk;
+    const int ybase = rb * kBpRows;
+    const int j = tile_cols[tile * kTileCols + col];
+    const int X0 = tile_org[2 * tile], Z0 = tile_org[2 * tile + 1];
+    int cX = 0, cZ = 0;
+    if (j >= 0) {
+        cX = coord[2 * j];
+        cZ = coord[2 * j + 1];
+    }
+    float acc[kBpRowsPerThr];
+#pragma unroll
+    for (int r = 0; r < kBpRowsPerThr; r++) acc[r] = 0.0f;
+
+    for (int a0 = 0; a0 < na; a0 += kBpAngBlk) {
+        const int nb = min(kBpAngBlk, na - a0);
+        // footprint window of the box for each staged angle
+        if (tid < nb) {
+            AngleConst A = ang[a0 + tid];
+            double lo = 1e300, hi = -1e300;
+#pragma unroll
+            for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+                for (int cx = 0; cx < 2; cx++) {
+                    double v = corner_xb<M>(X0 + cx * (kTileBox - 1), Z0 + cz * (kTileBox - 1), A,
+                                            radius, xcent);
+                    lo = fmin(lo, v);
+                    hi = fmax(hi, v);
+                }
+            int u0 = (int)floor(lo) - 2;
+            if ((int)floor(hi) + 3 - u0 >= kBpWin) atomicOr(err, 2);
+            s_u0[tid] = u0;
+        }
+        // taps of the 64 columns for the staged angles
+        for (int e = tid; e < nb * kTileCols; e += kBpThreads) {
+            int ab = e / kTileCols, c = e % kTileCols;
+            int jj = tile_cols[tile * kTileCols + c];
+            Taps t = {0, 0.f, 0.f, 0.f};
+            if (jj >= 0) t = tap_weights<M>(coord[2 * jj], coord[2 * jj + 1], ang[a0 + ab], radius, xcent);
+            s_p0[ab][c] = t.p0;
+            s_w[ab][c][0] = t.w0;
+            s_w[ab][c][1] = t.w1;
+            s_w[ab][c][2] = t.w2;
+        }
+        __syncthreads();
+        // stage the footprints
+        for (int e = tid; e < nb * kBpRows * kBpWin; e += kBpThreads) {
+            int ab = e / (kBpRows * kBpWin);
+            int rem = e % (kBpRows * kBpWin);
+            int row = rem / kBpWin, w = rem % kBpWin;
+            int y = ybase + row, u = s_u0[ab] + w;
+            float v = 0.0f;
+            if (y < ny && u >= 0 && u < nx) v = p[((size_t)(a0 + ab) * ny + y) * nx + u];
+            s_p[ab][row][w] = v;
+        }
+        __syncthreads();
+        if (j >= 0) {
+            for (int ab = 0; ab < nb; ab++) {
+                int o = s_p0[ab][col] - s_u0[ab];
+                if (o < 0 || o > kBpWin - 3) {  // outside the staged window
+                    atomicOr(err, 4);
+                    continue;
+                }
+                float w0 = s_w[ab][col][0], w1 = s_w[ab][col][1], w2 = s_w[ab][col][2];
+                const float *rowp = &s_p[ab][rg * kBpRowsPerThr][o];
+#pragma unroll
+                for (int r = 0; r < kBpRowsPerThr; r++) {
+                    const float *q = rowp + r * kBpWin;
+                    float v = acc[r];
+                    v = fmaf(w0, q[0], v);
+                    v = fmaf(w1, q[1], v);
+                    v = fmaf(w2, q[2], v);
+                    acc[r] = v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    double part = 0.0;
+    if (j >= 0) {
+        float *o = out + (size_t)j * ny;
+#pragma unroll
+        for (int r = 0; r < kBpRowsPerThr; r++) {
+            int y = ybase + rg * kBpRowsPerThr + r;
+            if (y < ny) {
+                float v = acc[r];
+                if (MODE == kBpSirtGrad) {
+                    v = bvec[(size_t)j * ny + y] - v;
+                    part += (double)v * (double)v;
+                } else if (MODE == kBpStoreNorm) {
+                    float s = v * v;
+                    part += (double)s;
+                }
+                o[y] = v;
+            }
+        }
+    }
+    if (MODE != kBpStore) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        if ((tid & 63) == 0) s_red[rg] = part;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < kBpThreads / 64; w++) s += s_red[w];
+            partials[blockIdx.x] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static int fwd_rpl(int ny) { return ny >= 512 ? 2 : 1; }
+
+int64_t forward_partial_count(const Geometry &g) {
+    int rpl = fwd_rpl(g.ny);
+    int nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
+    return (int64_t)g.na * nchunks;
+}
+
+int64_t backward_partial_count(const Geometry &g) {
+    int nrowblk = (g.ny + kBpRows - 1) / kBpRows;
+    return (int64_t)g.ntiles * nrowblk;
+}
+
+template <typename T>
+static int upload(T **dst, const std::vector<T> &src, hipStream_t st) {
+    AREC_HIP(hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(1, src.size())));
+    if (!src.empty())
+        AREC_HIP(hipMemcpyAsync(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, st));
+    return 0;
+}
+
+void geometry_free_device(Geometry &g) {
+    hipFree(g.d_ang);
+    hipFree(g.d_coord);
+    hipFree(g.d_tile_cols);
+    hipFree(g.d_tile_org);
+    hipFree(g.d_fwd);
+    hipFree(g.d_err);
+    g.d_ang = nullptr;
+    g.d_coord = nullptr;
+    g.d_tile_cols = nullptr;
+    g.d_tile_org = nullptr;
+    g.d_fwd = nullptr;
+    g.d_err = nullptr;
+    g.device_ready = false;
+}
+
+int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
+    int rc;
+    if ((rc = upload(&g.d_ang, g.ang, st))) return rc;
+    if ((rc = upload(&g.d_coord, g.coord, st))) return rc;
+    if ((rc = upload(&g.d_tile_cols, g.tile_cols, st))) return rc;
+    if ((rc = upload(&g.d_tile_org, g.tile_org, st))) return rc;
+    AREC_HIP(hipMalloc((void **)&g.d_err, sizeof(int)));
+    AREC_HIP(hipMemsetAsync(g.d_err, 0, sizeof(int), st));
+    size_t entries = (size_t)g.na * std::max(1, g.nvalid);
+    AREC_HIP(hipMalloc((void **)&g.d_fwd, sizeof(uint4) * entries));
+    // Build the forward table in batches of angles: order on the host (stable
+    // counting sort by first tap), weights on the device.
+    int batch = (int)std::max<int64_t>(1, std::min<int64_t>(g.na, (int64_t)(64 << 20) / std::max(1, g.nvalid)));
+    std::vector<int> order;
+    int *d_order = nullptr;
+    AREC_HIP(hipMalloc((void **)&d_order, sizeof(int) * (size_t)batch * std::max(1, g.nvalid)));
+    for (int a0 = 0; a0 < g.na && g.nvalid > 0; a0 += batch) {
+        int a1 = std::min(g.na, a0 + batch);
+        geometry_sort_angles(g, a0, a1, order);
+        AREC_HIP(hipStreamSynchronize(st));  // previous batch done with d_order
+        AREC_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, st));
+        dim3 grid((g.nvalid + 255) / 256, a1 - a0);
+        if (grid.x > 1024) grid.x = 1024;
+        if (g.method == kSQ)
+            build_fwd_table<kSQ><<<grid, 256, 0, st>>>(d_order, a0, a1 - a0, g.nvalid, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_fwd, g.d_err);
+        else if (g.method == kKB)
+            build_fwd_table<kKB><<<grid, 256, 0, st>>>(d_order, a0, a1 - a0, g.nvalid, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_fwd, g.d_err);
+        else
+            build_fwd_table<kLinear><<<grid, 256, 0, st>>>(d_order, a0, a1 - a0, g.nvalid, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_fwd, g.d_err);
+        AREC_HIP(hipGetLastError());
+    }
+    AREC_HIP(hipStreamSynchronize(st));
+    hipFree(d_order);
+    int herr = 0;
+    AREC_HIP(hipMemcpy(&herr, g.d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) {
+        *err = "forward table: device taps disagree with the host ordering";
+        return AREC_EGEOM;
+    }
+    g.device_ready = true;
+    return 0;
+}
+
+int launch_forward(const Geometry &g, const float *x, float *p, double *partials, hipStream_t st,
+                   const int *skip) {
+    if (g.nvalid == 0) return fill(p, 0.0f, g.nproj, st, skip);
+    int rpl = fwd_rpl(g.ny);
+    int nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
+    int ablk = (g.na + kFwdWaves - 1) / kFwdWaves;
+    dim3 grid(ablk * nchunks);
+    if (rpl == 2)
+        forward_kernel<2><<<grid, 64 * kFwdWaves, 0, st>>>(x, p, g.d_fwd, g.nvalid, g.nx, g.ny, g.na, nchunks, partials, skip);
+    else
+        forward_kernel<1><<<grid, 64 * kFwdWaves, 0, st>>>(x, p, g.d_fwd, g.nvalid, g.nx, g.ny, g.na, nchunks, partials, skip);
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int M>
+static void bp_dispatch(int mode, dim3 grid, hipStream_t st, const Geometry &g, const float *p,
+                        float *x, const float *bvec, int nrowblk, double *partials, const int *skip) {
+    if (mode == kBpSirtGrad)
+        backward_kernel<M, kBpSirtGrad><<<grid, kBpThreads, 0, st>>>(p, x, bvec, g.d_tile_cols, g.d_tile_org, g.d_coord, g.d_ang, g.radius, g.xcent, g.nx, g.ny, g.na, nrowblk, partials, g.d_err, skip);
+    else if (mode == kBpStoreNorm)
+        backward_kernel<M, kBpStoreNorm><<<grid, kBpThreads, 0, st>>>(p, x, bvec, g.d_tile_cols, g.d_tile_org, g.d_coord, g.d_ang, g.radius, g.xcent, g.nx, g.ny, g.na, nrowblk, partials, g.d_err, skip);
+    else
+        backward_kernel<M, kBpStore><<<grid, kBpThreads, 0, st>>>(p, x, bvec, g.d_tile_cols, g.d_tile_org, g.d_coord, g.d_ang, g.radius, g.xcent, g.nx, g.ny, g.na, nrowblk, partials, g.d_err, skip);
+}
+
+int launch_backward(const Geometry &g, const float *p, float *x, int mode, const float *bvec,
+                    double *partials, hipStream_t st, const int *skip) {
+    // columns that never contribute (linear pair's last two) stay zero
+    if (g.nvalid < g.nrays) {
+        int rc = fill(x + (size_t)g.nvalid * g.ny, 0.0f, (int64_t)(g.nrays - g.nvalid) * g.ny, st, skip);
+        if (rc) return rc;
+    }
+    if (g.ntiles == 0) return 0;
+    int nrowblk = (g.ny + kBpRows - 1) / kBpRows;
+    dim3 grid((unsigned)(g.ntiles * nrowblk));
+    if (g.method == kSQ)
+        bp_dispatch<kSQ>(mode, grid, st, g, p, x, bvec, nrowblk, partials, skip);
+    else if (g.method == kKB)
+        bp_dispatch<kKB>(mode, grid, st, g, p, x, bvec, nrowblk, partials, skip);
+    else
+        bp_dispatch<kLinear>(mode, grid, st, g, p, x, bvec, nrowblk, partials, skip);
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace arec

@@ -1,0 +1,649 @@
+// Device-resident SIRT (Landweber) and CGLS, gfx950.
+//
+// The reference's solver loops (arecsirt.cpp:11-191, areccgls.cpp:12-243,
+// areccgls_stat.cpp:49-275) make every control decision on the host from
+// FP64 norms.  Here those decisions run in one-workgroup control kernels that
+// read the norms on the device, so one pass of a loop is a fixed sequence of
+// kernels: it is captured once as a hipGraph and replayed, and the host only
+// polls the "done" flag between batches of replays.  Once the solver has
+// stopped, further replays are no-ops (every kernel checks a skip flag).
+//
+// Norms follow the reference's arithmetic: squares of FP32 values are formed
+// in FP32 or FP64 exactly where the reference forms them, and always summed
+// in FP64 (per-thread FP64 partials, wavefront shuffles, then one workgroup
+// summing the per-block partials in a fixed order, so results are
+// deterministic from run to run).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "arec_internal.hpp"
+
+namespace arec {
+
+constexpr int kVecThreads = 256;
+constexpr int kVecBlocks = 1024;
+
+struct SirtState {
+    int iter, restarts, done, first, zero_x, update, skip_all, skip_proj;
+    int nhist, hist_cap, maxit, pad;
+    float lam, tol;
+    double bnorm2, bnorm, relnrm0;
+};
+
+struct CglsState {
+    int iter, done, skip, best, iterbest, nhist, hist_cap, maxit;
+    double gamma, gamma0, alpha, beta, rnorm0, rnorm_min, tol;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum of n FP64 partials by one workgroup of kVecThreads, fixed order.  The
+// result is valid in every thread.
+__device__ double block_sum(const double *__restrict__ part, int64_t n) {
+    __shared__ double s[kVecThreads / 64];
+    double v = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kVecThreads) v += part[i];
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < kVecThreads / 64; w++) t += s[w];
+    return t;
+}
+
+// Per-block partial of a grid-stride reduction (written to out[blockIdx.x]).
+__device__ __forceinline__ void block_partial(double v, double *__restrict__ out) {
+    __shared__ double s[kVecThreads / 64];
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kVecThreads / 64; w++) t += s[w];
+        out[blockIdx.x] = t;
+    }
+}
+
+#define GRID_STRIDE(i, n) \
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+__global__ void fill_kernel(float *__restrict__ x, float v, int64_t n, const int *skip) {
+    if (skip && *skip) return;
+    GRID_STRIDE(i, n) x[i] = v;
+}
+
+static dim3 vec_grid(int64_t n) {
+    int64_t b = (n + kVecThreads - 1) / kVecThreads;
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(b, kVecBlocks)));
+}
+
+int fill(float *x, float v, int64_t n, hipStream_t st, const int *skip) {
+    if (n <= 0) return 0;
+    fill_kernel<<<vec_grid(n), kVecThreads, 0, st>>>(x, v, n, skip);
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
+__device__ __forceinline__ void push_hist(arec_hist *h, int cap, int *n, int iter, int event,
+                                          double relnrm, double rnorm) {
+    if (*n < cap) {
+        h[*n].iter = iter;
+        h[*n].event = event;
+        h[*n].relnrm = relnrm;
+        h[*n].rnorm = rnorm;
+    }
+    (*n)++;
+}
+
+// ------------------------------------------------------------------ SIRT --
+
+__global__ void sirt_begin(SirtState *st) {
+    st->skip_all = st->done;
+    st->skip_proj = st->done || st->first;
+}
+
+// First pass after the start or a restart (arecsirt.cpp:96-108):
+// grad = bdata; partials hold sum (double)(float)(b*b) in [0, G) and
+// sum grad*(double)grad in [G, 2G).
+__global__ __launch_bounds__(kVecThreads) void sirt_first_pass(const float *__restrict__ bdata,
+                                                               float *__restrict__ grad, int64_t n,
+                                                               double *__restrict__ part,
+                                                               const SirtState *st) {
+    if (st->skip_all || !st->first) return;
+    double sb = 0.0, sg = 0.0;
+    GRID_STRIDE(i, n) {
+        float b = bdata[i];
+        float s = b * b;
+        sb += (double)s;
+        sg += (double)b * (double)b;
+        grad[i] = b;
+    }
+    block_partial(sb, part);
+    block_partial(sg, part + gridDim.x);
+}
+
+// Decisions of one pass (arecsirt.cpp:121-177).
+__global__ __launch_bounds__(kVecThreads) void sirt_control(SirtState *st,
+                                                            const double *__restrict__ bp_part,
+                                                            int64_t n_bp,
+                                                            const double *__restrict__ fp_part,
+                                                            int n_fp, arec_hist *hist) {
+    if (st->skip_all) return;
+    const int first = st->first;
+    double s_b = 0.0, s_g;
+    if (first) {
+        s_b = block_sum(fp_part, n_fp);
+        s_g = block_sum(fp_part + n_fp, n_fp);
+    } else {
+        s_g = block_sum(bp_part, n_bp);
+    }
+    if (threadIdx.x != 0) return;
+    if (first) {
+        st->bnorm2 += s_b;  // accumulates across restarts (:91,104)
+        st->bnorm = sqrt(st->bnorm2);
+    }
+    double rnorm = sqrt(s_g);
+    double relnrm = rnorm / st->bnorm;
+    st->zero_x = 0;
+    st->update = 0;
+    if (relnrm > st->relnrm0) {
+        if (st->restarts > 20) {
+            push_hist(hist, st->hist_cap, &st->nhist, st->iter, 2, relnrm, rnorm);
+            st->done = 1;
+            return;
+        }
+        push_hist(hist, st->hist_cap, &st->nhist, st->iter, 1, relnrm, rnorm);
+        st->restarts++;
+        st->iter = 1;
+        st->lam = (float)(st->lam / 2.0);
+        st->relnrm0 = 1.0001;
+        st->zero_x = 1;
+        st->first = 1;
+        return;
+    }
+    if (relnrm < st->tol) {
+        push_hist(hist, st->hist_cap, &st->nhist, st->iter, 2, relnrm, rnorm);
+        st->done = 1;
+        return;
+    }
+    push_hist(hist, st->hist_cap, &st->nhist, st->iter, 0, relnrm, rnorm);
+    st->relnrm0 = relnrm;
+    st->update = 1;
+    st->first = 0;
+    st->iter++;
+    if (st->iter > st->maxit) st->done = 1;
+}
+
+// x += lam*grad (FP32), or x = 0 after a restart (arecsirt.cpp:145-177).
+__global__ __launch_bounds__(kVecThreads) void sirt_update(float *__restrict__ x,
+                                                           const float *__restrict__ grad, int64_t n,
+                                                           const SirtState *st) {
+    if (st->skip_all) return;
+    if (st->zero_x) {
+        GRID_STRIDE(i, n) x[i] = 0.0f;
+    } else if (st->update) {
+        const float lam = st->lam;
+        GRID_STRIDE(i, n) {
+            float t = lam * grad[i];
+            x[i] = x[i] + t;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ CGLS --
+
+__global__ void cgls_begin(CglsState *st) { st->skip = st->done; }
+
+// s = b - prj, then s *= w for the weighted solver (areccgls.cpp:111-112,
+// areccgls_stat.cpp:138-149).
+__global__ __launch_bounds__(kVecThreads) void cgls_residual(const float *__restrict__ b,
+                                                             const float *__restrict__ prj,
+                                                             const float *__restrict__ w,
+                                                             float *__restrict__ s, int64_t n) {
+    GRID_STRIDE(i, n) {
+        float v = b[i] - prj[i];
+        if (w) v = v * w[i];
+        s[i] = v;
+    }
+}
+
+// gamma = ||g||^2, rnorm0 (areccgls.cpp:127-139); p = g.
+__global__ __launch_bounds__(kVecThreads) void cgls_init_control(CglsState *st,
+                                                                 const double *__restrict__ part,
+                                                                 int64_t n, arec_hist *hist) {
+    double s = block_sum(part, n);
+    if (threadIdx.x != 0) return;
+    st->gamma = s;
+    st->rnorm0 = sqrt(s);
+    st->rnorm_min = st->rnorm0;
+    push_hist(hist, st->hist_cap, &st->nhist, 0, 0, 1.0, st->rnorm0);
+}
+
+// Weighted ||q||^2: per-block partials of (double)(float)((float)(q*q) * w).
+__global__ __launch_bounds__(kVecThreads) void weighted_sq(const float *__restrict__ q,
+                                                           const float *__restrict__ w, int64_t n,
+                                                           double *__restrict__ part,
+                                                           const CglsState *st) {
+    if (st->skip) return;
+    double acc = 0.0;
+    GRID_STRIDE(i, n) {
+        float t = q[i] * q[i];
+        float u = t * w[i];
+        acc += (double)u;
+    }
+    block_partial(acc, part);
+}
+
+// alpha = gamma / ||q||^2 (areccgls.cpp:155-163).
+__global__ __launch_bounds__(kVecThreads) void cgls_alpha(CglsState *st,
+                                                          const double *__restrict__ part, int64_t n) {
+    if (st->skip) return;
+    double s = block_sum(part, n);
+    if (threadIdx.x == 0) st->alpha = st->gamma / s;
+}
+
+// s -= alpha q (or alpha q w), evaluated in FP64 and rounded (:170-171).
+__global__ __launch_bounds__(kVecThreads) void cgls_update_s(float *__restrict__ s,
+                                                             const float *__restrict__ q,
+                                                             const float *__restrict__ w, int64_t n,
+                                                             const CglsState *st) {
+    if (st->skip) return;
+    const double alpha = st->alpha;
+    GRID_STRIDE(i, n) {
+        double t = alpha * (double)q[i];
+        if (w) t = t * (double)w[i];
+        s[i] = (float)((double)s[i] - t);
+    }
+}
+
+// gamma, relnrm, best iterate, termination, beta (areccgls.cpp:173-220).
+__global__ __launch_bounds__(kVecThreads) void cgls_gamma(CglsState *st,
+                                                          const double *__restrict__ part, int64_t n,
+                                                          arec_hist *hist) {
+    if (st->skip) return;
+    double s = block_sum(part, n);
+    if (threadIdx.x != 0) return;
+    st->gamma0 = st->gamma;
+    st->gamma = s;
+    double rnorm = sqrt(s);
+    double relnrm = rnorm / st->rnorm0;
+    st->best = 0;
+    if (rnorm < st->rnorm_min) {
+        st->rnorm_min = rnorm;
+        st->iterbest = st->iter;
+        st->best = 1;
+    }
+    int stop = relnrm < st->tol;
+    push_hist(hist, st->hist_cap, &st->nhist, st->iter, stop ? 2 : 0, relnrm, rnorm);
+    st->beta = st->gamma / st->gamma0;
+    st->iter++;
+    if (stop || st->iter > st->maxit) st->done = 1;
+}
+
+// x += alpha p; xsave = x when this iterate is the best so far; p = g + beta p
+// for the next pass (areccgls.cpp:143-151,165-168,190-195).
+__global__ __launch_bounds__(kVecThreads) void cgls_update_x(float *__restrict__ x,
+                                                             float *__restrict__ p,
+                                                             const float *__restrict__ g,
+                                                             float *__restrict__ xsave, int64_t n,
+                                                             const CglsState *st) {
+    if (st->skip) return;
+    const double alpha = st->alpha, beta = st->beta;
+    const int best = st->best;
+    GRID_STRIDE(i, n) {
+        float pi = p[i];
+        float xi = (float)((double)x[i] + alpha * (double)pi);
+        x[i] = xi;
+        if (best) xsave[i] = xi;
+        p[i] = (float)((double)g[i] + beta * (double)pi);
+    }
+}
+
+__global__ void copy_kernel(float *__restrict__ dst, const float *__restrict__ src, int64_t n) {
+    GRID_STRIDE(i, n) dst[i] = src[i];
+}
+
+// ------------------------------------------------------------ host drivers --
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) hipFree(p);
+    }
+    template <typename T>
+    T *as() {
+        return (T *)p;
+    }
+};
+
+int alloc(DevBuf &b, size_t bytes) {
+    AREC_HIP(hipMalloc(&b.p, std::max<size_t>(bytes, 16)));
+    return 0;
+}
+
+// Replays a captured pass until the state's done flag is set, polling every
+// `batch` replays through pinned memory.
+struct PassRunner {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~PassRunner() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+    }
+};
+
+}  // namespace
+
+int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float lam, int maxit,
+             float tol, arec_hist *hist, int hist_cap, int *nhist, float *lam_out, bool use_graph) {
+    const int64_t nnz = g.nnz, nproj = g.nproj;
+    DevBuf bdata, grad, proj, bp_part, fp_part, dstate, dhist;
+    int rc;
+    if ((rc = alloc(bdata, sizeof(float) * nnz)) || (rc = alloc(grad, sizeof(float) * nnz)) ||
+        (rc = alloc(proj, sizeof(float) * nproj)) ||
+        (rc = alloc(bp_part, sizeof(double) * std::max<int64_t>(1, backward_partial_count(g)))) ||
+        (rc = alloc(fp_part, sizeof(double) * 2 * kVecBlocks)) ||
+        (rc = alloc(dstate, sizeof(SirtState))) ||
+        (rc = alloc(dhist, sizeof(arec_hist) * std::max(1, hist_cap))))
+        return rc;
+    SirtState s0 = {};
+    s0.iter = 1;
+    s0.first = 1;
+    s0.lam = lam;
+    s0.tol = tol;
+    s0.maxit = maxit;
+    s0.hist_cap = hist_cap;
+    s0.relnrm0 = 1.00001;
+    s0.done = maxit < 1;
+    SirtState *S = dstate.as<SirtState>();
+    AREC_HIP(hipMemcpyAsync(S, &s0, sizeof s0, hipMemcpyHostToDevice, st));
+    // x = 0 (arecsirt.cpp:53-56); bdata = A^T b (:98)
+    if ((rc = fill(x, 0.0f, nnz, st, nullptr))) return rc;
+    if ((rc = launch_backward(g, b, bdata.as<float>(), kBpStore, nullptr, nullptr, st, nullptr))) return rc;
+
+    const int fpG = (int)vec_grid(nnz).x;
+    auto pass = [&]() -> int {
+        sirt_begin<<<1, 1, 0, st>>>(S);
+        int r;
+        if ((r = launch_forward(g, x, proj.as<float>(), nullptr, st, &S->skip_proj))) return r;
+        if ((r = launch_backward(g, proj.as<float>(), grad.as<float>(), kBpSirtGrad, bdata.as<float>(),
+                                 bp_part.as<double>(), st, &S->skip_proj)))
+            return r;
+        sirt_first_pass<<<fpG, kVecThreads, 0, st>>>(bdata.as<float>(), grad.as<float>(), nnz,
+                                                     fp_part.as<double>(), S);
+        sirt_control<<<1, kVecThreads, 0, st>>>(S, bp_part.as<double>(), backward_partial_count(g),
+                                                fp_part.as<double>(), fpG, dhist.as<arec_hist>());
+        sirt_update<<<vec_grid(nnz), kVecThreads, 0, st>>>(x, grad.as<float>(), nnz, S);
+        AREC_HIP(hipGetLastError());
+        return 0;
+    };
+
+    PassRunner runner;
+    if (use_graph) {
+        AREC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        rc = pass();
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc) return rc;
+        AREC_HIP(e);
+        runner.graph = graph;
+        AREC_HIP(hipGraphInstantiate(&runner.exec, graph, nullptr, nullptr, 0));
+    }
+    int *done_h = nullptr;
+    AREC_HIP(hipHostMalloc((void **)&done_h, sizeof(int), 0));
+    *done_h = 0;
+    // At most 22 runs of maxit passes (21 restarts), plus slack.
+    const int64_t max_passes = (int64_t)(maxit + 1) * 22 + 2;
+    int64_t passes = 0;
+    int batch = 4;
+    while (passes < max_passes) {
+        for (int k = 0; k < batch; k++) {
+            if (use_graph) {
+                hipError_t e = hipGraphLaunch(runner.exec, st);
+                if (e != hipSuccess) {
+                    hipHostFree(done_h);
+                    return hip_fail(e, "hipGraphLaunch");
+                }
+            } else if ((rc = pass())) {
+                hipHostFree(done_h);
+                return rc;
+            }
+        }
+        passes += batch;
+        hipMemcpyAsync(done_h, &S->done, sizeof(int), hipMemcpyDeviceToHost, st);
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            hipHostFree(done_h);
+            return hip_fail(e, "hipStreamSynchronize");
+        }
+        if (*done_h) break;
+        batch = std::min(batch * 2, 32);
+    }
+    hipHostFree(done_h);
+    SirtState sf;
+    AREC_HIP(hipMemcpy(&sf, S, sizeof sf, hipMemcpyDeviceToHost));
+    int n = std::min(sf.nhist, hist_cap);
+    if (hist && n > 0) AREC_HIP(hipMemcpy(hist, dhist.p, sizeof(arec_hist) * n, hipMemcpyDeviceToHost));
+    if (nhist) *nhist = sf.nhist;
+    if (lam_out) *lam_out = sf.lam;
+    return 0;
+}
+
+int cgls_run(const Geometry &g, hipStream_t st, const float *b, const float *w, float *x, int warm,
+             int maxit, float tol, arec_hist *hist, int hist_cap, int *nhist, bool use_graph) {
+    const int64_t nnz = g.nnz, nproj = g.nproj;
+    DevBuf prj, s, gv, pv, xsave, bp_part, fw_part, dstate, dhist;
+    int rc;
+    const int64_t nfp = std::max<int64_t>(forward_partial_count(g), kVecBlocks);
+    if ((rc = alloc(prj, sizeof(float) * nproj)) || (rc = alloc(s, sizeof(float) * nproj)) ||
+        (rc = alloc(gv, sizeof(float) * nnz)) || (rc = alloc(pv, sizeof(float) * nnz)) ||
+        (rc = alloc(xsave, sizeof(float) * nnz)) ||
+        (rc = alloc(bp_part, sizeof(double) * std::max<int64_t>(1, backward_partial_count(g)))) ||
+        (rc = alloc(fw_part, sizeof(double) * nfp)) || (rc = alloc(dstate, sizeof(CglsState))) ||
+        (rc = alloc(dhist, sizeof(arec_hist) * std::max(1, hist_cap))))
+        return rc;
+    CglsState c0 = {};
+    c0.iter = 1;
+    c0.iterbest = maxit - 1;
+    c0.maxit = maxit;
+    c0.hist_cap = hist_cap;
+    c0.tol = tol;
+    c0.done = maxit < 1;
+    CglsState *S = dstate.as<CglsState>();
+    AREC_HIP(hipMemcpyAsync(S, &c0, sizeof c0, hipMemcpyHostToDevice, st));
+    if (warm) {
+        if ((rc = launch_forward(g, x, prj.as<float>(), nullptr, st, nullptr))) return rc;
+    } else {
+        if ((rc = fill(x, 0.0f, nnz, st, nullptr)) || (rc = fill(prj.as<float>(), 0.0f, nproj, st, nullptr)))
+            return rc;
+    }
+    cgls_residual<<<vec_grid(nproj), kVecThreads, 0, st>>>(b, prj.as<float>(), w, s.as<float>(), nproj);
+    if ((rc = launch_backward(g, s.as<float>(), gv.as<float>(), kBpStoreNorm, nullptr, bp_part.as<double>(), st, nullptr)))
+        return rc;
+    cgls_init_control<<<1, kVecThreads, 0, st>>>(S, bp_part.as<double>(), backward_partial_count(g), dhist.as<arec_hist>());
+    copy_kernel<<<vec_grid(nnz), kVecThreads, 0, st>>>(pv.as<float>(), gv.as<float>(), nnz);
+    AREC_HIP(hipGetLastError());
+
+    const int64_t nfwd = w ? (int64_t)vec_grid(nproj).x : forward_partial_count(g);
+    auto pass = [&]() -> int {
+        cgls_begin<<<1, 1, 0, st>>>(S);
+        int r;
+        if ((r = launch_forward(g, pv.as<float>(), prj.as<float>(), w ? nullptr : fw_part.as<double>(), st, &S->skip)))
+            return r;
+        if (w)
+            weighted_sq<<<vec_grid(nproj), kVecThreads, 0, st>>>(prj.as<float>(), w, nproj, fw_part.as<double>(), S);
+        cgls_alpha<<<1, kVecThreads, 0, st>>>(S, fw_part.as<double>(), nfwd);
+        cgls_update_s<<<vec_grid(nproj), kVecThreads, 0, st>>>(s.as<float>(), prj.as<float>(), w, nproj, S);
+        if ((r = launch_backward(g, s.as<float>(), gv.as<float>(), kBpStoreNorm, nullptr, bp_part.as<double>(), st, &S->skip)))
+            return r;
+        cgls_gamma<<<1, kVecThreads, 0, st>>>(S, bp_part.as<double>(), backward_partial_count(g), dhist.as<arec_hist>());
+        cgls_update_x<<<vec_grid(nnz), kVecThreads, 0, st>>>(x, pv.as<float>(), gv.as<float>(), xsave.as<float>(), nnz, S);
+        AREC_HIP(hipGetLastError());
+        return 0;
+    };
+    PassRunner runner;
+    if (use_graph) {
+        AREC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        rc = pass();
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc) return rc;
+        AREC_HIP(e);
+        runner.graph = graph;
+        AREC_HIP(hipGraphInstantiate(&runner.exec, graph, nullptr, nullptr, 0));
+    }
+    int *done_h = nullptr;
+    AREC_HIP(hipHostMalloc((void **)&done_h, sizeof(int), 0));
+    *done_h = 0;
+    int64_t passes = 0;
+    int batch = 4;
+    while (passes < (int64_t)maxit + 1) {
+        int nb = (int)std::min<int64_t>(batch, maxit + 1 - passes);
+        for (int k = 0; k < nb; k++) {
+            if (use_graph) {
+                hipError_t e = hipGraphLaunch(runner.exec, st);
+                if (e != hipSuccess) {
+                    hipHostFree(done_h);
+                    return hip_fail(e, "hipGraphLaunch");
+                }
+            } else if ((rc = pass())) {
+                hipHostFree(done_h);
+                return rc;
+            }
+        }
+        passes += nb;
+        hipMemcpyAsync(done_h, &S->done, sizeof(int), hipMemcpyDeviceToHost, st);
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            hipHostFree(done_h);
+            return hip_fail(e, "hipStreamSynchronize");
+        }
+        if (*done_h) break;
+        batch = std::min(batch * 2, 32);
+    }
+    hipHostFree(done_h);
+    CglsState cf;
+    AREC_HIP(hipMemcpy(&cf, S, sizeof cf, hipMemcpyDeviceToHost));
+    // retrieve the best approximation (areccgls.cpp:226-230)
+    if (cf.iterbest < maxit - 1) {
+        copy_kernel<<<vec_grid(nnz), kVecThreads, 0, st>>>(x, xsave.as<float>(), nnz);
+        AREC_HIP(hipGetLastError());
+    }
+    int n = std::min(cf.nhist, hist_cap);
+    if (hist && n > 0) AREC_HIP(hipMemcpy(hist, dhist.p, sizeof(arec_hist) * n, hipMemcpyDeviceToHost));
+    if (nhist) *nhist = cf.nhist;
+    AREC_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+// ------------------------------------------------------------- utilities --
+
+// init_weights (areccgls_stat.cpp:12-47) with tukey_filter_x_inplace
+// (imagetools.cpp:369-391): w = (float)exp(-b) then w = (float)(w * hx).
+__global__ void init_weights_kernel(const float *__restrict__ b, float *__restrict__ w, int64_t nx,
+                                    int64_t n) {
+    const double alpha = 0.5, pi = 3.141592653589793;
+    const double xa = 0.5 * alpha * (double)(nx - 1);
+    const double xbnd = (1.0 - 0.5 * alpha) * (double)(nx - 1);
+    const double A = 2.0 * pi / alpha;
+    GRID_STRIDE(i, n) {
+        int64_t x = i % nx;
+        float e = (float)exp(-(double)b[i]);
+        double hx = 1.0;
+        if ((double)x < xa)
+            hx = 0.5 * (1.0 + cos(A * (1.0 * (double)x / (double)(nx - 1) - alpha / 2)));
+        else if ((double)x > xbnd)
+            hx = 0.5 * (1.0 + cos(A * (1.0 * (double)x / (double)(nx - 1) - 1.0 + alpha / 2)));
+        w[i] = (float)((double)e * hx);
+    }
+}
+
+int init_weights(const float *b, float *w, int64_t nx, int64_t ny, int64_t nz, hipStream_t st) {
+    int64_t n = nx * ny * nz;
+    init_weights_kernel<<<vec_grid(n), kVecThreads, 0, st>>>(b, w, nx, n);
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
+__global__ void soft_noneg_kernel(float *__restrict__ x, int64_t n) {
+    GRID_STRIDE(i, n) {
+        float v = x[i];
+        if (v < 0.0f) x[i] = v * 0.5f;
+    }
+}
+
+int soft_noneg(float *x, int64_t n, hipStream_t st) {
+    soft_noneg_kernel<<<vec_grid(n), kVecThreads, 0, st>>>(x, n);
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
+// Dot product: FP64 sum of exact FP64 products of FP32 values.
+__global__ __launch_bounds__(kVecThreads) void dot_kernel(const float *__restrict__ a,
+                                                          const float *__restrict__ b, int64_t n,
+                                                          double *__restrict__ part) {
+    double acc = 0.0;
+    GRID_STRIDE(i, n) acc += (double)a[i] * (double)b[i];
+    block_partial(acc, part);
+}
+
+__global__ __launch_bounds__(kVecThreads) void sum_kernel(const double *__restrict__ part, int64_t n,
+                                                          double *__restrict__ out) {
+    double s = block_sum(part, n);
+    if (threadIdx.x == 0) *out = s;
+}
+
+int dot(const float *a, const float *b, int64_t n, double *out, hipStream_t st) {
+    dim3 grid = vec_grid(n);
+    DevBuf part;
+    int rc;
+    if ((rc = alloc(part, sizeof(double) * (grid.x + 1)))) return rc;
+    dot_kernel<<<grid, kVecThreads, 0, st>>>(a, b, n, part.as<double>());
+    sum_kernel<<<1, kVecThreads, 0, st>>>(part.as<double>(), grid.x, part.as<double>() + grid.x);
+    AREC_HIP(hipGetLastError());
+    AREC_HIP(hipMemcpyAsync(out, part.as<double>() + grid.x, sizeof(double), hipMemcpyDeviceToHost, st));
+    AREC_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+__global__ __launch_bounds__(kVecThreads) void max_kernel(const float *__restrict__ x, int64_t n,
+                                                          float *__restrict__ part) {
+    __shared__ float s[kVecThreads / 64];
+    float m = -INFINITY;
+    GRID_STRIDE(i, n) m = fmaxf(m, x[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = s[0];
+        for (int w = 1; w < kVecThreads / 64; w++) t = fmaxf(t, s[w]);
+        part[blockIdx.x] = t;
+    }
+}
+
+int reduce_max(const float *x, int64_t n, float *out, hipStream_t st) {
+    dim3 grid = vec_grid(n);
+    DevBuf part, fin;
+    int rc;
+    if ((rc = alloc(part, sizeof(float) * grid.x)) || (rc = alloc(fin, sizeof(float)))) return rc;
+    max_kernel<<<grid, kVecThreads, 0, st>>>(x, n, part.as<float>());
+    max_kernel<<<1, kVecThreads, 0, st>>>(part.as<float>(), grid.x, fin.as<float>());
+    AREC_HIP(hipGetLastError());
+    AREC_HIP(hipMemcpyAsync(out, fin.p, sizeof(float), hipMemcpyDeviceToHost, st));
+    AREC_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // namespace arec

@@ -1,0 +1,165 @@
+/*
+ * arec3d_amd -- MI355X-native reconstruction hot path of AREC3D.
+ *
+ * C ABI of libarec3d_amd.so: plain pointers, 64-bit sizes, no C++ or torch
+ * types.  Two layers:
+ *
+ *  1. Host-pointer entry points (arec_h_*).  Same arguments, meaning and
+ *     status codes as the reference's projector / solver / alignment
+ *     functions; data is uploaded, computed on the GPU and downloaded.  Each
+ *     cites the reference function it replaces.  A C++ caller that wants the
+ *     reference's exact symbols (arecImage by value, MPI_Comm) links
+ *     libarec3d_dropin.so, whose declarations are in arec3d_dropin.h.
+ *
+ *  2. Device-resident entry points (arec_*).  Pointers are device pointers;
+ *     geometry tables live on the GPU; work is queued on the context's
+ *     stream.  These are what the solvers and benchmarks use.
+ *
+ * Data layouts are the reference's (AREC3D/AREC3D/arecImage.h:5-22):
+ *   volume      x[nrays][ny]      (compact cylinder, y fastest)
+ *   projections p[na][ny][nx]     (x fastest)
+ *   cord        (ix, iz) per column, iz-major
+ * Projector methods follow the reference's "pmethod" key: 1 = linear,
+ * 2 = Kaiser-Bessel, 3 = SQ (square-voxel footprint).
+ *
+ * Every function returns 0 on success; a negative AREC_E* code for errors
+ * this library adds; or the reference's own status code where the reference
+ * defines one.  arec_last_error() describes the last failure of the calling
+ * thread.
+ */
+#ifndef AREC3D_AMD_H
+#define AREC3D_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AREC_OK 0
+#define AREC_EINVAL (-101)  /* bad argument */
+#define AREC_EHIP (-102)    /* HIP runtime failure (no GPU, out of memory, ...) */
+#define AREC_ESIZE (-103)   /* size beyond what the tables encode */
+#define AREC_EGEOM (-104)   /* geometry self-check failed on the device */
+
+#define AREC_LINEAR 1
+#define AREC_KB 2
+#define AREC_SQ 3
+
+typedef struct arec_ctx arec_ctx;
+typedef struct arec_geom arec_geom;
+
+/* One line of a solver's residual history, as the reference prints it
+ * (arecsirt.cpp:131-132, areccgls.cpp:197-198).  event: 0 iteration,
+ * 1 SIRT restart with lam halved, 2 termination. */
+typedef struct {
+    int32_t iter;
+    int32_t event;
+    double relnrm;
+    double rnorm;
+} arec_hist;
+
+const char *arec_last_error(void);
+const char *arec_version(void);
+
+/* ---- context, memory ------------------------------------------------ */
+int arec_create(arec_ctx **ctx, int device);
+int arec_destroy(arec_ctx *ctx);
+int arec_synchronize(arec_ctx *ctx);
+int arec_dev_alloc(arec_ctx *ctx, void **ptr, size_t bytes);
+int arec_dev_free(arec_ctx *ctx, void *ptr);
+int arec_memcpy_h2d(arec_ctx *ctx, void *dst, const void *src, size_t bytes);
+int arec_memcpy_d2h(arec_ctx *ctx, void *dst, const void *src, size_t bytes);
+int arec_memset(arec_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* ---- geometry ------------------------------------------------------- */
+/* Cylinder mask of arecAllocateCylImage (arecImage.cpp:29-86).  Returns
+ * nrays; fills 2*nrays ints when cord is not NULL.  Host only. */
+int64_t arec_cyl_nrays(double radius, int32_t *cord);
+
+/* Geometry of one projector: detector nx x ny, cylinder radius, FP32 angles
+ * in radians.  Builds the per-angle constants, back-projection tiles and the
+ * forward table on the device.  nx must equal round(2 radius + 1). */
+int arec_geom_create(arec_ctx *ctx, arec_geom **geom, int method, int64_t nx, int64_t ny,
+                     double radius, const float *angles, int64_t nangles);
+int arec_geom_destroy(arec_geom *geom);
+/* nrays, nnz = nrays*ny, nproj = nx*ny*na; any pointer may be NULL. */
+int arec_geom_info(const arec_geom *geom, int64_t *nrays, int64_t *nnz, int64_t *nproj);
+int64_t arec_geom_device_bytes(const arec_geom *geom);
+
+/* ---- device-resident operators --------------------------------------- */
+/* p = A x   (arecProject2D / _KB / _SQ, arecproject.cpp:19-99,351-437,540-648) */
+int arec_project(arec_ctx *ctx, const arec_geom *geom, const float *x_dev, float *p_dev);
+/* x = A^T p (arecBackProject2D / _KB / _SQ, arecproject.cpp:109-183,447-535,650-765) */
+int arec_backproject(arec_ctx *ctx, const arec_geom *geom, const float *p_dev, float *x_dev);
+
+/* SIRT (Landweber) of cyl_sirt / _KB / _SQ (arecsirt.cpp:11-191,193-373,
+ * arecsirt_KB.cpp:11-191): x is zeroed, the restart chain halves lam.  The
+ * whole iteration runs on the device; hist (host memory) receives at most
+ * hist_cap lines. */
+int arec_sirt(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x_dev, float lam,
+              int32_t maxit, float tol, arec_hist *hist, int32_t hist_cap, int32_t *nhist,
+              float *lam_out);
+
+/* CGLS of cyl_cgls / _KB / _SQ (areccgls.cpp:12-243,245-478,
+ * areccgls_KB.cpp:12-240).  warm != 0 starts from x_dev (reference:
+ * xcvol->data != NULL); else x is zeroed.  w_dev != NULL gives the weighted
+ * solver of cyl_cgls*_stat (areccgls_stat.cpp:49-731) with weights w_dev
+ * [na][ny][nx] (see arec_init_weights). */
+int arec_cgls(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, const float *w_dev,
+              float *x_dev, int32_t warm, int32_t maxit, float tol, arec_hist *hist,
+              int32_t hist_cap, int32_t *nhist);
+
+/* init_weights (areccgls_stat.cpp:12-47): w = exp(-b) times the x-Tukey
+ * window (alpha 0.5) of each view. */
+int arec_init_weights(arec_ctx *ctx, const float *b_dev, float *w_dev, int64_t nx, int64_t ny,
+                      int64_t nz);
+
+/* soft_noneg (arecImage.cpp:812-826): negative values halved. */
+int arec_soft_noneg(arec_ctx *ctx, float *x_dev, int64_t n);
+
+/* Dot products for checks: FP64 sum of FP32 products (both compensated). */
+int arec_dot(arec_ctx *ctx, const float *a_dev, const float *b_dev, int64_t n, double *out);
+
+/* Average duration in ms of `reps` launches of one operator (which: 0
+ * forward A, 1 back A^T), timed with HIP events on the context's stream. */
+int arec_time_operator(arec_ctx *ctx, const arec_geom *geom, int which, const float *in_dev,
+                       float *out_dev, int32_t reps, float *ms_per_launch);
+
+/* Largest value of x (reduction), e.g. to scale noise to 1% of max(b). */
+int arec_max(arec_ctx *ctx, const float *x_dev, int64_t n, float *out);
+
+/* ---- synthetic inputs (no counterpart in the reference) --------------- */
+/* Sum of ellipsoids at the voxel centres of the compact cylinder of the
+ * given radius and height ny, zero outside `support` (radius about the
+ * axis).  params: 16 doubles per ellipsoid (cx, cy, cz, 3x3 world->body
+ * rotation, semi-axes ax, ay, az, density); drawn by arec3d_amd.phantom. */
+int arec_phantom(arec_ctx *ctx, double radius, int64_t ny, double support, const double *params,
+                 int32_t ne, float *x_dev);
+/* mode 0: b += sigma N(0,1).  mode 1: Beer-Lambert counts at i0 photons per
+ * ray with attenuation scale, b = -ln(counts/i0).  Counter-based RNG: the
+ * result depends only on (seed, index). */
+int arec_add_noise(arec_ctx *ctx, float *b_dev, int64_t n, uint64_t seed, int32_t mode,
+                   double sigma, double i0, double scale);
+
+/* ---- host-pointer drop-in layer -------------------------------------- */
+/* arecProject2D* (arecproject.h:55-62) with the cylinder given by radius and
+ * nrays from arecAllocateCylImage.  Status codes as the reference: 1 not a
+ * cylinder, 2 inconsistent dimension. */
+int arec_h_project(int method, double radius, int32_t nx, int32_t ny, const float *x,
+                   const float *angles, int32_t nangles, float *p);
+/* arecBackProject2D*: status 1 when nz != nangles, -1 size mismatch. */
+int arec_h_backproject(int method, double radius, int32_t nx, int32_t ny, int32_t nz,
+                       const float *p, const float *angles, int32_t nangles, float *x);
+/* cyl_sirt* / cyl_cgls* on host arrays; x receives nrays*ny values (the
+ * caller sizes it with arec_cyl_nrays(0.5*(nx-1))).  solver: 1 SIRT, 2 CGLS,
+ * 3 weighted CGLS.  For CGLS warm != 0 uses x as the start. */
+int arec_h_solve(int solver, int method, const float *b, int32_t nx, int32_t ny, int32_t nz,
+                 const float *angles, float lam, int32_t maxit, float tol, int32_t warm,
+                 float *x, arec_hist *hist, int32_t hist_cap, int32_t *nhist, float *lam_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
