@@ -69,6 +69,12 @@ def _load():
         "arec_time_operator": ([vp, vp, C.c_int, vp, vp, i32, P(flt)], C.c_int),
         "arec_phantom": ([vp, dbl, i64, dbl, vp, i32, vp], C.c_int),
         "arec_add_noise": ([vp, vp, i64, C.c_uint64, i32, dbl, dbl, dbl], C.c_int),
+        "arec_align_ccorr": ([vp, vp, vp, i32, i32, i32, vp], C.c_int),
+        "arec_align_cc": ([vp, vp, vp, i32, i32, i32, vp, vp], C.c_int),
+        "arec_align_peak": ([vp, vp, i32, i32, i32, vp, vp], C.c_int),
+        "arec_align_shift": ([vp, vp, i32, i32, i32, vp, vp, flt], C.c_int),
+        "arec_h_cc_images": ([i32, i32, i32, vp, i32, i32, i32, vp, vp, vp], C.c_int),
+        "arec_h_shift_images": ([i32, i32, i32, vp, vp, vp], C.c_int),
         "arec_h_project": ([C.c_int, dbl, i32, i32, vp, vp, i32, vp], C.c_int),
         "arec_h_backproject": ([C.c_int, dbl, i32, i32, i32, vp, vp, i32, vp], C.c_int),
         "arec_h_solve": ([C.c_int, C.c_int, vp, i32, i32, i32, vp, flt, i32, flt, i32, vp, P(Hist), i32, P(i32), P(flt)], C.c_int),
@@ -88,7 +94,8 @@ EXPORTED = [
     "arec_cyl_nrays", "arec_geom_create", "arec_geom_destroy", "arec_geom_info",
     "arec_geom_device_bytes", "arec_project", "arec_backproject", "arec_sirt", "arec_cgls",
     "arec_init_weights", "arec_soft_noneg", "arec_dot", "arec_max", "arec_time_operator", "arec_phantom",
-    "arec_add_noise", "arec_h_project", "arec_h_backproject",
+    "arec_add_noise", "arec_align_ccorr", "arec_align_cc", "arec_align_peak", "arec_align_shift",
+    "arec_h_cc_images", "arec_h_shift_images", "arec_h_project", "arec_h_backproject",
     "arec_h_solve",
 ]
 
@@ -274,6 +281,55 @@ def dot(ctx, a, b):
     out = C.c_double()
     _check(lib.arec_dot(ctx.h, a.ptr, b.ptr, a.size, C.byref(out)))
     return out.value
+
+
+def align_ccorr(ctx, x, y, c):
+    """Correlation maps of view stacks x, y ([nv, ny, nx] DeviceArrays) into c."""
+    nv, ny, nx = x.shape
+    _check(lib.arec_align_ccorr(ctx.h, x.ptr, y.ptr, nx, ny, nv, c.ptr))
+
+
+def align_cc(ctx, proj, meas, sx, sy):
+    """Sub-pixel shifts of meas against proj (arecCCImages) into sx, sy."""
+    nv, ny, nx = proj.shape
+    _check(lib.arec_align_cc(ctx.h, proj.ptr, meas.ptr, nx, ny, nv, sx.ptr, sy.ptr))
+
+
+def align_peak(ctx, cmap, sx, sy):
+    """speak's sub-pixel peak of each map ([nv, ny, nx] DeviceArray)."""
+    nv, ny, nx = cmap.shape
+    _check(lib.arec_align_peak(ctx.h, cmap.ptr, nx, ny, nv, sx.ptr, sy.ptr))
+
+
+def align_shift(ctx, data, sx, sy, sign=-1.0):
+    """Fourier shift of every view by (sign*sx, sign*sy) (arecShiftImages: sign -1)."""
+    nv, ny, nx = data.shape
+    _check(lib.arec_align_shift(ctx.h, data.ptr, nx, ny, nv, sx.ptr, sy.ptr, sign))
+
+
+def arecCCImages(comm, images1, images2, sx, sy):
+    """arecCCImages (arecImage.cpp:384-434) on host ArecImage stacks."""
+    a = np.ascontiguousarray(images1.data, np.float32)
+    b = np.ascontiguousarray(images2.data, np.float32)
+    ox = np.zeros(images1.nz, np.float32)
+    oy = np.zeros(images1.nz, np.float32)
+    rc = lib.arec_h_cc_images(images1.nx, images1.ny, images1.nz, _ptr(a), images2.nx, images2.ny, images2.nz,
+                              _ptr(b), _ptr(ox), _ptr(oy))
+    if rc <= -100:
+        raise ArecError(rc, lib.arec_last_error().decode())
+    if rc == 0:
+        sx[: images1.nz] = ox
+        sy[: images1.nz] = oy
+    return rc
+
+
+def arecShiftImages(images, sx, sy):
+    """arecShiftImages (arecImage.cpp:653-671): views shifted by (-sx, -sy)."""
+    d = np.ascontiguousarray(images.data, np.float32)
+    ax = np.ascontiguousarray(sx, np.float32)
+    ay = np.ascontiguousarray(sy, np.float32)
+    _check(lib.arec_h_shift_images(images.nx, images.ny, images.nz, _ptr(d), _ptr(ax), _ptr(ay)))
+    images.data[...] = d
 
 
 def reduce_max(ctx, a):

@@ -73,7 +73,37 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_dropin(verbose)
     return LIB
+
+
+MPI_ROOT = os.environ.get("MPI_ROOT", "/opt/conda")
+DROPIN = os.path.join(HERE, "libarec3d_dropin.so")
+
+
+def build_dropin(verbose=False):
+    """libarec3d_dropin.so: the reference's C++ symbols over the C ABI (host
+    code, g++), built when an MPI installation provides mpi.h."""
+    if not os.path.exists(os.path.join(MPI_ROOT, "include", "mpi.h")):
+        if verbose:
+            print("mpi.h not found: skipping libarec3d_dropin.so")
+        return None
+    tmp = DROPIN + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(MPI_ROOT, "include"),
+           "-DMPICH_SKIP_MPICXX", "-DOMPI_SKIP_MPICXX",
+           os.path.join(CSRC, "dropin.cpp"), "-o", tmp,
+           "-L", HERE, "-larec3d_amd", "-Wl,-rpath,$ORIGIN",
+           # system libraries first: the MPI directory also holds an older libstdc++
+           os.path.join(MPI_ROOT, "lib", "libmpi.so"),
+           "-Wl,-rpath,/usr/lib/x86_64-linux-gnu:" + os.path.join(MPI_ROOT, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"drop-in build failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, DROPIN)
+    if verbose:
+        print("built", DROPIN)
+    return DROPIN
 
 
 if __name__ == "__main__":

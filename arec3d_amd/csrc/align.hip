@@ -1,0 +1,532 @@
+// Translational alignment of measured views against reprojections, gfx950:
+// batched 2-D FFTs, conj-multiply, sub-pixel peak search and Fourier shift.
+//
+//  ccorr2d  (align2dstack.cpp:480-521):  c = IDFT_unnorm(conj(DFT x) DFT y)
+//  speak    (align2dstack.cpp:1059-1105): x-major scan for |c| > pmax with
+//           the signed pmax, circular 3x3 neighbourhood, quad2dfit
+//           (:1125-1166), wrap to (-n/2, n/2]
+//  fshift2d (align2dstack.cpp:566-630):  DFT, phase ramp with the FP32
+//           frequency tables and FP32 phase, inverse DFT, Re/(nx ny)
+//
+// FFTs are complex-to-complex in FP32 with FP64-accurate twiddle tables.
+// Power-of-two lengths run a radix-2 transform in LDS (one workgroup per
+// group of transforms); other lengths use a direct DFT (one output per
+// thread), so every view size the reference accepts is handled on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <vector>
+
+#include "arec_internal.hpp"
+
+namespace arec {
+
+constexpr int kFftThreads = 256;
+constexpr int kFftLds = 4096;  // complex values per workgroup
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Twiddles w[k] = exp(-2 pi i k / n), k < n (FP64-accurate, stored FP32).
+struct Twiddles {
+    int n = 0;
+    float2 *d = nullptr;
+};
+
+static std::map<int, Twiddles> &twiddle_cache() {
+    static std::map<int, Twiddles> c;
+    return c;
+}
+
+static int get_twiddles(int n, hipStream_t st, float2 **out) {
+    auto &c = twiddle_cache();
+    auto it = c.find(n);
+    if (it == c.end()) {
+        std::vector<float2> h(n);
+        const double pi = 3.141592653589793238462643;
+        for (int k = 0; k < n; k++) {
+            double a = -2.0 * pi * (double)k / (double)n;
+            h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        Twiddles t;
+        t.n = n;
+        AREC_HIP(hipMalloc((void **)&t.d, sizeof(float2) * n));
+        AREC_HIP(hipMemcpyAsync(t.d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice, st));
+        AREC_HIP(hipStreamSynchronize(st));
+        it = c.emplace(n, t).first;
+    }
+    *out = it->second.d;
+    return 0;
+}
+
+// Radix-2 FFT of `cnt` transforms of length n = 2^logn, in place.  Element m
+// of transform t of batch b sits at data[b*bstride + t*tstride + m*estride].
+// One workgroup handles `cnt` transforms (cnt*n <= kFftLds) of one batch.
+// dir = -1 forward, +1 inverse (unnormalised).
+__global__ __launch_bounds__(kFftThreads) void fft_pow2_kernel(float2 *__restrict__ data, int n,
+                                                               int logn, int64_t estride,
+                                                               int64_t tstride, int64_t bstride,
+                                                               int ntrans, int cnt,
+                                                               const float2 *__restrict__ tw, int dir) {
+    __shared__ float2 s[kFftLds];
+    const int groups = (ntrans + cnt - 1) / cnt;
+    const int b = blockIdx.x / groups;
+    const int t0 = (blockIdx.x % groups) * cnt;
+    const int c_here = min(cnt, ntrans - t0);
+    float2 *base = data + (int64_t)b * bstride + (int64_t)t0 * tstride;
+    // load with bit-reversed placement; consecutive threads take consecutive
+    // transforms first so that column transforms read contiguous rows
+    for (int e = threadIdx.x; e < c_here * n; e += blockDim.x) {
+        int t = e % c_here, m = e / c_here;
+        int r = __brev(m) >> (32 - logn);
+        s[t * n + r] = base[(int64_t)t * tstride + (int64_t)m * estride];
+    }
+    __syncthreads();
+    for (int half = 1; half < n; half <<= 1) {
+        const int step = n / (2 * half);
+        for (int e = threadIdx.x; e < c_here * (n / 2); e += blockDim.x) {
+            int t = e / (n / 2), k = e % (n / 2);
+            int grp = k / half, j = k % half;
+            int i0 = t * n + grp * 2 * half + j, i1 = i0 + half;
+            float2 w = tw[j * step];
+            if (dir > 0) w.y = -w.y;
+            float2 u = s[i0], v = cmul(s[i1], w);
+            s[i0] = make_float2(u.x + v.x, u.y + v.y);
+            s[i1] = make_float2(u.x - v.x, u.y - v.y);
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < c_here * n; e += blockDim.x) {
+        int t = e % c_here, m = e / c_here;
+        base[(int64_t)t * tstride + (int64_t)m * estride] = s[t * n + m];
+    }
+}
+
+// Direct DFT for lengths that are not powers of two: out-of-place, one
+// output element per thread.
+__global__ void dft_direct_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int n,
+                                  int64_t estride, int64_t tstride, int64_t bstride, int ntrans,
+                                  int64_t total, const float2 *__restrict__ tw, int dir) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int k = (int)(e % n);
+        int64_t rest = e / n;
+        int t = (int)(rest % ntrans);
+        int64_t b = rest / ntrans;
+        const float2 *src = in + b * bstride + (int64_t)t * tstride;
+        double sr = 0.0, si = 0.0;
+        for (int m = 0; m < n; m++) {
+            float2 w = tw[(int)(((int64_t)k * m) % n)];
+            double wi = dir > 0 ? -(double)w.y : (double)w.y;
+            float2 v = src[(int64_t)m * estride];
+            sr += (double)v.x * w.x - (double)v.y * wi;
+            si += (double)v.x * wi + (double)v.y * w.x;
+        }
+        out[b * bstride + (int64_t)t * tstride + (int64_t)k * estride] = make_float2((float)sr, (float)si);
+    }
+}
+
+__global__ void copy2_kernel(float2 *__restrict__ dst, const float2 *__restrict__ src, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// One batch of 1-D transforms along one axis.
+static int fft_axis(float2 *data, float2 *tmp, int n, int64_t estride, int64_t tstride,
+                    int64_t bstride, int ntrans, int nbatch, int dir, hipStream_t st) {
+    float2 *tw;
+    int rc = get_twiddles(n, st, &tw);
+    if (rc) return rc;
+    if (n == 1) return 0;
+    if ((n & (n - 1)) == 0 && n <= kFftLds) {
+        int logn = 0;
+        while ((1 << logn) < n) logn++;
+        int cnt = std::max(1, std::min(kFftLds / n, 16));
+        cnt = std::min(cnt, ntrans);
+        int groups = (ntrans + cnt - 1) / cnt;
+        fft_pow2_kernel<<<groups * nbatch, kFftThreads, 0, st>>>(data, n, logn, estride, tstride, bstride,
+                                                                ntrans, cnt, tw, dir);
+    } else {
+        int64_t total = (int64_t)n * ntrans * nbatch;
+        dft_direct_kernel<<<(unsigned)std::min<int64_t>((total + 255) / 256, 65535), 256, 0, st>>>(
+            data, tmp, n, estride, tstride, bstride, ntrans, total, tw, dir);
+        // copy back: the layout of tmp equals data's for the touched elements
+        int64_t span = (int64_t)nbatch * bstride;
+        copy2_kernel<<<(unsigned)std::min<int64_t>((span + 255) / 256, 65535), 256, 0, st>>>(data, tmp, span);
+    }
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
+// 2-D transform of nv views of [ny][nx] complex values.
+static int fft2(float2 *data, float2 *tmp, int nx, int ny, int nv, int dir, hipStream_t st) {
+    int rc = fft_axis(data, tmp, nx, 1, nx, (int64_t)nx * ny, ny, nv, dir, st);
+    if (!rc) rc = fft_axis(data, tmp, ny, nx, 1, (int64_t)nx * ny, nx, nv, dir, st);
+    return rc;
+}
+
+__global__ void pack_real_kernel(const float *__restrict__ x, float2 *__restrict__ c, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        c[i] = make_float2(x[i], 0.0f);
+}
+
+// conj(X) * Y (align2dstack.cpp:503-506, FP32)
+__global__ void conj_mul_kernel(float2 *__restrict__ X, const float2 *__restrict__ Y, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float2 a = X[i], b = Y[i];
+        X[i] = make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+    }
+}
+
+// Re(c) / div (fshift2d divides by (float)(nx*ny), align2dstack.cpp:619-621)
+__global__ void real_part_kernel(const float2 *__restrict__ c, float *__restrict__ out, int64_t n, float div) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = c[i].x / div;
+}
+
+// quad2dfit (align2dstack.cpp:1125-1166), FP64 on FP32 samples.
+__device__ void quad2dfit_dev(const float *Zp, float *xsh, float *ysh) {
+#define Z(i, j) ((double)Zp[((j)-1) * 3 + (i)-1])
+    double C2 = (8. * Z(1, 1) - 8. * Z(1, 2) + 5. * Z(2, 1) - 8. * Z(2, 2) + 3. * Z(2, 3) +
+                 2. * Z(3, 1) - 8. * Z(3, 2) + 6. * Z(3, 3)) /
+                (-6.);
+    double C3 = (Z(1, 1) - 2. * Z(1, 2) + Z(1, 3) + Z(2, 1) - 2. * Z(2, 2) + Z(2, 3) + Z(3, 1) -
+                 2. * Z(3, 2) + Z(3, 3)) /
+                6.0;
+    double C4 = (8. * Z(1, 1) + 5. * Z(1, 2) + 2. * Z(1, 3) - 8. * Z(2, 1) - 8. * Z(2, 2) -
+                 8. * Z(2, 3) + 3. * Z(3, 2) + 6. * Z(3, 3)) /
+                (-6.0);
+    double C5 = (Z(1, 1) - Z(1, 3) - Z(3, 1) + Z(3, 3)) / 4.0;
+    double C6 = (Z(1, 1) + Z(1, 2) + Z(1, 3) - 2. * Z(2, 1) - 2. * Z(2, 2) - 2. * Z(2, 3) +
+                 Z(3, 1) + Z(3, 2) + Z(3, 3)) /
+                6.0;
+#undef Z
+    float ys = 0.0f, xs = 0.0f;
+    double denom = 4. * C3 * C6 - C5 * C5;
+    if (denom != 0.0) {
+        ys = (float)((C4 * C5 - 2. * C2 * C6) / denom - 2.0);
+        xs = (float)((C2 * C5 - 2. * C4 * C3) / denom - 2.0);
+        if (ys < -1.0f) ys = -1.0f;
+        if (ys > 1.0f) ys = 1.0f;
+        if (xs < -1.0f) xs = -1.0f;
+        if (xs > 1.0f) xs = 1.0f;
+    }
+    *xsh = xs;
+    *ysh = ys;
+}
+
+// speak on each view's correlation map (one workgroup per view).  The scan
+// order key is x*ny + y; the reference keeps the first strictly larger |c|
+// and stores the signed value.  A positive winner is the sequential scan's
+// answer; after a negative winner the sequential state machine is replayed
+// exactly by one thread from that point on.
+__global__ __launch_bounds__(256) void speak_kernel(const float *__restrict__ cmap, int nx, int ny,
+                                                    float *__restrict__ sx, float *__restrict__ sy) {
+    __shared__ float s_v[256];
+    __shared__ int s_k[256];
+    __shared__ int s_ix, s_iy;
+    const int v = blockIdx.x;
+    const float *c = cmap + (int64_t)v * nx * ny;
+    const int n = nx * ny;
+    float best = 0.0f;
+    int bestk = n;  // none
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        int x = k / ny, y = k % ny;  // scan order: x outer, y inner
+        float a = fabsf(c[(int64_t)y * nx + x]);
+        if (a > best) {  // NaN never wins, like the reference's comparison
+            best = a;
+            bestk = k;
+        }
+    }
+    s_v[threadIdx.x] = best;
+    s_k[threadIdx.x] = bestk;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            float o = s_v[threadIdx.x + off];
+            int ok = s_k[threadIdx.x + off];
+            if (o > s_v[threadIdx.x] || (o == s_v[threadIdx.x] && ok < s_k[threadIdx.x])) {
+                s_v[threadIdx.x] = o;
+                s_k[threadIdx.x] = ok;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int k = s_k[0];
+        int ix = 1, iy = 1;  // no element above 0: reference indices undefined
+        if (k < n) {
+            float val = c[(int64_t)(k % ny) * nx + k / ny];
+            if (val > 0.0f) {
+                ix = k / ny + 1;
+                iy = k % ny + 1;
+            } else {
+                // exact replay of the sequential scan from the first element
+                float pmax = 0.0f;
+                for (int kk = 0; kk < n; kk++) {
+                    int x = kk / ny, y = kk % ny;
+                    float d = c[(int64_t)y * nx + x];
+                    if (fabsf(d) > pmax) {
+                        ix = x + 1;
+                        iy = y + 1;
+                        pmax = d;
+                    }
+                }
+            }
+        }
+        s_ix = ix;
+        s_iy = iy;
+        float rsq[9];
+        for (int j = -1; j <= 1; j++) {
+            int jc = iy + j;
+            if (jc < 1) jc += ny;
+            if (jc > ny) jc -= ny;
+            for (int i = -1; i <= 1; i++) {
+                int ir = ix + i;
+                if (ir < 1) ir += nx;
+                if (ir > nx) ir -= nx;
+                rsq[(j + 1) * 3 + i + 1] = c[(int64_t)(jc - 1) * nx + (ir - 1)];
+            }
+        }
+        float xsh, ysh;
+        quad2dfit_dev(rsq, &xsh, &ysh);
+        float xt = xsh + (float)(ix - 1), yt = ysh + (float)(iy - 1);
+        if (xt > nx / 2) xt -= (float)nx;
+        if (yt > ny / 2) yt -= (float)ny;
+        sx[v] = xt;
+        sy[v] = yt;
+    }
+}
+
+// Phase ramp of fshift2d with the FP32 tables and FP32 phase
+// (align2dstack.cpp:593-613); shift by (sgn*sx[v], sgn*sy[v]).
+__global__ void phase_ramp_kernel(float2 *__restrict__ F, int nx, int ny, int nv,
+                                  const float *__restrict__ sx, const float *__restrict__ sy, float sgn) {
+    const int64_t total = (int64_t)nx * ny * nv;
+    const int hx = (nx + 1) / 2, hy = (ny + 1) / 2;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int i = (int)(e % nx);
+        int64_t r = e / nx;
+        int j = (int)(r % ny);
+        int v = (int)(r / ny);
+        float kx = i < hx ? (float)i / (float)nx : (float)(i - nx) / (float)nx;
+        float ky = j < hy ? (float)j / (float)ny : (float)(j - ny) / (float)ny;
+        float ssx = sgn * sx[v], ssy = sgn * sy[v];
+        float t1 = __fmul_rn(ssx, kx), t2 = __fmul_rn(ssy, ky);
+        float phase = (float)(-2.0 * 3.141592653589793 * (double)__fadd_rn(t1, t2));
+        float cph, sph;
+        sincosf(phase, &sph, &cph);
+        float2 f = F[e];
+        F[e] = make_float2(f.x * cph - f.y * sph, f.x * sph + f.y * cph);
+    }
+}
+
+static dim3 ew_grid(int64_t n) {
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)));
+}
+
+// Workspace for batches of views.
+struct AlignWork {
+    float2 *a = nullptr, *b = nullptr, *t = nullptr;
+    size_t cap = 0;
+    ~AlignWork() {
+        hipFree(a);
+        hipFree(b);
+        hipFree(t);
+    }
+    int ensure(size_t n) {
+        if (n <= cap) return 0;
+        hipFree(a);
+        hipFree(b);
+        hipFree(t);
+        a = b = t = nullptr;
+        AREC_HIP(hipMalloc((void **)&a, sizeof(float2) * n));
+        AREC_HIP(hipMalloc((void **)&b, sizeof(float2) * n));
+        AREC_HIP(hipMalloc((void **)&t, sizeof(float2) * n));
+        cap = n;
+        return 0;
+    }
+};
+
+static int views_per_batch(int64_t nx, int64_t ny, int64_t nv) {
+    int64_t per = nx * ny;
+    int64_t vb = std::max<int64_t>(1, (int64_t)(256ll << 20) / per);  // <= 2 GiB per buffer
+    return (int)std::min<int64_t>(vb, nv);
+}
+
+// c (optional, [nv][ny][nx]) = correlation maps; sx, sy (optional) = peaks.
+int cc_views(const float *proj, const float *meas, int nx, int ny, int nv, float *cmap_out,
+             float *sx, float *sy, hipStream_t st) {
+    AlignWork w;
+    const int vb = views_per_batch(nx, ny, nv);
+    const int64_t per = (int64_t)nx * ny;
+    int rc;
+    if ((rc = w.ensure((size_t)vb * per))) return rc;
+    float *cbuf = nullptr;
+    if (!cmap_out) AREC_HIP(hipMalloc((void **)&cbuf, sizeof(float) * vb * per));
+    for (int v0 = 0; v0 < nv; v0 += vb) {
+        int n = std::min(vb, nv - v0);
+        int64_t cnt = (int64_t)n * per;
+        pack_real_kernel<<<ew_grid(cnt), 256, 0, st>>>(proj + v0 * per, w.a, cnt);
+        pack_real_kernel<<<ew_grid(cnt), 256, 0, st>>>(meas + v0 * per, w.b, cnt);
+        if ((rc = fft2(w.a, w.t, nx, ny, n, -1, st)) || (rc = fft2(w.b, w.t, nx, ny, n, -1, st))) break;
+        conj_mul_kernel<<<ew_grid(cnt), 256, 0, st>>>(w.a, w.b, cnt);
+        if ((rc = fft2(w.a, w.t, nx, ny, n, +1, st))) break;
+        float *dst = cmap_out ? cmap_out + v0 * per : cbuf;
+        real_part_kernel<<<ew_grid(cnt), 256, 0, st>>>(w.a, dst, cnt, 1.0f);
+        if (sx && sy) speak_kernel<<<n, 256, 0, st>>>(dst, nx, ny, sx + v0, sy + v0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "cc_views");
+            break;
+        }
+    }
+    hipStreamSynchronize(st);
+    hipFree(cbuf);
+    return rc;
+}
+
+// data[v] <- fshift2d(data[v], sgn*sx[v], sgn*sy[v]); arecShiftImages uses
+// sgn = -1 (arecImage.cpp:664).
+int shift_views(float *data, int nx, int ny, int nv, const float *sx, const float *sy, float sgn,
+                hipStream_t st) {
+    AlignWork w;
+    const int vb = views_per_batch(nx, ny, nv);
+    const int64_t per = (int64_t)nx * ny;
+    int rc;
+    if ((rc = w.ensure((size_t)vb * per))) return rc;
+    for (int v0 = 0; v0 < nv; v0 += vb) {
+        int n = std::min(vb, nv - v0);
+        int64_t cnt = (int64_t)n * per;
+        pack_real_kernel<<<ew_grid(cnt), 256, 0, st>>>(data + v0 * per, w.a, cnt);
+        if ((rc = fft2(w.a, w.t, nx, ny, n, -1, st))) break;
+        phase_ramp_kernel<<<ew_grid(cnt), 256, 0, st>>>(w.a, nx, ny, n, sx + v0, sy + v0, sgn);
+        if ((rc = fft2(w.a, w.t, nx, ny, n, +1, st))) break;
+        real_part_kernel<<<ew_grid(cnt), 256, 0, st>>>(w.a, data + v0 * per, cnt, (float)(nx * ny));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "shift_views");
+            break;
+        }
+    }
+    hipStreamSynchronize(st);
+    return rc;
+}
+
+}  // namespace arec
+
+using namespace arec;
+
+extern "C" {
+
+int arec_align_ccorr(arec_ctx *ctx, const float *x_dev, const float *y_dev, int32_t nx, int32_t ny,
+                     int32_t nviews, float *c_dev) {
+    if (!ctx || !x_dev || !y_dev || !c_dev || nx < 1 || ny < 1 || nviews < 1) {
+        set_error("arec_align_ccorr: bad argument");
+        return AREC_EINVAL;
+    }
+    return cc_views(x_dev, y_dev, nx, ny, nviews, c_dev, nullptr, nullptr, ctx->stream);
+}
+
+int arec_align_cc(arec_ctx *ctx, const float *proj_dev, const float *meas_dev, int32_t nx, int32_t ny,
+                  int32_t nviews, float *sx_dev, float *sy_dev) {
+    if (!ctx || !proj_dev || !meas_dev || !sx_dev || !sy_dev || nx < 1 || ny < 1 || nviews < 1) {
+        set_error("arec_align_cc: bad argument");
+        return AREC_EINVAL;
+    }
+    return cc_views(proj_dev, meas_dev, nx, ny, nviews, nullptr, sx_dev, sy_dev, ctx->stream);
+}
+
+int arec_align_peak(arec_ctx *ctx, const float *c_dev, int32_t nx, int32_t ny, int32_t nviews,
+                    float *sx_dev, float *sy_dev) {
+    if (!ctx || !c_dev || !sx_dev || !sy_dev || nx < 1 || ny < 1 || nviews < 1) {
+        set_error("arec_align_peak: bad argument");
+        return AREC_EINVAL;
+    }
+    speak_kernel<<<nviews, 256, 0, ctx->stream>>>(c_dev, nx, ny, sx_dev, sy_dev);
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
+int arec_align_shift(arec_ctx *ctx, float *data_dev, int32_t nx, int32_t ny, int32_t nviews,
+                     const float *sx_dev, const float *sy_dev, float sign) {
+    if (!ctx || !data_dev || !sx_dev || !sy_dev || nx < 1 || ny < 1 || nviews < 1) {
+        set_error("arec_align_shift: bad argument");
+        return AREC_EINVAL;
+    }
+    return shift_views(data_dev, nx, ny, nviews, sx_dev, sy_dev, sign, ctx->stream);
+}
+
+// Host-pointer versions: arecCCImages (arecImage.cpp:384-434; status -1
+// when the stacks differ in size) and arecShiftImages (:653-671).
+int arec_h_cc_images(int32_t nx1, int32_t ny1, int32_t nz1, const float *images1, int32_t nx2,
+                     int32_t ny2, int32_t nz2, const float *images2, float *sx, float *sy) {
+    if (nx1 != nx2 || ny1 != ny2 || nz1 != nz2) {
+        set_error("arecCCImages: mismatch in dimension");
+        return -1;
+    }
+    if (!images1 || !images2 || !sx || !sy || nx1 < 1 || ny1 < 1 || nz1 < 1) {
+        set_error("arec_h_cc_images: bad argument");
+        return AREC_EINVAL;
+    }
+    int dev = 0;
+    hipStream_t st;
+    AREC_HIP(hipGetDevice(&dev));
+    AREC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    size_t n = (size_t)nx1 * ny1 * nz1;
+    float *a = nullptr, *b = nullptr, *s = nullptr;
+    int rc = 0;
+    if (hipMalloc((void **)&a, n * 4) != hipSuccess || hipMalloc((void **)&b, n * 4) != hipSuccess ||
+        hipMalloc((void **)&s, 8 * (size_t)nz1) != hipSuccess) {
+        rc = hip_fail(hipErrorOutOfMemory, "arec_h_cc_images");
+    } else {
+        hipMemcpyAsync(a, images1, n * 4, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(b, images2, n * 4, hipMemcpyHostToDevice, st);
+        rc = cc_views(a, b, nx1, ny1, nz1, nullptr, s, s + nz1, st);
+        if (!rc) {
+            hipMemcpyAsync(sx, s, 4 * (size_t)nz1, hipMemcpyDeviceToHost, st);
+            hipMemcpyAsync(sy, s + nz1, 4 * (size_t)nz1, hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) rc = hip_fail(hipGetLastError(), "arec_h_cc_images");
+        }
+    }
+    hipFree(a);
+    hipFree(b);
+    hipFree(s);
+    hipStreamDestroy(st);
+    return rc;
+}
+
+int arec_h_shift_images(int32_t nx, int32_t ny, int32_t nz, float *data, const float *sx,
+                        const float *sy) {
+    if (!data || !sx || !sy || nx < 1 || ny < 1 || nz < 1) {
+        set_error("arec_h_shift_images: bad argument");
+        return AREC_EINVAL;
+    }
+    hipStream_t st;
+    AREC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    size_t n = (size_t)nx * ny * nz;
+    float *a = nullptr, *s = nullptr;
+    int rc = 0;
+    if (hipMalloc((void **)&a, n * 4) != hipSuccess || hipMalloc((void **)&s, 8 * (size_t)nz) != hipSuccess) {
+        rc = hip_fail(hipErrorOutOfMemory, "arec_h_shift_images");
+    } else {
+        hipMemcpyAsync(a, data, n * 4, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(s, sx, 4 * (size_t)nz, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(s + nz, sy, 4 * (size_t)nz, hipMemcpyHostToDevice, st);
+        rc = shift_views(a, nx, ny, nz, s, s + nz, -1.0f, st);
+        if (!rc) {
+            hipMemcpyAsync(data, a, n * 4, hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) rc = hip_fail(hipGetLastError(), "arec_h_shift_images");
+        }
+    }
+    hipFree(a);
+    hipFree(s);
+    hipStreamDestroy(st);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+"""Benchmark of the reconstruction hot path on one MI355X.
+
+Headline workload (default): a seeded 512 x 512 x 512 ellipsoid phantom in
+the compact cylinder (support radius 250, 40 ellipsoids, semi-axes U[8,120],
+density U[0.05,1]), 720 parallel-beam angles on [0, pi), detector 512 x 512,
+SQ projector, sinogram = A x + Gaussian noise (sigma = 1% of max).  One step
+is one SIRT solve (the reference's Landweber, arecsirt.cpp:193-373) from zero
+to a relative normal-equation residual ||A^T(b - A x)|| / ||A^T b|| <= 1e-3,
+lam = 0.9 / sigma_max^2 from power iteration.  With a trained warm-start
+network the step is the warm-started solve and the plain solve is reported
+beside it.
+
+Prints one JSON line (rank 0) with the roofline of the dominant kernel and a
+CPU baseline (the oracle, test infrastructure, timed on a bounded sub-slab).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time to 1e-3 rel. residual, 512^3/720 angles: warm-start vs plain SIRT"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def power_iteration(A, ctx, g, iters=20, seed=0):
+    v = ctx.to_device(np.random.default_rng(seed).standard_normal(g.vol_shape).astype(np.float32))
+    w = ctx.empty(g.proj_shape)
+    v2 = ctx.empty(g.vol_shape)
+    lam = 0.0
+    for _ in range(iters):
+        n = np.sqrt(A.dot(ctx, v, v))
+        v.upload(v.download() / n)
+        g.project(v, w)
+        g.backproject(w, v2)
+        lam = A.dot(ctx, v, v2)
+        v, v2 = v2, v
+    return lam
+
+
+def cpu_baseline(nx, na, rows, iters_gpu, ny_full):
+    """Oracle SQ forward + back on `rows` detector rows of the same geometry
+    (rows are independent and cost the same), scaled to the full height."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    nr = len(O.cyl_geometry(0.5 * (nx - 1)))
+    x = np.random.default_rng(1).random((nr, rows)).astype(np.float32)
+    t0 = time.perf_counter()
+    p = O.project("sq", x, ang, nx, rows)
+    O.backproject("sq", p, ang, nx, rows)
+    dt = time.perf_counter() - t0
+    per_iter_full = dt * ny_full / rows
+    vu = 2 * nr * rows * na
+    return {
+        "value": per_iter_full * iters_gpu,
+        "unit": "s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle SQ forward+back (1 SIRT iteration) on {rows} of {ny_full} rows at "
+                  f"{nx} wide, {na} angles: {dt:.2f} s = {vu / dt / 1e6:.1f} MVU/s; "
+                  f"scaled x{ny_full // rows} rows x {iters_gpu} iterations (the GPU's count)",
+        "vu_per_s": vu / dt,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512, help="detector width = volume width")
+    ap.add_argument("--rows", type=int, default=512)
+    ap.add_argument("--angles", type=int, default=720)
+    ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--maxit", type=int, default=3000)
+    ap.add_argument("--cpu-rows", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
+              "layout (one Y-slab per GPU) is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import arec3d_amd as A
+    from arec3d_amd import phantom as P
+
+    nx, ny, na = args.n, args.rows, args.angles
+    ctx = A.Context(0)
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    t0 = time.perf_counter()
+    g = A.Geometry(ctx, "sq", nx, ny, ang)
+    t_geom = time.perf_counter() - t0
+    support = 250.0 * nx / 512
+    x_true = ctx.empty(g.vol_shape)
+    prm = P.ellipsoid_params(49000, 40, support, ny, axes=(8.0 * nx / 512, 120.0 * nx / 512), dens=(0.05, 1.0))
+    P.phantom_device(ctx, x_true, g.radius, ny, support, prm)
+    b = ctx.empty(g.proj_shape)
+    g.project(x_true, b)
+    P.add_noise(ctx, b, 49000, rel_sigma=0.01)
+    ctx.synchronize()
+    sig2 = power_iteration(A, ctx, g)
+    lam = 0.9 / sig2
+    log(f"geometry {t_geom:.2f}s, sigma_max^2 {sig2:.4e}, lam {lam:.4e}")
+
+    x = ctx.empty(g.vol_shape)
+    for _ in range(args.warmup):
+        g.sirt(b, x, lam, 3, args.tol)
+    ctx.synchronize()
+    times, iters, final = [], [], None
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        hist, lo = g.sirt(b, x, lam, args.maxit, args.tol, hist_cap=args.maxit + 64)
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+        iters.append(hist[-1][0])
+        final = hist[-1]
+        log(f"step {s}: {times[-1]:.2f} s, {hist[-1][0]} iterations, relnrm {hist[-1][2]:.3e}")
+    t_plain = float(np.median(times))
+    n_it = int(np.median(iters))
+
+    # roofline of the projector pair: per-view streaming model
+    B = 4.0 * na * (g.nnz + nx * ny)
+    vu = g.nnz * na
+    p_tmp = ctx.empty(g.proj_shape)
+    ms_f = g.time_operator(0, x, p_tmp, 5)
+    ms_b = g.time_operator(1, p_tmp, x, 5)
+    dom, ms = ("forward_kernel", ms_f) if ms_f >= ms_b else ("backward_kernel", ms_b)
+    achieved = B / (ms * 1e-3) / 1e9
+    log(f"forward {ms_f:.2f} ms ({vu / ms_f / 1e6:.1f} GVU/s), back {ms_b:.2f} ms ({vu / ms_b / 1e6:.1f} GVU/s)")
+
+    out = {
+        "metric": METRIC,
+        "value": round(t_plain, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_plain * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantom, 1% Gaussian noise)",
+        "config": {
+            "workload": f"plain SIRT (SQ) to relnrm {args.tol:g}, {nx}x{nx}x{ny} volume, {na} angles, detector {nx}x{ny}",
+            "iterations": n_it,
+            "ms_per_iteration": round(t_plain * 1e3 / max(1, n_it), 3),
+            "final_relnrm": final[2],
+            "lam": lam,
+            "warm_start": "not built yet: value is the plain solve",
+            "geometry_setup_s": round(t_geom, 3),
+            "forward_ms": round(ms_f, 3),
+            "backward_ms": round(ms_b, 3),
+            "voxel_updates_per_s": round(2 * vu / ((ms_f + ms_b) * 1e-3), 1),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "model": "algorithmic bytes per launch 4*na*(nnz + nx*ny) (per-view streaming)",
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(nx, na, args.cpu_rows, n_it, ny)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -130,6 +130,25 @@ int arec_time_operator(arec_ctx *ctx, const arec_geom *geom, int which, const fl
 /* Largest value of x (reduction), e.g. to scale noise to 1% of max(b). */
 int arec_max(arec_ctx *ctx, const float *x_dev, int64_t n, float *out);
 
+/* ---- alignment ------------------------------------------------------ */
+/* Correlation maps c = IDFT_unnorm(conj(DFT x) DFT y) of nviews view pairs
+ * [ny][nx] (ccorr2d, align2dstack.cpp:480-521). */
+int arec_align_ccorr(arec_ctx *ctx, const float *x_dev, const float *y_dev, int32_t nx, int32_t ny,
+                     int32_t nviews, float *c_dev);
+/* Per view: ccorr2d(projection, measurement) then speak's sub-pixel peak
+ * (arecCCImages, arecImage.cpp:384-434; speak/quad2dfit,
+ * align2dstack.cpp:1059-1166).  sx_dev, sy_dev: nviews floats each. */
+int arec_align_cc(arec_ctx *ctx, const float *proj_dev, const float *meas_dev, int32_t nx, int32_t ny,
+                  int32_t nviews, float *sx_dev, float *sy_dev);
+/* speak (align2dstack.cpp:1059-1105) on nviews maps [ny][nx]. */
+int arec_align_peak(arec_ctx *ctx, const float *c_dev, int32_t nx, int32_t ny, int32_t nviews,
+                    float *sx_dev, float *sy_dev);
+/* Fourier shift of each view by (sign*sx, sign*sy) (fshift2d,
+ * align2dstack.cpp:566-630); arecShiftImages (arecImage.cpp:653-671) is
+ * sign = -1. */
+int arec_align_shift(arec_ctx *ctx, float *data_dev, int32_t nx, int32_t ny, int32_t nviews,
+                     const float *sx_dev, const float *sy_dev, float sign);
+
 /* ---- synthetic inputs (no counterpart in the reference) --------------- */
 /* Sum of ellipsoids at the voxel centres of the compact cylinder of the
  * given radius and height ny, zero outside `support` (radius about the
@@ -158,6 +177,14 @@ int arec_h_backproject(int method, double radius, int32_t nx, int32_t ny, int32_
 int arec_h_solve(int solver, int method, const float *b, int32_t nx, int32_t ny, int32_t nz,
                  const float *angles, float lam, int32_t maxit, float tol, int32_t warm,
                  float *x, arec_hist *hist, int32_t hist_cap, int32_t *nhist, float *lam_out);
+
+/* arecCCImages (arecImage.cpp:384-434) on host stacks: -1 when the two
+ * stacks differ in size. */
+int arec_h_cc_images(int32_t nx1, int32_t ny1, int32_t nz1, const float *images1, int32_t nx2,
+                     int32_t ny2, int32_t nz2, const float *images2, float *sx, float *sy);
+/* arecShiftImages (arecImage.cpp:653-671): each view shifted by (-sx, -sy). */
+int arec_h_shift_images(int32_t nx, int32_t ny, int32_t nz, float *data, const float *sx,
+                        const float *sy);
 
 #ifdef __cplusplus
 }

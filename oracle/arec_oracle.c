@@ -224,6 +224,25 @@ int orc_cgls_stat_f(int method, const float *b, int nx, int ny, int na, const fl
     return st;
 }
 
+/* FP64 accuracy variant of the weighted CGLS: the same FP32 weights,
+ * FP64 storage and arithmetic everywhere else. */
+int orc_cgls_stat_d(int method, const double *b, int nx, int ny, int na, const float *angles,
+                    int maxit, double tol, int warm, double *x, orc_hist *hist, int hist_cap,
+                    int *nhist) {
+    size_t n = (size_t)nx * ny * na;
+    float *bf = (float *)malloc(sizeof(float) * n), *wf = (float *)malloc(sizeof(float) * n);
+    double *w = (double *)malloc(sizeof(double) * n);
+    for (size_t i = 0; i < n; i++) bf[i] = (float)b[i];
+    orc_init_weights_f(bf, nx, ny, na, wf);
+    for (size_t i = 0; i < n; i++) w[i] = wf[i];
+    int st = cgls_core_d(method, b, w, nx, ny, na, angles, (double)(nx / 2), maxit, tol, warm, x,
+                         hist, hist_cap, nhist);
+    free(bf);
+    free(wf);
+    free(w);
+    return st;
+}
+
 /* ------------------------------------------------------------------ */
 /* Alignment                                                           */
 /* ------------------------------------------------------------------ */

@@ -64,8 +64,8 @@ int orc_sirt_d(int method, const double *b, int nx, int ny, int na, const float 
 
 /* CGLS, AREC3D/AREC3D/areccgls.cpp:12-243 (and _SQ/_KB).  warm != 0 uses x
  * as the starting volume (reference: xcvol->data != NULL); otherwise x is
- * zeroed.  kb_radius_quirk != 0 reproduces cyl_cgls_KB's integer radius
- * nx/2 (AREC3D/AREC3D/areccgls_KB.cpp:39). */
+ * zeroed.  Method 2 (KB) uses cyl_cgls_KB's integer radius nx/2
+ * (AREC3D/AREC3D/areccgls_KB.cpp:39): even nx fails with status -1. */
 int orc_cgls_f(int method, const float *b, int nx, int ny, int na, const float *angles,
                int maxit, float tol, int warm, float *x, orc_hist *hist, int hist_cap,
                int *nhist);
@@ -79,6 +79,9 @@ int orc_cgls_d(int method, const double *b, int nx, int ny, int na, const float 
 int orc_init_weights_f(const float *b, int nx, int ny, int na, float *w);
 int orc_cgls_stat_f(int method, const float *b, int nx, int ny, int na, const float *angles,
                     int maxit, float tol, int warm, float *x, orc_hist *hist, int hist_cap,
+                    int *nhist);
+int orc_cgls_stat_d(int method, const double *b, int nx, int ny, int na, const float *angles,
+                    int maxit, double tol, int warm, double *x, orc_hist *hist, int hist_cap,
                     int *nhist);
 
 /* Alignment (FP64 arithmetic, FP32 interfaces like the reference). */

@@ -57,6 +57,7 @@ def lib():
         L.orc_cgls_d.argtypes = [C.c_int, _D, C.c_int, C.c_int, C.c_int, _F, C.c_int, C.c_double, C.c_int, _D,
                                  C.POINTER(Hist), C.c_int, C.POINTER(C.c_int)]
         L.orc_cgls_stat_f.argtypes = L.orc_cgls_f.argtypes
+        L.orc_cgls_stat_d.argtypes = L.orc_cgls_d.argtypes
         L.orc_init_weights_f.argtypes = [_F, C.c_int, C.c_int, C.c_int, _F]
         L.orc_ccorr2d.argtypes = [C.c_int, C.c_int, _F, _F, _D]
         L.orc_quad2dfit.argtypes = [_F, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -156,8 +157,10 @@ def cgls(method, b, angles, maxit, tol, x0=None, dtype=np.float32, weighted=Fals
     x = np.zeros((nrays, ny), dtype) if x0 is None else np.array(x0, dtype, copy=True)
     H = (Hist * (maxit + 4))()
     nh = C.c_int(0)
-    if weighted:
+    if weighted and dtype == np.float32:
         f, bb = lib().orc_cgls_stat_f, np.ascontiguousarray(b, np.float32)
+    elif weighted:
+        f, bb = lib().orc_cgls_stat_d, np.ascontiguousarray(b, np.float64)
     elif dtype == np.float32:
         f, bb = lib().orc_cgls_f, np.ascontiguousarray(b, np.float32)
     else:

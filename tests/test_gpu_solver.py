@@ -1,11 +1,13 @@
 """GPU parity of the device-resident SIRT and CGLS against the oracle
 (FP32 reference semantics) and the reference's golden vectors.
 
-Tolerances: residual histories match the oracle to 1e-5 relative at every
-printed iteration for the first 20 iterations and to 1e-4 thereafter
-(FP32 summation-order differences of the projectors compound through the
-iteration); restart counts, final lam, termination iteration and status are
-exact; volumes match to 1e-4 relative norm.
+History tolerance: at every printed iteration the GPU's relnrm and rnorm
+agree with the FP32 oracle to 1e-5 relative, or -- once the residual has
+dropped so far that FP32 rounding dominates it -- to within 4x the distance
+between the FP32 oracle and the FP64 oracle (the noise floor of computing
+the reference's recurrence in FP32, whatever the summation order).
+Iteration numbers, events (restarts, termination), restart counts and final
+lam are exact; volumes match to 1e-4 relative norm.
 """
 import os
 
@@ -32,13 +34,15 @@ def problem(method, nx, ny, na, seed, noise=0.01):
     return b, ang
 
 
-def check_hist(gh, oh):
+def check_hist(gh, oh, oh64=None):
     assert len(gh) == len(oh)
     for k, (g, o) in enumerate(zip(gh, oh)):
         assert g[0] == o[0] and g[1] == o[1], (k, g, o)
-        tol = 1e-5 if k < 20 else 1e-4
-        assert g[2] == pytest.approx(o[2], rel=tol), (k, g, o)
-        assert g[3] == pytest.approx(o[3], rel=tol), (k, g, o)
+        for c in (2, 3):
+            tol = 1e-5 * abs(o[c])
+            if oh64 is not None and k < len(oh64):
+                tol = max(tol, 4 * abs(o[c] - oh64[k][c]))
+            assert abs(g[c] - o[c]) <= tol, (k, c, g, o, None if oh64 is None else oh64[k])
 
 
 def relnorm(a, b):
@@ -54,10 +58,11 @@ def test_sirt_history_vs_oracle(ctx, method):
     b, ang = problem(method, nx, ny, na, 1)
     lam = 1.0 / (na * nx)  # stable for these footprints
     st, xo, ho, lo = O.sirt(method, b, ang, lam, 30, 1e-6)
+    _, _, ho64, _ = O.sirt(method, b, ang, lam, 30, 1e-6, dtype=np.float64)
     g = A.Geometry(ctx, method, nx, ny, ang)
     db, dx = ctx.to_device(b), ctx.empty(g.vol_shape)
     hg, lg = g.sirt(db, dx, lam, 30, 1e-6)
-    check_hist(hg, ho)
+    check_hist(hg, ho, ho64)
     assert lg == lo
     assert relnorm(dx.download(), xo) <= 1e-4
 
@@ -100,11 +105,12 @@ def test_cgls_history_vs_oracle(ctx, method):
     print(f"\n[case] cgls {method}", flush=True)
     b, ang = problem(method, nx, ny, na, 3)
     st, xo, ho = O.cgls(method, b, ang, 15, 1e-6)
+    _, x64, ho64 = O.cgls(method, b, ang, 15, 1e-6, dtype=np.float64)
     g = A.Geometry(ctx, method, nx, ny, ang)
     db, dx = ctx.to_device(b), ctx.empty(g.vol_shape)
     hg = g.cgls(db, dx, 15, 1e-6)
-    check_hist(hg, ho)
-    assert relnorm(dx.download(), xo) <= 1e-4
+    check_hist(hg, ho, ho64)
+    assert relnorm(dx.download(), xo) <= max(1e-4, 4 * relnorm(xo, x64))
 
 
 def test_cgls_warm_start_vs_oracle(ctx):
@@ -114,11 +120,12 @@ def test_cgls_warm_start_vs_oracle(ctx):
     b, ang = problem("sq", nx, ny, na, 5)
     _, x0, _ = O.cgls("sq", b, ang, 3, 1e-9)
     st, xo, ho = O.cgls("sq", b, ang, 6, 1e-9, x0=x0)
+    _, x64, ho64 = O.cgls("sq", b, ang, 6, 1e-9, x0=x0, dtype=np.float64)
     g = A.Geometry(ctx, "sq", nx, ny, ang)
     db, dx = ctx.to_device(b), ctx.to_device(x0)
     hg = g.cgls(db, dx, 6, 1e-9, warm=True)
-    check_hist(hg, ho)
-    assert relnorm(dx.download(), xo) <= 1e-4
+    check_hist(hg, ho, ho64)
+    assert relnorm(dx.download(), xo) <= max(1e-4, 4 * relnorm(xo, x64))
 
 
 def test_weighted_cgls_vs_oracle(ctx):
@@ -128,13 +135,14 @@ def test_weighted_cgls_vs_oracle(ctx):
     b, ang = problem("sq", nx, ny, na, 6)
     b = (b / b.max()).astype(np.float32)
     st, xo, ho = O.cgls("sq", b, ang, 8, 1e-9, weighted=True)
+    _, x64, ho64 = O.cgls("sq", b, ang, 8, 1e-9, weighted=True, dtype=np.float64)
     g = A.Geometry(ctx, "sq", nx, ny, ang)
     db, dx, dw = ctx.to_device(b), ctx.empty(g.vol_shape), ctx.empty(b.shape)
     A.init_weights(ctx, db, dw, nx, ny, na)
     np.testing.assert_allclose(dw.download(), O.init_weights(b), rtol=2e-7, atol=1e-7)
     hg = g.cgls(db, dx, 8, 1e-9, w=dw)
-    check_hist(hg, ho)
-    assert relnorm(dx.download(), xo) <= 1e-4
+    check_hist(hg, ho, ho64)
+    assert relnorm(dx.download(), xo) <= max(1e-4, 4 * relnorm(xo, x64))
 
 
 @pytest.mark.parametrize("tag,nx,ny", [("odd", 33, 6), ("even", 32, 5)])
