@@ -66,6 +66,7 @@ def _load():
         "arec_soft_noneg": ([vp, vp, i64], C.c_int),
         "arec_dot": ([vp, vp, vp, i64, P(dbl)], C.c_int),
         "arec_max": ([vp, vp, i64, P(flt)], C.c_int),
+        "arec_copy": ([vp, vp, vp, i64], C.c_int),
         "arec_time_operator": ([vp, vp, C.c_int, vp, vp, i32, P(flt)], C.c_int),
         "arec_phantom": ([vp, dbl, i64, dbl, vp, i32, vp], C.c_int),
         "arec_add_noise": ([vp, vp, i64, C.c_uint64, i32, dbl, dbl, dbl], C.c_int),
@@ -93,7 +94,7 @@ EXPORTED = [
     "arec_dev_alloc", "arec_dev_free", "arec_memcpy_h2d", "arec_memcpy_d2h", "arec_memset",
     "arec_cyl_nrays", "arec_geom_create", "arec_geom_destroy", "arec_geom_info",
     "arec_geom_device_bytes", "arec_project", "arec_backproject", "arec_sirt", "arec_cgls",
-    "arec_init_weights", "arec_soft_noneg", "arec_dot", "arec_max", "arec_time_operator", "arec_phantom",
+    "arec_init_weights", "arec_soft_noneg", "arec_dot", "arec_max", "arec_copy", "arec_time_operator", "arec_phantom",
     "arec_add_noise", "arec_align_ccorr", "arec_align_cc", "arec_align_peak", "arec_align_shift",
     "arec_h_cc_images", "arec_h_shift_images", "arec_h_project", "arec_h_backproject",
     "arec_h_solve",
@@ -330,6 +331,10 @@ def arecShiftImages(images, sx, sy):
     ay = np.ascontiguousarray(sy, np.float32)
     _check(lib.arec_h_shift_images(images.nx, images.ny, images.nz, _ptr(d), _ptr(ax), _ptr(ay)))
     images.data[...] = d
+
+
+def copy(ctx, dst, src):
+    _check(lib.arec_copy(ctx.h, dst.ptr, src.ptr, src.size))
 
 
 def reduce_max(ctx, a):

@@ -28,6 +28,7 @@ int init_weights(const float *b, float *w, int64_t nx, int64_t ny, int64_t nz, h
 int soft_noneg(float *x, int64_t n, hipStream_t st);
 int dot(const float *a, const float *b, int64_t n, double *out, hipStream_t st);
 int reduce_max(const float *x, int64_t n, float *out, hipStream_t st);
+int copy(float *dst, const float *src, int64_t n, hipStream_t st);
 
 static bool use_graphs() {
     const char *e = getenv("AREC_NO_GRAPH");
@@ -233,6 +234,11 @@ int arec_time_operator(arec_ctx *ctx, const arec_geom *geom, int which, const fl
     hipEventDestroy(e1);
     *ms_per_launch = ms / reps;
     return rc;
+}
+
+int arec_copy(arec_ctx *ctx, float *dst_dev, const float *src_dev, int64_t n) {
+    CHECK_ARG(ctx && dst_dev && src_dev && n >= 0, "bad argument");
+    return copy(dst_dev, src_dev, n, ctx->stream);
 }
 
 int arec_max(arec_ctx *ctx, const float *x_dev, int64_t n, float *out) {

@@ -569,6 +569,12 @@ __global__ void init_weights_kernel(const float *__restrict__ b, float *__restri
     }
 }
 
+int copy(float *dst, const float *src, int64_t n, hipStream_t st) {
+    copy_kernel<<<vec_grid(n), kVecThreads, 0, st>>>(dst, src, n);
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
 int init_weights(const float *b, float *w, int64_t nx, int64_t ny, int64_t nz, hipStream_t st) {
     int64_t n = nx * ny * nz;
     init_weights_kernel<<<vec_grid(n), kVecThreads, 0, st>>>(b, w, nx, n);

@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--maxit", type=int, default=3000)
     ap.add_argument("--cpu-rows", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="profiling run: 3 forward + 3 back launches and a 1 GiB copy kernel, no solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -114,6 +116,16 @@ def main():
     log(f"geometry {t_geom:.2f}s, sigma_max^2 {sig2:.4e}, lam {lam:.4e}")
 
     x = ctx.empty(g.vol_shape)
+    if args.kernels_only:
+        for _ in range(3):
+            g.project(x_true, b)
+            g.backproject(b, x)
+        n_copy = 1 << 28  # 1 GiB read + 1 GiB written
+        src, dst = ctx.empty(n_copy), ctx.empty(n_copy)
+        A.copy(ctx, dst, src)
+        ctx.synchronize()
+        log(f"kernels-only: 3 x forward, 3 x back, copy of {n_copy * 4} bytes")
+        return
     for _ in range(args.warmup):
         g.sirt(b, x, lam, 3, args.tol)
     ctx.synchronize()
@@ -175,6 +187,11 @@ def main():
             "model": "algorithmic bytes per launch 4*na*(nnz + nx*ny) (per-view streaming)",
         },
     }
+    pmc = os.path.join(ROOT, "profiles", f"pmc_sq_{nx}x{nx}x{ny}_{na}.json")
+    if os.path.exists(pmc):
+        t = json.load(open(pmc)).get(dom)
+        if t:
+            out["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nx, na, args.cpu_rows, n_it, ny)
     print(json.dumps(out), flush=True)

@@ -127,6 +127,10 @@ int arec_dot(arec_ctx *ctx, const float *a_dev, const float *b_dev, int64_t n, d
 int arec_time_operator(arec_ctx *ctx, const arec_geom *geom, int which, const float *in_dev,
                        float *out_dev, int32_t reps, float *ms_per_launch);
 
+/* dst = src, n floats, by a streaming copy kernel (also the reference
+ * kernel for calibrating HBM byte counters). */
+int arec_copy(arec_ctx *ctx, float *dst_dev, const float *src_dev, int64_t n);
+
 /* Largest value of x (reduction), e.g. to scale noise to 1% of max(b). */
 int arec_max(arec_ctx *ctx, const float *x_dev, int64_t n, float *out);
 

@@ -162,8 +162,10 @@ def test_solvers_vs_reference_golden(ctx, tag, nx, ny):
         xv = A.ArecImage()
         st = cgls(None, img, ang, xv, 0.0, 8, 1e-6)
         assert st == G[f"{tag}_{m}_cgls_status"]  # KB with even nx: -1 (F8)
-        if st == 0:
-            assert relnorm(xv.data, G[f"{tag}_{m}_cgls"]) <= 1e-4
+        if st == 0:  # CGLS amplifies FP32 rounding: bound by the FP32-vs-FP64 spread
+            _, x64, _ = O.cgls(m, b, ang, 8, 1e-6, dtype=np.float64)
+            ref = G[f"{tag}_{m}_cgls"]
+            assert relnorm(xv.data, ref) <= max(1e-4, 4 * relnorm(ref, x64))
 
 
 def test_graph_and_eager_agree(ctx, monkeypatch):
