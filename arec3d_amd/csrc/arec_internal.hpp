@@ -25,10 +25,10 @@ struct AngleConst {
     double a, b, lmax;
 };
 
-// Columns of one back-projection tile: up to 64 voxel columns whose
-// coordinates lie in one 8x8 box.
-constexpr int kTileCols = 64;
-constexpr int kTileBox = 8;
+// Columns of one back-projection tile: up to 256 voxel columns whose
+// coordinates lie in one 16x16 box (one column per thread).
+constexpr int kTileCols = 256;
+constexpr int kTileBox = 16;
 
 // Entry of the forward table: voxel columns of one angle in order of their
 // first detector tap.  x = column index (low 22 bits) | increase of the first
@@ -37,6 +37,8 @@ constexpr int kColBits = 22;
 constexpr int kMaxCols = 1 << kColBits;
 constexpr int kMaxTapStep = 1023;
 constexpr int kTapBase = -4;  // first tap of the virtual entry before entry 0
+constexpr int kFwdAngles = 8;  // angles (wavefronts) per forward workgroup
+constexpr int kFwdOut = 8;     // pixels per staged output segment (32 bytes)
 
 struct Geometry {
     int method = kSQ;
@@ -60,6 +62,11 @@ struct Geometry {
     int *d_tile_cols = nullptr;
     int *d_tile_org = nullptr;
     uint4 *d_fwd = nullptr;       // na * nvalid entries
+    // forward sweep split into detector segments: per (angle, segment)
+    // {first entry, end entry, first tap of the entry before the first}
+    int nseg = 1, segw = 0, fwd_rpl = 1, fwd_chunks = 1;
+    std::vector<int> seg;         // na * nseg * 4
+    int *d_seg = nullptr;
     int *d_err = nullptr;         // geometry self-check flag
     bool device_ready = false;
 };
@@ -71,7 +78,8 @@ int geometry_build_host(Geometry &g, int method, int nx, int ny, double radius,
 // First detector tap of column j at angle a, the same arithmetic as the
 // device weight function (used to order the forward table).
 int host_first_tap(const Geometry &g, int a, int j);
-void geometry_sort_angles(const Geometry &g, int a0, int a1, std::vector<int> &order);
+void geometry_sort_angles(Geometry &g, int a0, int a1, std::vector<int> &order);
+void geometry_plan_forward(Geometry &g);
 
 // Device setup and operators (projector.hip).
 int geometry_upload(Geometry &g, hipStream_t st, std::string *err);

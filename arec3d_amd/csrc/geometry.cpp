@@ -132,9 +132,28 @@ int host_first_tap(const Geometry &g, int a, int j) {
     return (int)std::nearbyint(xb) - 1;
 }
 
+// Launch shape of the forward projector: rows per lane (4 with 16-byte
+// loads when the row count allows), row chunks of 64 lanes, and enough
+// detector segments per angle that the grid has >= 1024 workgroups of
+// kFwdAngles waves.
+void geometry_plan_forward(Geometry &g) {
+    g.fwd_rpl = (g.ny >= 256 && g.ny % 4 == 0) ? 4 : (g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1;
+    g.fwd_chunks = (g.ny + 64 * g.fwd_rpl - 1) / (64 * g.fwd_rpl);
+    int64_t groups = (int64_t)((g.na + kFwdAngles - 1) / kFwdAngles) * g.fwd_chunks;
+    int nseg = (int)std::max<int64_t>(1, (1024 + groups - 1) / groups);
+    nseg = std::min(nseg, std::max(1, g.nx / (4 * kFwdOut)));
+    int segw = (g.nx + nseg - 1) / nseg;
+    segw = (segw + kFwdOut - 1) / kFwdOut * kFwdOut;
+    g.segw = segw;
+    g.nseg = (g.nx + segw - 1) / segw;
+    g.seg.assign((size_t)g.na * g.nseg * 4, 0);
+}
+
 // Stable counting sort of the contributing columns of angles [a0, a1) by
-// their first tap; order receives (a1-a0)*nvalid column indices.
-void geometry_sort_angles(const Geometry &g, int a0, int a1, std::vector<int> &order) {
+// their first tap; order receives (a1-a0)*nvalid column indices.  Also
+// fills the angles' segment table: segment s (pixels [s*segw, (s+1)*segw))
+// needs the entries whose first tap lies in [s*segw - 2, (s+1)*segw).
+void geometry_sort_angles(Geometry &g, int a0, int a1, std::vector<int> &order) {
     const int nv = g.nvalid;
     order.resize((size_t)(a1 - a0) * nv);
     unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -154,6 +173,19 @@ void geometry_sort_angles(const Geometry &g, int a0, int a1, std::vector<int> &o
             for (size_t k = 1; k < count.size(); k++) count[k] += count[k - 1];
             int *out = order.data() + (size_t)(a - a0) * nv;
             for (int j = 0; j < nv; j++) out[count[key[j] - kmin]++] = j;
+            // segment bounds from the sorted keys
+            int i = 0;
+            for (int sgi = 0; sgi < g.nseg; sgi++) {
+                int lo = sgi * g.segw - 2, hi = (sgi + 1) * g.segw;
+                while (i < nv && key[out[i]] < lo) i++;
+                int start = i, end = start;
+                while (end < nv && key[out[end]] < hi) end++;
+                int *e = &g.seg[((size_t)a * g.nseg + sgi) * 4];
+                e[0] = start;
+                e[1] = end;
+                e[2] = start > 0 ? key[out[start - 1]] : kTapBase;
+                e[3] = 0;
+            }
         }
     };
     std::vector<std::thread> pool;

@@ -97,14 +97,15 @@ __device__ __forceinline__ Taps tap_weights(int X, int Z, const AngleConst &A, d
         t.w2 = kb_interp((float)(xb - (double)(iqx + 1)));
         return t;
     }
+    // |xv| <= 1/2 and the footprint half-width b <= 0.7072, so the CDF at
+    // xv - 1.5 is exactly 0 and at xv + 1.5 exactly 1: w0 = v1 - 0 and
+    // w2 = 1 - v2 are the reference's differences bit for bit.
     double xv = (double)iqx - xb;
-    double v0 = trap_cdf(xv - 1.5, A.a, A.b, A.lmax);
     double v1 = trap_cdf(xv - 0.5, A.a, A.b, A.lmax);
     double v2 = trap_cdf(xv + 0.5, A.a, A.b, A.lmax);
-    double v3 = trap_cdf(xv + 1.5, A.a, A.b, A.lmax);
-    t.w0 = (float)(v1 - v0);
+    t.w0 = (float)v1;
     t.w1 = (float)(v2 - v1);
-    t.w2 = (float)(v3 - v2);
+    t.w2 = (float)(1.0 - v2);
     return t;
 }
 
@@ -146,131 +147,178 @@ __global__ void build_fwd_table(const int *__restrict__ order, int a0, int na_ba
 // ---------------------------------------------------------------------------
 // Forward projector
 // ---------------------------------------------------------------------------
-constexpr int kFwdWaves = 4;  // angles per workgroup
+constexpr int kFwdBatch = 8;  // table entries whose loads are in flight together
 
-// Store pixel q of the wave's rows (q outside the detector is dropped).
+// One wavefront: angle a, rows y0 + lane*RPL + r (r < RPL), detector pixels
+// [u_lo, u_hi).  It walks the angle's table entries whose first tap lies in
+// [u_lo - 2, u_hi) in tap order, keeping the three pixels base..base+2 in
+// registers.  A pixel is complete when the walk moves past it; completed
+// pixels go to an LDS strip of kFwdOut columns which is written to global
+// memory as 32-byte row segments.  partials (optional) receives the FP64 sum
+// of the FP32 squares of the wave's stored pixels.
 template <int RPL>
-__device__ __forceinline__ void fwd_store(float *__restrict__ pa, const float *v, int q, int nx,
-                                          int ny, int y0, double &sq) {
-    if (q < 0 || q >= nx) return;
-#pragma unroll
-    for (int r = 0; r < RPL; r++) {
-        int y = y0 + r;
-        if (y < ny) {
-            pa[(size_t)y * nx + q] = v[r];
-            float s = v[r] * v[r];
-            sq += (double)s;
-        }
-    }
-}
-
-// p[a][y][:] for the wave's angle and rows.  partials (optional) receives one
-// FP64 sum of FP32 squares of the written values per wave.
-template <int RPL>
-__global__ __launch_bounds__(64 * kFwdWaves) void forward_kernel(
-    const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ table, int nvalid,
-    int nx, int ny, int na, int nchunks, double *__restrict__ partials, const int *__restrict__ skip) {
+__global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(
+    const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ table,
+    const int4 *__restrict__ seg, int nvalid, int nseg, int segw, int nx, int ny, int na,
+    int nchunks, double *__restrict__ partials, const int *__restrict__ skip) {
     if (skip && *skip) return;
+    constexpr int ROWS = 64 * RPL;
+    constexpr int LDW = kFwdOut + 1;  // padded strip row (conflict-free column writes)
+    __shared__ float s_out[kFwdAngles][ROWS * LDW];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    // Workgroups with consecutive ids run on different XCDs; give each XCD a
-    // contiguous range of (row chunk, angle block) pairs, angle fastest, so
-    // that the workgroups resident on one XCD read the same rows of nearly
-    // the same columns (neighbouring angles sweep the disk almost together)
-    // and share its L2.
+    // Workgroups with consecutive ids run on different XCDs: give each XCD a
+    // contiguous range of (chunk, segment, angle group) triples, angle group
+    // fastest, so that the workgroups resident on one XCD sweep the same rows
+    // of nearly the same columns (neighbouring angles walk the disk almost
+    // together) and share its L2.
     const int nblk = gridDim.x;
     int b = blockIdx.x;
     if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
-    const int nablk = nblk / nchunks;
-    const int chunk = b / nablk;
-    const int a = (b % nablk) * kFwdWaves + wave;
+    const int nag = (na + kFwdAngles - 1) / kFwdAngles;
+    const int ag = b % nag;
+    const int rest = b / nag;
+    const int sg = rest % nseg;
+    const int chunk = rest / nseg;
+    const int a = ag * kFwdAngles + wave;
     if (a >= na) return;
-    const int y0 = chunk * 64 * RPL + lane * RPL;
+    const int u_lo = sg * segw, u_hi = min(nx, u_lo + segw);
+    const int ybase = chunk * ROWS;
+    const int y0 = ybase + lane * RPL;
     float *__restrict__ pa = p + (size_t)a * nx * ny;
     const uint4 *__restrict__ tab = table + (size_t)a * nvalid;
+    const int4 sd = seg[(size_t)a * nseg + sg];
+    const int i0 = __builtin_amdgcn_readfirstlane(sd.x), i1 = __builtin_amdgcn_readfirstlane(sd.y);
+    float *strip = s_out[wave];
 
     float acc0[RPL], acc1[RPL], acc2[RPL];
 #pragma unroll
     for (int r = 0; r < RPL; r++) acc0[r] = acc1[r] = acc2[r] = 0.0f;
     double sq = 0.0;
-    int base = kTapBase;  // window covers pixels base, base+1, base+2
+    int base = __builtin_amdgcn_readfirstlane(sd.z);  // window: pixels base..base+2
+    int strip_lo = u_lo;                                 // strip holds [strip_lo, +kFwdOut)
+    const bool full_rows = (ybase + ROWS <= ny);
 
-    for (int i = 0; i < nvalid; i++) {
-        uint4 e = tab[i];  // wave-uniform
-        int j = (int)(e.x & (kMaxCols - 1));
-        int step = (int)(e.x >> kColBits);
-        if (step) {
-            // pixels base .. base+step-1 are complete
-            if (step >= 3) {
-                fwd_store<RPL>(pa, acc0, base, nx, ny, y0, sq);
-                fwd_store<RPL>(pa, acc1, base + 1, nx, ny, y0, sq);
-                fwd_store<RPL>(pa, acc2, base + 2, nx, ny, y0, sq);
-                float z[RPL];
-#pragma unroll
-                for (int r = 0; r < RPL; r++) z[r] = 0.0f;
-                for (int q = base + 3; q < base + step; q++) fwd_store<RPL>(pa, z, q, nx, ny, y0, sq);
-#pragma unroll
-                for (int r = 0; r < RPL; r++) acc0[r] = acc1[r] = acc2[r] = 0.0f;
-            } else if (step == 1) {
-                fwd_store<RPL>(pa, acc0, base, nx, ny, y0, sq);
-#pragma unroll
-                for (int r = 0; r < RPL; r++) {
-                    acc0[r] = acc1[r];
-                    acc1[r] = acc2[r];
-                    acc2[r] = 0.0f;
-                }
-            } else {
-                fwd_store<RPL>(pa, acc0, base, nx, ny, y0, sq);
-                fwd_store<RPL>(pa, acc1, base + 1, nx, ny, y0, sq);
-#pragma unroll
-                for (int r = 0; r < RPL; r++) {
-                    acc0[r] = acc2[r];
-                    acc1[r] = 0.0f;
-                    acc2[r] = 0.0f;
-                }
-            }
-            base += step;
+    // write the strip [strip_lo, strip_lo + kFwdOut) of the wave's rows
+    auto flush_strip = [&]() {
+        __threadfence_block();
+        for (int e = lane; e < ROWS * kFwdOut; e += 64) {
+            int r = e / kFwdOut, c = e % kFwdOut;
+            int y = ybase + r, q = strip_lo + c;
+            if (y < ny && q < u_hi) pa[(size_t)y * nx + q] = strip[r * LDW + c];
         }
-        const float w0 = __uint_as_float(e.y), w1 = __uint_as_float(e.z), w2 = __uint_as_float(e.w);
-        const float *__restrict__ xj = x + (size_t)j * ny + y0;
-        float v[RPL];
-        if (y0 + RPL <= ny) {
-#pragma unroll
-            for (int r = 0; r < RPL; r++) v[r] = xj[r];
-        } else {
-#pragma unroll
-            for (int r = 0; r < RPL; r++) v[r] = (y0 + r < ny) ? xj[r] : 0.0f;
-        }
+        __threadfence_block();
+        strip_lo += kFwdOut;
+    };
+    // pixel q of the wave's rows is complete with values v
+    auto complete = [&](int q, const float *v) {
+        if (q < u_lo || q >= u_hi) return;
+        while (q >= strip_lo + kFwdOut) flush_strip();
 #pragma unroll
         for (int r = 0; r < RPL; r++) {
-            acc0[r] = fmaf(w0, v[r], acc0[r]);
-            acc1[r] = fmaf(w1, v[r], acc1[r]);
-            acc2[r] = fmaf(w2, v[r], acc2[r]);
+            strip[(lane * RPL + r) * LDW + (q - strip_lo)] = v[r];
+            if (y0 + r < ny) {
+                float t = v[r] * v[r];
+                sq += (double)t;
+            }
+        }
+    };
+    float zero[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; r++) zero[r] = 0.0f;
+    // advance the window so that it starts at pixel nb (nb >= base)
+    auto advance = [&](int nb) {
+        int step = nb - base;
+        if (step == 0) return;
+        if (step >= 3) {
+            complete(base, acc0);
+            complete(base + 1, acc1);
+            complete(base + 2, acc2);
+            for (int q = max(base + 3, u_lo); q < min(nb, u_hi); q++) complete(q, zero);
+#pragma unroll
+            for (int r = 0; r < RPL; r++) acc0[r] = acc1[r] = acc2[r] = 0.0f;
+        } else if (step == 1) {
+            complete(base, acc0);
+#pragma unroll
+            for (int r = 0; r < RPL; r++) {
+                acc0[r] = acc1[r];
+                acc1[r] = acc2[r];
+                acc2[r] = 0.0f;
+            }
+        } else {
+            complete(base, acc0);
+            complete(base + 1, acc1);
+#pragma unroll
+            for (int r = 0; r < RPL; r++) {
+                acc0[r] = acc2[r];
+                acc1[r] = 0.0f;
+                acc2[r] = 0.0f;
+            }
+        }
+        base = nb;
+    };
+
+    for (int i = i0; i < i1; i += kFwdBatch) {
+        uint4 e[kFwdBatch];
+#pragma unroll
+        for (int k = 0; k < kFwdBatch; k++) {
+            if (i + k < i1) {
+                e[k] = tab[i + k];
+            } else {
+                e[k] = make_uint4(0u, 0u, 0u, 0u);  // no column, no step, zero weights
+            }
+        }
+        float v[kFwdBatch][RPL];
+#pragma unroll
+        for (int k = 0; k < kFwdBatch; k++) {
+            const int j = (int)(e[k].x & (kMaxCols - 1));
+            const float *__restrict__ xj = x + (size_t)j * ny + y0;
+            if (RPL == 4 && full_rows) {
+                float4 t = *reinterpret_cast<const float4 *>(xj);
+                v[k][0] = t.x;
+                v[k][RPL > 1 ? 1 : 0] = t.y;
+                v[k][RPL > 2 ? 2 : 0] = t.z;
+                v[k][RPL > 3 ? 3 : 0] = t.w;
+            } else if (RPL == 2 && full_rows) {
+                float2 t = *reinterpret_cast<const float2 *>(xj);
+                v[k][0] = t.x;
+                v[k][RPL > 1 ? 1 : 0] = t.y;
+            } else {
+#pragma unroll
+                for (int r = 0; r < RPL; r++) v[k][r] = (y0 + r < ny) ? xj[r] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kFwdBatch; k++) {
+            const int step = (int)(e[k].x >> kColBits);
+            if (step) advance(base + step);
+            const float w0 = __uint_as_float(e[k].y), w1 = __uint_as_float(e[k].z),
+                        w2 = __uint_as_float(e[k].w);
+#pragma unroll
+            for (int r = 0; r < RPL; r++) {
+                acc0[r] = fmaf(w0, v[k][r], acc0[r]);
+                acc1[r] = fmaf(w1, v[k][r], acc1[r]);
+                acc2[r] = fmaf(w2, v[k][r], acc2[r]);
+            }
         }
     }
-    fwd_store<RPL>(pa, acc0, base, nx, ny, y0, sq);
-    fwd_store<RPL>(pa, acc1, base + 1, nx, ny, y0, sq);
-    fwd_store<RPL>(pa, acc2, base + 2, nx, ny, y0, sq);
-    {
-        float z[RPL];
-#pragma unroll
-        for (int r = 0; r < RPL; r++) z[r] = 0.0f;
-        for (int q = std::max(base + 3, 0); q < nx; q++) fwd_store<RPL>(pa, z, q, nx, ny, y0, sq);
-    }
+    // the last window and the pixels above it up to the segment end
+    advance(max(base + 3, u_hi));
+    while (strip_lo < u_hi) flush_strip();
     if (partials) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
-        if (lane == 0) partials[(size_t)a * nchunks + chunk] = sq;
+        if (lane == 0) partials[((size_t)a * nchunks + chunk) * nseg + sg] = sq;
     }
 }
 
 // ---------------------------------------------------------------------------
 // Back projector
 // ---------------------------------------------------------------------------
-constexpr int kBpThreads = 256;
-constexpr int kBpRows = 64;        // rows per workgroup
-constexpr int kBpRowsPerThr = 16;  // rows per thread (4 row groups)
-constexpr int kBpAngBlk = 8;       // angles staged per LDS pass
-constexpr int kBpWin = 20;         // staged pixels per (row, angle)
+constexpr int kBpThreads = kTileCols;  // one voxel column per thread
+constexpr int kBpRows = 64;            // rows per workgroup (registers per thread)
+constexpr int kBpRowPad = kBpRows + 4; // LDS row stride (16-byte aligned)
+constexpr int kBpAngBlk = 4;           // angles staged per LDS pass
+constexpr int kBpWin = 32;             // staged pixels per (angle)
 
 template <int M>
 __device__ __forceinline__ double corner_xb(int X, int Z, const AngleConst &A, double radius,
@@ -280,7 +328,10 @@ __device__ __forceinline__ double corner_xb(int X, int Z, const AngleConst &A, d
     return (((double)X - radius) * A.c + ((double)Z - radius) * A.s) + radius;
 }
 
-// x[j][y] = sum_a sum_k w_k(a,j) p[a][y][p0+k].  Epilogue by mode:
+// x[j][y] = sum_a sum_k w_k(a,j) p[a][y][p0+k] for the tile's columns (one
+// per thread) and 64 rows.  Per block of angles the tile's footprint (<= 32
+// pixels x 64 rows per angle) is staged in LDS with rows fastest, so one
+// 16-byte LDS read gives 4 rows of one tap.  Epilogue by mode:
 //  kBpStore      out = acc
 //  kBpSirtGrad   out = bvec - acc (FP32), partial += sum out*(double)out
 //  kBpStoreNorm  out = acc, partial += sum (double)(acc*acc)
@@ -291,27 +342,24 @@ __global__ __launch_bounds__(kBpThreads) void backward_kernel(
     const AngleConst *__restrict__ ang, double radius, int xcent, int nx, int ny, int na, int nrowblk,
     double *__restrict__ partials, int *__restrict__ err, const int *__restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ float s_p[kBpAngBlk][kBpRows][kBpWin];
-    __shared__ int s_p0[kBpAngBlk][kTileCols];
-    __shared__ float s_w[kBpAngBlk][kTileCols][3];
+    __shared__ __align__(16) float s_p[kBpAngBlk][kBpWin][kBpRowPad];
     __shared__ int s_u0[kBpAngBlk];
     __shared__ double s_red[kBpThreads / 64];
 
     const int tid = threadIdx.x;
-    const int col = tid & 63, rg = tid >> 6;
     const int tile = blockIdx.x / nrowblk;
     const int rb = blockIdx.x % nrowblk;
     const int ybase = rb * kBpRows;
-    const int j = tile_cols[tile * kTileCols + col];
+    const int j = tile_cols[tile * kTileCols + tid];
     const int X0 = tile_org[2 * tile], Z0 = tile_org[2 * tile + 1];
     int cX = 0, cZ = 0;
     if (j >= 0) {
         cX = coord[2 * j];
         cZ = coord[2 * j + 1];
     }
-    float acc[kBpRowsPerThr];
+    float4 acc[kBpRows / 4];
 #pragma unroll
-    for (int r = 0; r < kBpRowsPerThr; r++) acc[r] = 0.0f;
+    for (int q = 0; q < kBpRows / 4; q++) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
 
     for (int a0 = 0; a0 < na; a0 += kBpAngBlk) {
         const int nb = min(kBpAngBlk, na - a0);
@@ -332,46 +380,49 @@ __global__ __launch_bounds__(kBpThreads) void backward_kernel(
             if ((int)floor(hi) + 3 - u0 >= kBpWin) atomicOr(err, 2);
             s_u0[tid] = u0;
         }
-        // taps of the 64 columns for the staged angles
-        for (int e = tid; e < nb * kTileCols; e += kBpThreads) {
-            int ab = e / kTileCols, c = e % kTileCols;
-            int jj = tile_cols[tile * kTileCols + c];
-            Taps t = {0, 0.f, 0.f, 0.f};
-            if (jj >= 0) t = tap_weights<M>(coord[2 * jj], coord[2 * jj + 1], ang[a0 + ab], radius, xcent);
-            s_p0[ab][c] = t.p0;
-            s_w[ab][c][0] = t.w0;
-            s_w[ab][c][1] = t.w1;
-            s_w[ab][c][2] = t.w2;
+        // this column's taps for the staged angles (registers)
+        Taps t[kBpAngBlk];
+#pragma unroll
+        for (int ab = 0; ab < kBpAngBlk; ab++) {
+            t[ab] = Taps{0, 0.f, 0.f, 0.f};
+            if (j >= 0 && ab < nb) t[ab] = tap_weights<M>(cX, cZ, ang[a0 + ab], radius, xcent);
         }
         __syncthreads();
-        // stage the footprints
-        for (int e = tid; e < nb * kBpRows * kBpWin; e += kBpThreads) {
-            int ab = e / (kBpRows * kBpWin);
-            int rem = e % (kBpRows * kBpWin);
-            int row = rem / kBpWin, w = rem % kBpWin;
-            int y = ybase + row, u = s_u0[ab] + w;
-            float v = 0.0f;
-            if (y < ny && u >= 0 && u < nx) v = p[((size_t)(a0 + ab) * ny + y) * nx + u];
-            s_p[ab][row][w] = v;
+        // stage the footprints, rows fastest in LDS
+        for (int ab = 0; ab < nb; ab++) {
+            const int u0 = s_u0[ab];
+            const float *pa = p + (size_t)(a0 + ab) * ny * nx;
+#pragma unroll
+            for (int e0 = 0; e0 < kBpRows * kBpWin; e0 += kBpThreads) {
+                int e = e0 + tid;
+                int row = e >> 5, w = e & (kBpWin - 1);
+                int y = ybase + row, u = u0 + w;
+                float v = 0.0f;
+                if (y < ny && u >= 0 && u < nx) v = pa[(size_t)y * nx + u];
+                s_p[ab][w][row] = v;
+            }
         }
         __syncthreads();
         if (j >= 0) {
             for (int ab = 0; ab < nb; ab++) {
-                int o = s_p0[ab][col] - s_u0[ab];
+                int o = t[ab].p0 - s_u0[ab];
                 if (o < 0 || o > kBpWin - 3) {  // outside the staged window
                     atomicOr(err, 4);
                     continue;
                 }
-                float w0 = s_w[ab][col][0], w1 = s_w[ab][col][1], w2 = s_w[ab][col][2];
-                const float *rowp = &s_p[ab][rg * kBpRowsPerThr][o];
+                const float w0 = t[ab].w0, w1 = t[ab].w1, w2 = t[ab].w2;
+                const float4 *c0 = reinterpret_cast<const float4 *>(&s_p[ab][o][0]);
+                const float4 *c1 = reinterpret_cast<const float4 *>(&s_p[ab][o + 1][0]);
+                const float4 *c2 = reinterpret_cast<const float4 *>(&s_p[ab][o + 2][0]);
 #pragma unroll
-                for (int r = 0; r < kBpRowsPerThr; r++) {
-                    const float *q = rowp + r * kBpWin;
-                    float v = acc[r];
-                    v = fmaf(w0, q[0], v);
-                    v = fmaf(w1, q[1], v);
-                    v = fmaf(w2, q[2], v);
-                    acc[r] = v;
+                for (int q = 0; q < kBpRows / 4; q++) {
+                    float4 v0 = c0[q], v1 = c1[q], v2 = c2[q];
+                    float4 a = acc[q];
+                    a.x = fmaf(w2, v2.x, fmaf(w1, v1.x, fmaf(w0, v0.x, a.x)));
+                    a.y = fmaf(w2, v2.y, fmaf(w1, v1.y, fmaf(w0, v0.y, a.y)));
+                    a.z = fmaf(w2, v2.z, fmaf(w1, v1.z, fmaf(w0, v0.z, a.z)));
+                    a.w = fmaf(w2, v2.w, fmaf(w1, v1.w, fmaf(w0, v0.w, a.w)));
+                    acc[q] = a;
                 }
             }
         }
@@ -380,27 +431,32 @@ __global__ __launch_bounds__(kBpThreads) void backward_kernel(
 
     double part = 0.0;
     if (j >= 0) {
-        float *o = out + (size_t)j * ny;
+        float *o = out + (size_t)j * ny + ybase;
+        const float *bv = bvec ? bvec + (size_t)j * ny + ybase : nullptr;
 #pragma unroll
-        for (int r = 0; r < kBpRowsPerThr; r++) {
-            int y = ybase + rg * kBpRowsPerThr + r;
-            if (y < ny) {
-                float v = acc[r];
-                if (MODE == kBpSirtGrad) {
-                    v = bvec[(size_t)j * ny + y] - v;
-                    part += (double)v * (double)v;
-                } else if (MODE == kBpStoreNorm) {
-                    float s = v * v;
-                    part += (double)s;
+        for (int q = 0; q < kBpRows / 4; q++) {
+            float v[4] = {acc[q].x, acc[q].y, acc[q].z, acc[q].w};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int yy = 4 * q + r;
+                if (ybase + yy < ny) {
+                    float u = v[r];
+                    if (MODE == kBpSirtGrad) {
+                        u = bv[yy] - u;
+                        part += (double)u * (double)u;
+                    } else if (MODE == kBpStoreNorm) {
+                        float s2 = u * u;
+                        part += (double)s2;
+                    }
+                    o[yy] = u;
                 }
-                o[y] = v;
             }
         }
     }
     if (MODE != kBpStore) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-        if ((tid & 63) == 0) s_red[rg] = part;
+        if ((tid & 63) == 0) s_red[tid >> 6] = part;
         __syncthreads();
         if (tid == 0) {
             double s = 0.0;
@@ -413,13 +469,7 @@ __global__ __launch_bounds__(kBpThreads) void backward_kernel(
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-static int fwd_rpl(int ny) { return ny >= 512 ? 2 : 1; }
-
-int64_t forward_partial_count(const Geometry &g) {
-    int rpl = fwd_rpl(g.ny);
-    int nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
-    return (int64_t)g.na * nchunks;
-}
+int64_t forward_partial_count(const Geometry &g) { return (int64_t)g.na * g.fwd_chunks * g.nseg; }
 
 int64_t backward_partial_count(const Geometry &g) {
     int nrowblk = (g.ny + kBpRows - 1) / kBpRows;
@@ -441,6 +491,8 @@ void geometry_free_device(Geometry &g) {
     hipFree(g.d_tile_org);
     hipFree(g.d_fwd);
     hipFree(g.d_err);
+    hipFree(g.d_seg);
+    g.d_seg = nullptr;
     g.d_ang = nullptr;
     g.d_coord = nullptr;
     g.d_tile_cols = nullptr;
@@ -460,6 +512,7 @@ int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
     AREC_HIP(hipMemsetAsync(g.d_err, 0, sizeof(int), st));
     size_t entries = (size_t)g.na * std::max(1, g.nvalid);
     AREC_HIP(hipMalloc((void **)&g.d_fwd, sizeof(uint4) * entries));
+    geometry_plan_forward(g);
     // Build the forward table in batches of angles: order on the host (stable
     // counting sort by first tap), weights on the device.
     int batch = (int)std::max<int64_t>(1, std::min<int64_t>(g.na, (int64_t)(64 << 20) / std::max(1, g.nvalid)));
@@ -483,6 +536,8 @@ int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
     }
     AREC_HIP(hipStreamSynchronize(st));
     hipFree(d_order);
+    if ((rc = upload(&g.d_seg, g.seg, st))) return rc;
+    AREC_HIP(hipStreamSynchronize(st));
     int herr = 0;
     AREC_HIP(hipMemcpy(&herr, g.d_err, sizeof(int), hipMemcpyDeviceToHost));
     if (herr) {
@@ -496,14 +551,19 @@ int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
 int launch_forward(const Geometry &g, const float *x, float *p, double *partials, hipStream_t st,
                    const int *skip) {
     if (g.nvalid == 0) return fill(p, 0.0f, g.nproj, st, skip);
-    int rpl = fwd_rpl(g.ny);
-    int nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
-    int ablk = (g.na + kFwdWaves - 1) / kFwdWaves;
-    dim3 grid(ablk * nchunks);
-    if (rpl == 2)
-        forward_kernel<2><<<grid, 64 * kFwdWaves, 0, st>>>(x, p, g.d_fwd, g.nvalid, g.nx, g.ny, g.na, nchunks, partials, skip);
+    int nag = (g.na + kFwdAngles - 1) / kFwdAngles;
+    dim3 grid((unsigned)(nag * g.nseg * g.fwd_chunks));
+    const int4 *seg = reinterpret_cast<const int4 *>(g.d_seg);
+#define AREC_FWD(R)                                                                                  \
+    forward_kernel<R><<<grid, 64 * kFwdAngles, 0, st>>>(x, p, g.d_fwd, seg, g.nvalid, g.nseg, g.segw, \
+                                                        g.nx, g.ny, g.na, g.fwd_chunks, partials, skip)
+    if (g.fwd_rpl == 4)
+        AREC_FWD(4);
+    else if (g.fwd_rpl == 2)
+        AREC_FWD(2);
     else
-        forward_kernel<1><<<grid, 64 * kFwdWaves, 0, st>>>(x, p, g.d_fwd, g.nvalid, g.nx, g.ny, g.na, nchunks, partials, skip);
+        AREC_FWD(1);
+#undef AREC_FWD
     AREC_HIP(hipGetLastError());
     return 0;
 }
