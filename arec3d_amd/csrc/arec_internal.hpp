@@ -39,6 +39,7 @@ constexpr int kMaxTapStep = 1023;
 constexpr int kTapBase = -4;  // first tap of the virtual entry before entry 0
 constexpr int kFwdAngles = 8;  // angles (wavefronts) per forward workgroup
 constexpr int kFwdOut = 8;     // pixels per staged output segment (32 bytes)
+constexpr int kFwdPhase = 2;   // detector pixels per lockstep phase of a forward workgroup
 
 struct Geometry {
     int method = kSQ;
@@ -67,6 +68,11 @@ struct Geometry {
     int nseg = 1, segw = 0, fwd_rpl = 1, fwd_chunks = 1;
     std::vector<int> seg;         // na * nseg * 4
     int *d_seg = nullptr;
+    // first table entry whose first tap is >= q, for q = kTapBase .. nx+3
+    // (index q - kTapBase); the forward kernel's lockstep phases use it
+    int npix = 0;
+    std::vector<int> pixstart;    // na * npix
+    int *d_pixstart = nullptr;
     int *d_err = nullptr;         // geometry self-check flag
     bool device_ready = false;
 };

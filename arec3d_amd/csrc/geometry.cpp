@@ -147,6 +147,8 @@ void geometry_plan_forward(Geometry &g) {
     g.segw = segw;
     g.nseg = (g.nx + segw - 1) / segw;
     g.seg.assign((size_t)g.na * g.nseg * 4, 0);
+    g.npix = g.nx + 8;
+    g.pixstart.assign((size_t)g.na * g.npix, 0);
 }
 
 // Stable counting sort of the contributing columns of angles [a0, a1) by
@@ -185,6 +187,12 @@ void geometry_sort_angles(Geometry &g, int a0, int a1, std::vector<int> &order) 
                 e[1] = end;
                 e[2] = start > 0 ? key[out[start - 1]] : kTapBase;
                 e[3] = 0;
+            }
+            int *ps = &g.pixstart[(size_t)a * g.npix];
+            int ii = 0;
+            for (int q = 0; q < g.npix; q++) {
+                while (ii < nv && key[out[ii]] < q + kTapBase) ii++;
+                ps[q] = ii;
             }
         }
     };

@@ -159,8 +159,9 @@ constexpr int kFwdBatch = 8;  // table entries whose loads are in flight togethe
 template <int RPL>
 __global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(
     const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ table,
-    const int4 *__restrict__ seg, int nvalid, int nseg, int segw, int nx, int ny, int na,
-    int nchunks, double *__restrict__ partials, const int *__restrict__ skip) {
+    const int4 *__restrict__ seg, const int *__restrict__ pixstart, int npix, int nvalid, int nseg,
+    int segw, int nx, int ny, int na, int nchunks, double *__restrict__ partials,
+    const int *__restrict__ skip) {
     if (skip && *skip) return;
     constexpr int ROWS = 64 * RPL;
     constexpr int LDW = kFwdOut + 1;  // padded strip row (conflict-free column writes)
@@ -179,8 +180,9 @@ __global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(
     const int rest = b / nag;
     const int sg = rest % nseg;
     const int chunk = rest / nseg;
-    const int a = ag * kFwdAngles + wave;
-    if (a >= na) return;
+    const int a_raw = ag * kFwdAngles + wave;
+    const bool active = a_raw < na;  // inactive waves still take part in the barriers
+    const int a = active ? a_raw : na - 1;
     const int u_lo = sg * segw, u_hi = min(nx, u_lo + segw);
     const int ybase = chunk * ROWS;
     const int y0 = ybase + lane * RPL;
@@ -257,11 +259,22 @@ __global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(
         base = nb;
     };
 
-    for (int i = i0; i < i1; i += kFwdBatch) {
+    // The workgroup's waves (neighbouring angles) sweep the segment in
+    // lockstep: phase k covers the entries whose first tap is below
+    // u_lo - 2 + k*kFwdPhase, then all waves meet at a barrier, so that they
+    // read nearly the same columns at nearly the same time.
+    const int *__restrict__ ps = pixstart + (size_t)a * npix;
+    const int nphase = (u_hi - (u_lo - 2) + kFwdPhase - 1) / kFwdPhase;
+    int i = i0;
+    for (int ph = 1; ph <= nphase; ph++) {
+        int thr = u_lo - 2 + ph * kFwdPhase;
+        int iend = ph == nphase ? i1 : min(i1, ps[min(npix - 1, max(0, thr - kTapBase))]);
+        iend = __builtin_amdgcn_readfirstlane(active ? iend : i);
+        for (; i < iend; i += kFwdBatch) {
         uint4 e[kFwdBatch];
 #pragma unroll
         for (int k = 0; k < kFwdBatch; k++) {
-            if (i + k < i1) {
+            if (i + k < iend) {
                 e[k] = tab[i + k];
             } else {
                 e[k] = make_uint4(0u, 0u, 0u, 0u);  // no column, no step, zero weights
@@ -300,7 +313,11 @@ __global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(
                 acc2[r] = fmaf(w2, v[k][r], acc2[r]);
             }
         }
+        }
+        i = iend;
+        __syncthreads();
     }
+    if (!active) return;
     // the last window and the pixels above it up to the segment end
     advance(max(base + 3, u_hi));
     while (strip_lo < u_hi) flush_strip();
@@ -404,7 +421,9 @@ __global__ __launch_bounds__(kBpThreads) void backward_kernel(
         }
         __syncthreads();
         if (j >= 0) {
-            for (int ab = 0; ab < nb; ab++) {
+#pragma unroll
+            for (int ab = 0; ab < kBpAngBlk; ab++) {
+                if (ab >= nb) break;
                 int o = t[ab].p0 - s_u0[ab];
                 if (o < 0 || o > kBpWin - 3) {  // outside the staged window
                     atomicOr(err, 4);
@@ -492,7 +511,9 @@ void geometry_free_device(Geometry &g) {
     hipFree(g.d_fwd);
     hipFree(g.d_err);
     hipFree(g.d_seg);
+    hipFree(g.d_pixstart);
     g.d_seg = nullptr;
+    g.d_pixstart = nullptr;
     g.d_ang = nullptr;
     g.d_coord = nullptr;
     g.d_tile_cols = nullptr;
@@ -537,6 +558,7 @@ int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
     AREC_HIP(hipStreamSynchronize(st));
     hipFree(d_order);
     if ((rc = upload(&g.d_seg, g.seg, st))) return rc;
+    if ((rc = upload(&g.d_pixstart, g.pixstart, st))) return rc;
     AREC_HIP(hipStreamSynchronize(st));
     int herr = 0;
     AREC_HIP(hipMemcpy(&herr, g.d_err, sizeof(int), hipMemcpyDeviceToHost));
@@ -555,8 +577,9 @@ int launch_forward(const Geometry &g, const float *x, float *p, double *partials
     dim3 grid((unsigned)(nag * g.nseg * g.fwd_chunks));
     const int4 *seg = reinterpret_cast<const int4 *>(g.d_seg);
 #define AREC_FWD(R)                                                                                  \
-    forward_kernel<R><<<grid, 64 * kFwdAngles, 0, st>>>(x, p, g.d_fwd, seg, g.nvalid, g.nseg, g.segw, \
-                                                        g.nx, g.ny, g.na, g.fwd_chunks, partials, skip)
+    forward_kernel<R><<<grid, 64 * kFwdAngles, 0, st>>>(x, p, g.d_fwd, seg, g.d_pixstart, g.npix,  \
+                                                        g.nvalid, g.nseg, g.segw, g.nx, g.ny, g.na, \
+                                                        g.fwd_chunks, partials, skip)
     if (g.fwd_rpl == 4)
         AREC_FWD(4);
     else if (g.fwd_rpl == 2)
