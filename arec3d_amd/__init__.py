@@ -76,6 +76,20 @@ def _load():
         "arec_align_shift": ([vp, vp, i32, i32, i32, vp, vp, flt], C.c_int),
         "arec_h_cc_images": ([i32, i32, i32, vp, i32, i32, i32, vp, vp, vp], C.c_int),
         "arec_h_shift_images": ([i32, i32, i32, vp, vp, vp], C.c_int),
+        "arec_sirt_warm": ([vp, vp, vp, vp, flt, i32, flt, P(Hist), i32, P(i32), P(flt)], C.c_int),
+        "arec_unet_create": ([vp, P(vp), i32, i32, i32, i32, C.c_uint64], C.c_int),
+        "arec_unet_destroy": ([vp], C.c_int),
+        "arec_unet_nparams": ([vp], i64),
+        "arec_unet_get_params": ([vp, vp], C.c_int),
+        "arec_unet_set_params": ([vp, vp], C.c_int),
+        "arec_unet_set_scale": ([vp, flt], C.c_int),
+        "arec_unet_forward": ([vp, vp, vp], C.c_int),
+        "arec_unet_train_step": ([vp, vp, vp, flt, P(dbl)], C.c_int),
+        "arec_unet_gradient": ([vp, vp, vp, P(dbl), vp], C.c_int),
+        "arec_conv3_check": ([vp, vp, i32, vp, i32, vp, vp, vp, i32, i32, i32, i32, i32], C.c_int),
+        "arec_conv3_wgrad_check": ([vp, vp, i32, vp, i32, vp, i32, i32, i32, i32, vp], C.c_int),
+        "arec_cyl_to_cube": ([vp, vp, dbl, i64, vp], C.c_int),
+        "arec_cube_to_cyl": ([vp, vp, dbl, i64, vp], C.c_int),
         "arec_h_project": ([C.c_int, dbl, i32, i32, vp, vp, i32, vp], C.c_int),
         "arec_h_backproject": ([C.c_int, dbl, i32, i32, i32, vp, vp, i32, vp], C.c_int),
         "arec_h_solve": ([C.c_int, C.c_int, vp, i32, i32, i32, vp, flt, i32, flt, i32, vp, P(Hist), i32, P(i32), P(flt)], C.c_int),
@@ -96,7 +110,10 @@ EXPORTED = [
     "arec_geom_device_bytes", "arec_project", "arec_backproject", "arec_sirt", "arec_cgls",
     "arec_init_weights", "arec_soft_noneg", "arec_dot", "arec_max", "arec_copy", "arec_time_operator", "arec_phantom",
     "arec_add_noise", "arec_align_ccorr", "arec_align_cc", "arec_align_peak", "arec_align_shift",
-    "arec_h_cc_images", "arec_h_shift_images", "arec_h_project", "arec_h_backproject",
+    "arec_h_cc_images", "arec_h_shift_images", "arec_sirt_warm", "arec_unet_create",
+    "arec_unet_destroy", "arec_unet_nparams", "arec_unet_get_params", "arec_unet_set_params",
+    "arec_unet_set_scale", "arec_unet_forward", "arec_unet_train_step", "arec_unet_gradient",
+    "arec_conv3_check", "arec_conv3_wgrad_check", "arec_cyl_to_cube", "arec_cube_to_cyl", "arec_h_project", "arec_h_backproject",
     "arec_h_solve",
 ]
 
@@ -252,6 +269,15 @@ class Geometry:
         H = (Hist * cap)()
         nh, lo = C.c_int32(), C.c_float()
         _check(lib.arec_sirt(self.ctx.h, self.h, b.ptr, x.ptr, lam, maxit, tol, H, cap, C.byref(nh), C.byref(lo)))
+        return _hist(H, nh.value), lo.value
+
+    def sirt_warm(self, b, x, lam, maxit, tol, hist_cap=None):
+        """SIRT continued from the volume in x (fixed ||A^T b|| denominator)."""
+        cap = hist_cap or (maxit + 64)
+        H = (Hist * cap)()
+        nh, lo = C.c_int32(), C.c_float()
+        _check(lib.arec_sirt_warm(self.ctx.h, self.h, b.ptr, x.ptr, lam, maxit, tol, H, cap, C.byref(nh),
+                                  C.byref(lo)))
         return _hist(H, nh.value), lo.value
 
     def cgls(self, b, x, maxit, tol, warm=False, w=None, hist_cap=None):

@@ -21,7 +21,8 @@ int hip_fail(hipError_t e, const char *what) {
 
 // solver.hip
 int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float lam, int maxit,
-             float tol, arec_hist *hist, int hist_cap, int *nhist, float *lam_out, bool use_graph);
+             float tol, int warm, arec_hist *hist, int hist_cap, int *nhist, float *lam_out,
+             bool use_graph);
 int cgls_run(const Geometry &g, hipStream_t st, const float *b, const float *w, float *x, int warm,
              int maxit, float tol, arec_hist *hist, int hist_cap, int *nhist, bool use_graph);
 int init_weights(const float *b, float *w, int64_t nx, int64_t ny, int64_t nz, hipStream_t st);
@@ -183,7 +184,16 @@ int arec_sirt(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x
               float *lam_out) {
     CHECK_ARG(ctx && geom && b_dev && x_dev, "NULL argument");
     CHECK_ARG(hist_cap >= 0 && (hist || hist_cap == 0), "bad history buffer");
-    return sirt_run(geom->g, ctx->stream, b_dev, x_dev, lam, maxit, tol, hist, hist_cap, nhist,
+    return sirt_run(geom->g, ctx->stream, b_dev, x_dev, lam, maxit, tol, 0, hist, hist_cap, nhist,
+                    lam_out, use_graphs());
+}
+
+int arec_sirt_warm(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x_dev, float lam,
+                   int32_t maxit, float tol, arec_hist *hist, int32_t hist_cap, int32_t *nhist,
+                   float *lam_out) {
+    CHECK_ARG(ctx && geom && b_dev && x_dev, "NULL argument");
+    CHECK_ARG(hist_cap >= 0 && (hist || hist_cap == 0), "bad history buffer");
+    return sirt_run(geom->g, ctx->stream, b_dev, x_dev, lam, maxit, tol, 1, hist, hist_cap, nhist,
                     lam_out, use_graphs());
 }
 

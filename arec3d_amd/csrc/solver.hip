@@ -28,7 +28,7 @@ constexpr int kVecBlocks = 1024;
 
 struct SirtState {
     int iter, restarts, done, first, zero_x, update, skip_all, skip_proj;
-    int nhist, hist_cap, maxit, pad;
+    int nhist, hist_cap, maxit, warm;
     float lam, tol;
     double bnorm2, bnorm, relnrm0;
 };
@@ -108,7 +108,7 @@ __device__ __forceinline__ void push_hist(arec_hist *h, int cap, int *n, int ite
 
 __global__ void sirt_begin(SirtState *st) {
     st->skip_all = st->done;
-    st->skip_proj = st->done || st->first;
+    st->skip_proj = st->done || (st->first && !st->warm);
 }
 
 // First pass after the start or a restart (arecsirt.cpp:96-108):
@@ -119,13 +119,14 @@ __global__ __launch_bounds__(kVecThreads) void sirt_first_pass(const float *__re
                                                                double *__restrict__ part,
                                                                const SirtState *st) {
     if (st->skip_all || !st->first) return;
+    const int warm = st->warm;  // warm start: grad comes from the projection pass
     double sb = 0.0, sg = 0.0;
     GRID_STRIDE(i, n) {
         float b = bdata[i];
         float s = b * b;
         sb += (double)s;
         sg += (double)b * (double)b;
-        grad[i] = b;
+        if (!warm) grad[i] = b;
     }
     block_partial(sb, part);
     block_partial(sg, part + gridDim.x);
@@ -138,11 +139,11 @@ __global__ __launch_bounds__(kVecThreads) void sirt_control(SirtState *st,
                                                             const double *__restrict__ fp_part,
                                                             int n_fp, arec_hist *hist) {
     if (st->skip_all) return;
-    const int first = st->first;
+    const int first = st->first, warm = st->warm;
     double s_b = 0.0, s_g;
     if (first) {
         s_b = block_sum(fp_part, n_fp);
-        s_g = block_sum(fp_part + n_fp, n_fp);
+        s_g = warm ? block_sum(bp_part, n_bp) : block_sum(fp_part + n_fp, n_fp);
     } else {
         s_g = block_sum(bp_part, n_bp);
     }
@@ -151,6 +152,7 @@ __global__ __launch_bounds__(kVecThreads) void sirt_control(SirtState *st,
         st->bnorm2 += s_b;  // accumulates across restarts (:91,104)
         st->bnorm = sqrt(st->bnorm2);
     }
+    st->warm = 0;
     double rnorm = sqrt(s_g);
     double relnrm = rnorm / st->bnorm;
     st->zero_x = 0;
@@ -346,7 +348,8 @@ struct PassRunner {
 }  // namespace
 
 int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float lam, int maxit,
-             float tol, arec_hist *hist, int hist_cap, int *nhist, float *lam_out, bool use_graph) {
+             float tol, int warm, arec_hist *hist, int hist_cap, int *nhist, float *lam_out,
+             bool use_graph) {
     const int64_t nnz = g.nnz, nproj = g.nproj;
     DevBuf bdata, grad, proj, bp_part, fp_part, dstate, dhist;
     int rc;
@@ -366,10 +369,11 @@ int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float 
     s0.hist_cap = hist_cap;
     s0.relnrm0 = 1.00001;
     s0.done = maxit < 1;
+    s0.warm = warm != 0;
     SirtState *S = dstate.as<SirtState>();
     AREC_HIP(hipMemcpyAsync(S, &s0, sizeof s0, hipMemcpyHostToDevice, st));
-    // x = 0 (arecsirt.cpp:53-56); bdata = A^T b (:98)
-    if ((rc = fill(x, 0.0f, nnz, st, nullptr))) return rc;
+    // x = 0 (arecsirt.cpp:53-56) unless warm-started; bdata = A^T b (:98)
+    if (!warm && (rc = fill(x, 0.0f, nnz, st, nullptr))) return rc;
     if ((rc = launch_backward(g, b, bdata.as<float>(), kBpStore, nullptr, nullptr, st, nullptr))) return rc;
 
     const int fpG = (int)vec_grid(nnz).x;

@@ -1,0 +1,231 @@
+"""Learned warm start for SIRT: a 3-level 3-D U-Net run by the HIP kernels of
+unet.hip (no counterpart in the reference).
+
+* :class:`UNet` wraps one network instance for a fixed volume size; the
+  parameters are a flat FP32 vector shared between instances of different
+  sizes (train at 128^3, run at 512^3).
+* :func:`make_pairs` generates training pairs with the method itself: seeded
+  phantoms, their sinograms, and SIRT after ``k_in`` and ``k_out``
+  iterations.  Detector rows are independent, so ``stack`` phantoms are
+  stacked along the rows and solved as one problem (SIRT with a fixed lam
+  and no early termination is row-separable).
+* :func:`warm_sirt` runs SIRT for ``k_in`` iterations, applies the network,
+  keeps its output only if its residual is lower (SIRT remains the fallback
+  and judges the network), then continues SIRT to the tolerance.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import (ArecError, Context, DeviceArray, Geometry, Hist, _check, _hist, _ptr, dot, lib,
+               reduce_max)
+from . import phantom as P
+
+
+class UNet:
+    def __init__(self, ctx, D, H, W, training=False, seed=0, params=None, scale=1.0):
+        self.ctx = ctx
+        self.shape = (D, H, W)
+        self.h = C.c_void_p()
+        _check(lib.arec_unet_create(ctx.h, C.byref(self.h), D, H, W, int(training), seed))
+        self.nparams = int(lib.arec_unet_nparams(self.h))
+        if params is not None:
+            self.set_params(params)
+        self.set_scale(scale)
+
+    def set_scale(self, scale):
+        self.scale = float(scale)
+        _check(lib.arec_unet_set_scale(self.h, self.scale))
+
+    def get_params(self):
+        out = np.zeros(self.nparams, np.float32)
+        _check(lib.arec_unet_get_params(self.h, _ptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        _check(lib.arec_unet_set_params(self.h, _ptr(p)))
+
+    def forward(self, x_cube, y_cube):
+        _check(lib.arec_unet_forward(self.h, x_cube.ptr, y_cube.ptr))
+
+    def train_step(self, x_cube, t_cube, lr):
+        loss = C.c_double()
+        _check(lib.arec_unet_train_step(self.h, x_cube.ptr, t_cube.ptr, lr, C.byref(loss)))
+        return loss.value
+
+    def gradient(self, x_cube, t_cube):
+        loss = C.c_double()
+        g = np.zeros(self.nparams, np.float32)
+        _check(lib.arec_unet_gradient(self.h, x_cube.ptr, t_cube.ptr, C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def close(self):
+        if self.h:
+            lib.arec_unet_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cyl_to_cube(ctx, x, radius, ny, cube):
+    _check(lib.arec_cyl_to_cube(ctx.h, x.ptr, float(radius), int(ny), cube.ptr))
+
+
+def cube_to_cyl(ctx, cube, radius, ny, x):
+    _check(lib.arec_cube_to_cyl(ctx.h, cube.ptr, float(radius), int(ny), x.ptr))
+
+
+def sigma_max2(ctx, g, iters=20, seed=0):
+    """Largest eigenvalue of A^T A by power iteration (FP64 dots)."""
+    v = ctx.to_device(np.random.default_rng(seed).standard_normal(g.vol_shape).astype(np.float32))
+    w = ctx.empty(g.proj_shape)
+    v2 = ctx.empty(g.vol_shape)
+    lam = 0.0
+    for _ in range(iters):
+        n = np.sqrt(dot(ctx, v, v))
+        v.upload(v.download() / n)
+        g.project(v, w)
+        g.backproject(w, v2)
+        lam = dot(ctx, v, v2)
+        v, v2 = v2, v
+    return lam
+
+
+def phantom_params(seed, n, height):
+    """The eval generator of the 512^3 workload scaled to width n: support
+    radius 250/512 n, 40 ellipsoids, semi-axes U[8,120]/512 n, density
+    U[0.05, 1]."""
+    f = n / 512.0
+    return P.ellipsoid_params(seed, 40, 250.0 * f, height, axes=(8.0 * f, 120.0 * f), dens=(0.05, 1.0)), 250.0 * f
+
+
+def make_pairs(ctx, n_samples, n=128, na=360, k_in=5, k_out=200, stack=16, seed0=40000, noise=0.01,
+               log=print):
+    """Training pairs (SIRT after k_in and after k_out iterations) as FP32
+    cubes [n][n][n] on the host."""
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    ny = n * stack
+    g = Geometry(ctx, "sq", n, ny, ang)
+    lam = 0.9 / sigma_max2(ctx, g)
+    xs = ctx.empty(g.vol_shape)
+    x = ctx.empty(g.vol_shape)
+    b = ctx.empty(g.proj_shape)
+    cube = ctx.empty((ny, n, n))
+    xin = np.zeros((n_samples, n, n, n), np.float32)
+    xout = np.zeros((n_samples, n, n, n), np.float32)
+    tpl = np.zeros(g.vol_shape, np.float32)
+    t0 = time.time()
+    for s0 in range(0, n_samples, stack):
+        cnt = min(stack, n_samples - s0)
+        # phantoms for the stack, one height-n block each
+        for k in range(stack):
+            prm, sup = phantom_params(seed0 + s0 + k, n, n)
+            tmp = ctx.empty((g.nrays, n))
+            P.phantom_device(ctx, tmp, g.radius, n, sup, prm)
+            tpl[:, k * n:(k + 1) * n] = tmp.download()
+        xs.upload(tpl)
+        g.project(xs, b)
+        P.add_noise(ctx, b, seed0 + s0, rel_sigma=noise)
+        g.sirt(b, x, lam, k_in, 0.0)
+        cyl_to_cube(ctx, x, g.radius, ny, cube)
+        c_in = cube.download().reshape(stack, n, n, n)
+        g.sirt_warm(b, x, lam, k_out - k_in, 0.0)
+        cyl_to_cube(ctx, x, g.radius, ny, cube)
+        c_out = cube.download().reshape(stack, n, n, n)
+        xin[s0:s0 + cnt] = c_in[:cnt]
+        xout[s0:s0 + cnt] = c_out[:cnt]
+        if log and (s0 // stack) % 10 == 0:
+            log(f"pairs {s0 + cnt}/{n_samples} ({time.time() - t0:.1f}s)")
+    return xin, xout, lam
+
+
+def train(ctx, xin, xout, epochs=2, lr=1e-3, seed=0, log=print, scale=None):
+    """Adam on the MSE between net(x_in) and x_out; returns (params, scale, losses)."""
+    n = xin.shape[1]
+    net = UNet(ctx, n, n, n, training=True, seed=seed)
+    if scale is None:
+        scale = 1.0 / max(1e-12, float(np.std(xin)))
+    net.set_scale(scale)
+    dx, dt = ctx.empty(xin.shape[1:]), ctx.empty(xin.shape[1:])
+    rng = np.random.default_rng(seed)
+    losses = []
+    steps = epochs * len(xin)
+    t0 = time.time()
+    for e in range(epochs):
+        for i, k in enumerate(rng.permutation(len(xin))):
+            step = e * len(xin) + i
+            cur = lr * 0.5 * (1 + np.cos(np.pi * step / steps))  # cosine decay
+            dx.upload(xin[k])
+            dt.upload(xout[k])
+            losses.append(net.train_step(dx, dt, cur))
+            if log and step % 200 == 0:
+                log(f"step {step}/{steps} loss {np.mean(losses[-50:]):.4e} ({time.time() - t0:.1f}s)")
+    return net.get_params(), scale, losses
+
+
+def normal_residual(ctx, g, b, x, bnorm, tmp_p=None, tmp_x=None):
+    """||A^T (b - A x)|| / bnorm with FP64 dots."""
+    tmp_p = tmp_p or ctx.empty(g.proj_shape)
+    tmp_x = tmp_x or ctx.empty(g.vol_shape)
+    g.project(x, tmp_p)
+    r = tmp_p.download()
+    r = np.ascontiguousarray(b.download() - r)
+    tmp_p.upload(r)
+    g.backproject(tmp_p, tmp_x)
+    return np.sqrt(dot(ctx, tmp_x, tmp_x)) / bnorm
+
+
+def warm_sirt(ctx, g, b, x, lam, tol, params, scale, k_in=5, maxit=3000, net=None, log=None):
+    """Warm-started SIRT: k_in SIRT iterations, the network, then SIRT from the
+    better of (network output, k_in iterate) to relnrm <= tol.  Returns
+    (histories, seconds per phase, chosen start)."""
+    t = {}
+    t0 = time.perf_counter()
+    h1, _ = g.sirt(b, x, lam, k_in, 0.0)
+    ctx.synchronize()
+    t["sirt_in"] = time.perf_counter() - t0
+    n = g.nx
+    t0 = time.perf_counter()
+    cube_in = ctx.empty((g.ny, n, n))
+    cube_out = ctx.empty((g.ny, n, n))
+    cyl_to_cube(ctx, x, g.radius, g.ny, cube_in)
+    own = net is None
+    if own:
+        net = UNet(ctx, g.ny, n, n, training=False, params=params, scale=scale)
+    net.forward(cube_in, cube_out)
+    xn = ctx.empty(g.vol_shape)
+    cube_to_cyl(ctx, cube_out, g.radius, g.ny, xn)
+    ctx.synchronize()
+    t["net"] = time.perf_counter() - t0
+    if own:
+        net.close()
+    t0 = time.perf_counter()
+    # SIRT judges the network: one SIRT pass from its output gives its
+    # residual; if that is not below the k_in iterate's, continue from the
+    # SIRT iterate instead (SIRT remains the fallback).
+    from . import copy as dcopy
+
+    x_keep = ctx.empty(g.vol_shape)
+    dcopy(ctx, x_keep, x)
+    dcopy(ctx, x, xn)
+    hp, _ = g.sirt_warm(b, x, lam, 1, tol)
+    chosen = "network"
+    if hp[0][2] >= h1[-1][2]:
+        dcopy(ctx, x, x_keep)
+        chosen = "sirt"
+        hp = []
+    if not hp or hp[-1][1] != 2:
+        h2, _ = g.sirt_warm(b, x, lam, maxit, tol)
+    else:
+        h2 = []
+    h2 = hp + h2
+    ctx.synchronize()
+    t["sirt_out"] = time.perf_counter() - t0
+    return (h1, h2), t, chosen

@@ -102,6 +102,15 @@ int arec_sirt(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x
               int32_t maxit, float tol, arec_hist *hist, int32_t hist_cap, int32_t *nhist,
               float *lam_out);
 
+/* SIRT continued from the volume in x_dev (no counterpart in the
+ * reference, whose SIRT always starts from zero, arecsirt.cpp:53-56).  The
+ * relative residual keeps the reference's fixed denominator ||A^T b||; the
+ * first printed line is the residual of the given volume; a restart still
+ * zeroes x.  Used for the learned warm start. */
+int arec_sirt_warm(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x_dev, float lam,
+                   int32_t maxit, float tol, arec_hist *hist, int32_t hist_cap, int32_t *nhist,
+                   float *lam_out);
+
 /* CGLS of cyl_cgls / _KB / _SQ (areccgls.cpp:12-243,245-478,
  * areccgls_KB.cpp:12-240).  warm != 0 starts from x_dev (reference:
  * xcvol->data != NULL); else x is zeroed.  w_dev != NULL gives the weighted
@@ -165,6 +174,39 @@ int arec_phantom(arec_ctx *ctx, double radius, int64_t ny, double support, const
  * result depends only on (seed, index). */
 int arec_add_noise(arec_ctx *ctx, float *b_dev, int64_t n, uint64_t seed, int32_t mode,
                    double sigma, double i0, double scale);
+
+/* ---- learned warm start (no counterpart in the reference) ------------- */
+/* 3-level 3-D U-Net (16/32/64 channels, 3x3x3 convolutions, ReLU, BF16
+ * weights/activations, FP32 accumulation) on volumes D x H x W (multiples of
+ * 4; training also needs W <= 128 and (D/4)(H/4) divisible by 128).  y = x +
+ * net(x*scale)/scale on FP32 cubes [D][H][W]. */
+typedef struct arec_unet arec_unet;
+int arec_unet_create(arec_ctx *ctx, arec_unet **net, int32_t D, int32_t H, int32_t W,
+                     int32_t training, uint64_t seed);
+int arec_unet_destroy(arec_unet *net);
+int64_t arec_unet_nparams(const arec_unet *net);
+int arec_unet_get_params(arec_unet *net, float *host);
+int arec_unet_set_params(arec_unet *net, const float *host);
+int arec_unet_set_scale(arec_unet *net, float scale);
+int arec_unet_forward(arec_unet *net, const float *x_cube, float *y_cube);
+/* One Adam step on the mean-squared error to t_cube; *loss before the step. */
+int arec_unet_train_step(arec_unet *net, const float *x_cube, const float *t_cube, float lr,
+                         double *loss);
+/* Loss and its gradient with respect to every parameter (host array). */
+int arec_unet_gradient(arec_unet *net, const float *x_cube, const float *t_cube, double *loss,
+                       float *grad_host);
+/* One MFMA convolution (FP32 in/out, BF16 inside), for checks: input
+ * channels [0,C1) from in1 [D][H][W][C1], [C1,C1+C2) from in2 at half
+ * resolution (nearest upsampling); w [27][Cout][C1+C2]. */
+int arec_conv3_check(arec_ctx *ctx, const float *in1, int32_t C1, const float *in2, int32_t C2,
+                     const float *w, const float *bias, float *out, int32_t Cout, int32_t D,
+                     int32_t H, int32_t W, int32_t relu);
+int arec_conv3_wgrad_check(arec_ctx *ctx, const float *in1, int32_t C1, const float *in2,
+                           int32_t C2, const float *dz, int32_t Cout, int32_t D, int32_t H,
+                           int32_t W, float *dw);
+/* compact cylinder [nrays][ny] <-> bounding cube [ny][n][n], n = round(2r+1) */
+int arec_cyl_to_cube(arec_ctx *ctx, const float *x_dev, double radius, int64_t ny, float *cube_dev);
+int arec_cube_to_cyl(arec_ctx *ctx, const float *cube_dev, double radius, int64_t ny, float *x_dev);
 
 /* ---- host-pointer drop-in layer -------------------------------------- */
 /* arecProject2D* (arecproject.h:55-62) with the cylinder given by radius and
