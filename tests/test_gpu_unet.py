@@ -5,7 +5,8 @@
   (tolerance: 1% of the output scale for BF16 outputs, 1e-4 relative for the
   FP32-accumulated weight gradient);
 * the whole network's backward pass against central finite differences of
-  its own forward pass, layer by layer (5%: the forward runs in BF16);
+  its own forward pass, layer by layer (5%: the forward runs in BF16;
+  cubic fit over 8 steps);
 * a short training run lowers the loss.
 """
 import numpy as np
@@ -94,9 +95,16 @@ def test_network_gradient_finite_differences(ctx):
         gn = np.linalg.norm(v)
         assert gn > 0
         v /= gn
-        # 2% of the block's norm: well above the BF16 quantum of its weights
-        eps = 0.02 * np.linalg.norm(p0[lo:hi])
-        fd = (_loss(net, dx, dt, p0 + eps * v, out) - _loss(net, dx, dt, p0 - eps * v, out)) / (2 * eps)
+        # The BF16 forward makes the loss noisy (~1e-3 relative: weight
+        # changes flip activation roundings) and the ReLU network is curved
+        # along large steps, so a two-point difference is unreliable either
+        # way.  Fit a cubic to the loss at 8 steps of up to ~8% loss change;
+        # its linear coefficient is the directional derivative, with the
+        # curvature fitted out and the noise averaged.
+        h = 0.02 * loss / gn
+        ts = np.array([-4, -3, -2, -1, 1, 2, 3, 4], np.float64) * h
+        ls = np.array([_loss(net, dx, dt, p0 + t * v, out) for t in ts])
+        fd = np.polynomial.polynomial.polyfit(ts, ls, 3)[1]
         print(f"block {lo}:{hi} analytic {gn:.5e} finite-diff {fd:.5e}")
         assert fd == pytest.approx(gn, rel=0.05)
 
