@@ -76,7 +76,11 @@ def _load():
         "arec_align_shift": ([vp, vp, i32, i32, i32, vp, vp, flt], C.c_int),
         "arec_h_cc_images": ([i32, i32, i32, vp, i32, i32, i32, vp, vp, vp], C.c_int),
         "arec_h_shift_images": ([i32, i32, i32, vp, vp, vp], C.c_int),
-        "arec_sirt_warm": ([vp, vp, vp, vp, flt, i32, flt, P(Hist), i32, P(i32), P(flt)], C.c_int),
+        "arec_sirt_warm": ([vp, vp, vp, vp, flt, i32, flt, dbl, P(Hist), i32, P(i32), P(flt)], C.c_int),
+        "arec_sirt_batch": ([vp, i32, P(vp), P(vp), P(vp), P(flt), i32, flt, P(Hist), i32, P(i32), P(flt)],
+                            C.c_int),
+        "arec_geom_plan": ([vp, vp, i32, i32], C.c_int),
+        "arec_geom_get_plan": ([vp, P(i32), P(i32), P(i32)], C.c_int),
         "arec_unet_create": ([vp, P(vp), i32, i32, i32, i32, C.c_uint64], C.c_int),
         "arec_unet_destroy": ([vp], C.c_int),
         "arec_unet_nparams": ([vp], i64),
@@ -114,7 +118,7 @@ EXPORTED = [
     "arec_unet_destroy", "arec_unet_nparams", "arec_unet_get_params", "arec_unet_set_params",
     "arec_unet_set_scale", "arec_unet_forward", "arec_unet_train_step", "arec_unet_gradient",
     "arec_conv3_check", "arec_conv3_wgrad_check", "arec_cyl_to_cube", "arec_cube_to_cyl", "arec_h_project", "arec_h_backproject",
-    "arec_h_solve",
+    "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan",
 ]
 
 
@@ -271,14 +275,25 @@ class Geometry:
         _check(lib.arec_sirt(self.ctx.h, self.h, b.ptr, x.ptr, lam, maxit, tol, H, cap, C.byref(nh), C.byref(lo)))
         return _hist(H, nh.value), lo.value
 
-    def sirt_warm(self, b, x, lam, maxit, tol, hist_cap=None):
-        """SIRT continued from the volume in x (fixed ||A^T b|| denominator)."""
+    def sirt_warm(self, b, x, lam, maxit, tol, hist_cap=None, reject_above=0.0):
+        """SIRT continued from the volume in x (fixed ||A^T b|| denominator).
+        With reject_above > 0 a start whose relative residual is not below it
+        is rejected at once (one history line with event 3, x unchanged)."""
         cap = hist_cap or (maxit + 64)
         H = (Hist * cap)()
         nh, lo = C.c_int32(), C.c_float()
-        _check(lib.arec_sirt_warm(self.ctx.h, self.h, b.ptr, x.ptr, lam, maxit, tol, H, cap, C.byref(nh),
-                                  C.byref(lo)))
+        _check(lib.arec_sirt_warm(self.ctx.h, self.h, b.ptr, x.ptr, lam, maxit, tol, float(reject_above), H, cap,
+                                  C.byref(nh), C.byref(lo)))
         return _hist(H, nh.value), lo.value
+
+    def plan(self, rows_per_lane=0, min_groups=0):
+        """Set the forward projector's launch plan (results do not depend on it)."""
+        _check(lib.arec_geom_plan(self.ctx.h, self.h, int(rows_per_lane), int(min_groups)))
+
+    def get_plan(self):
+        r, m, s = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib.arec_geom_get_plan(self.h, C.byref(r), C.byref(m), C.byref(s)))
+        return r.value, m.value, s.value
 
     def cgls(self, b, x, maxit, tol, warm=False, w=None, hist_cap=None):
         cap = hist_cap or (maxit + 4)
@@ -308,6 +323,27 @@ def dot(ctx, a, b):
     out = C.c_double()
     _check(lib.arec_dot(ctx.h, a.ptr, b.ptr, a.size, C.byref(out)))
     return out.value
+
+
+def sirt_batch(ctx, geoms, bs, xs, lams, maxit, tol, hist_cap=None):
+    """Batched SIRT (arec_sirt_batch): problem k = (geoms[k], bs[k], xs[k],
+    lams[k]); returns per problem (history, final lam)."""
+    n = len(geoms)
+    cap = hist_cap or (maxit + 64)
+    H = (Hist * (cap * max(1, n)))()
+    nh = (C.c_int32 * max(1, n))()
+    lo = (C.c_float * max(1, n))()
+    G = (C.c_void_p * max(1, n))(*[g.h for g in geoms])
+    B = (C.c_void_p * max(1, n))(*[b.ptr for b in bs])
+    X = (C.c_void_p * max(1, n))(*[x.ptr for x in xs])
+    LM = (C.c_float * max(1, n))(*[float(v) for v in lams])
+    _check(lib.arec_sirt_batch(ctx.h, n, G, B, X, LM, maxit, tol, H, cap, nh, lo))
+    out = []
+    for k in range(n):
+        h = [(H[k * cap + i].iter, H[k * cap + i].event, H[k * cap + i].relnrm, H[k * cap + i].rnorm)
+             for i in range(min(nh[k], cap))]
+        out.append((h, lo[k]))
+    return out
 
 
 def align_ccorr(ctx, x, y, c):

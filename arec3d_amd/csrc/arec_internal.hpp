@@ -65,7 +65,7 @@ struct Geometry {
     uint4 *d_fwd = nullptr;       // na * nvalid entries
     // forward sweep split into detector segments: per (angle, segment)
     // {first entry, end entry, first tap of the entry before the first}
-    int nseg = 1, segw = 0, fwd_rpl = 1, fwd_chunks = 1;
+    int nseg = 1, segw = 0, fwd_rpl = 1, fwd_chunks = 1, fwd_min_groups = 1024;
     std::vector<int> seg;         // na * nseg * 4
     int *d_seg = nullptr;
     // first table entry whose first tap is >= q, for q = kTapBase .. nx+3
@@ -85,11 +85,50 @@ int geometry_build_host(Geometry &g, int method, int nx, int ny, double radius,
 // device weight function (used to order the forward table).
 int host_first_tap(const Geometry &g, int a, int j);
 void geometry_sort_angles(Geometry &g, int a0, int a1, std::vector<int> &order);
-void geometry_plan_forward(Geometry &g);
+bool fwd_rpl_allowed(int ny, int rpl);
+bool geometry_plan_forward(Geometry &g, int rpl, int min_groups);
+void geometry_segments(Geometry &g);
+
+// Arguments of one forward / back application (device code reads them).
+struct FwdArgs {
+    const float *x;
+    float *p;
+    const uint4 *table;
+    const int4 *seg;
+    const int *pixstart;
+    double *partials;
+    int npix, nvalid, nseg, segw, nx, ny, na, nchunks;
+};
+struct BpArgs {
+    const float *p;
+    float *out;
+    const float *bvec;
+    const int *tile_cols, *tile_org, *coord;
+    const AngleConst *ang;
+    double *partials;
+    int *err;
+    double radius;
+    int xcent, nx, ny, na, nrowblk;
+};
+// One problem of a persistent batched launch: its arguments, a skip flag
+// (nullptr: never skipped) and the index of its first work item.
+struct BatchFwd {
+    FwdArgs a;
+    const int *skip;
+    int item0, pad;
+};
+struct BatchBp {
+    BpArgs a;
+    const int *skip;
+    int item0, pad;
+};
 
 // Device setup and operators (projector.hip).
 int geometry_upload(Geometry &g, hipStream_t st, std::string *err);
 void geometry_free_device(Geometry &g);
+// Re-plan the forward launch (rows per lane, minimum workgroup count) of an
+// uploaded geometry: recomputes and uploads the segment table.
+int geometry_replan_forward(Geometry &g, int rpl, int min_groups, hipStream_t st);
 // Modes of the back projector's epilogue.
 enum BpMode { kBpStore = 0, kBpSirtGrad = 1, kBpStoreNorm = 2 };
 int launch_forward(const Geometry &g, const float *x, float *p, double *partials, hipStream_t st,
@@ -98,6 +137,30 @@ int launch_backward(const Geometry &g, const float *p, float *x, int mode, const
                     double *partials, hipStream_t st, const int *skip);
 int64_t forward_partial_count(const Geometry &g);
 int64_t backward_partial_count(const Geometry &g);
+// Work items (workgroup jobs) of one application and their arguments.
+int64_t forward_items(const Geometry &g);
+int64_t backward_items(const Geometry &g);
+FwdArgs fwd_args(const Geometry &g, const float *x, float *p, double *partials);
+BpArgs bp_args(const Geometry &g, const float *p, float *x, const float *bvec, double *partials);
+// Persistent launches over nprob problems (device arrays of descriptors,
+// item0 ascending, `total` items); d_counter is an int of scratch.  All
+// forward problems of one launch share rpl, all back problems the method.
+int launch_forward_batch(int rpl, const BatchFwd *d_probs, int nprob, int64_t total, int *d_counter,
+                         hipStream_t st);
+int launch_backward_batch(int method, int mode, const BatchBp *d_probs, int nprob, int64_t total,
+                          int *d_counter, hipStream_t st);
+
+// One problem of a (batched) SIRT solve (solver.hip).
+struct SirtProblem {
+    const Geometry *g;
+    const float *b;
+    float *x;
+    float lam;
+};
+int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit, float tol, int warm,
+                   double judge, arec_hist *hist, int hist_cap, int *nhist, float *lam_out, bool use_graph);
+int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float lam, int maxit, float tol,
+             int warm, double judge, arec_hist *hist, int hist_cap, int *nhist, float *lam_out, bool use_graph);
 
 // Fill with a constant by a kernel (graph-capture safe); skip as above.
 int fill(float *x, float v, int64_t n, hipStream_t st, const int *skip);

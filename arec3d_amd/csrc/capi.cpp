@@ -20,9 +20,6 @@ int hip_fail(hipError_t e, const char *what) {
 }
 
 // solver.hip
-int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float lam, int maxit,
-             float tol, int warm, arec_hist *hist, int hist_cap, int *nhist, float *lam_out,
-             bool use_graph);
 int cgls_run(const Geometry &g, hipStream_t st, const float *b, const float *w, float *x, int warm,
              int maxit, float tol, arec_hist *hist, int hist_cap, int *nhist, bool use_graph);
 int init_weights(const float *b, float *w, int64_t nx, int64_t ny, int64_t nz, hipStream_t st);
@@ -184,17 +181,44 @@ int arec_sirt(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x
               float *lam_out) {
     CHECK_ARG(ctx && geom && b_dev && x_dev, "NULL argument");
     CHECK_ARG(hist_cap >= 0 && (hist || hist_cap == 0), "bad history buffer");
-    return sirt_run(geom->g, ctx->stream, b_dev, x_dev, lam, maxit, tol, 0, hist, hist_cap, nhist,
+    return sirt_run(geom->g, ctx->stream, b_dev, x_dev, lam, maxit, tol, 0, 0.0, hist, hist_cap, nhist,
                     lam_out, use_graphs());
 }
 
 int arec_sirt_warm(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x_dev, float lam,
-                   int32_t maxit, float tol, arec_hist *hist, int32_t hist_cap, int32_t *nhist,
-                   float *lam_out) {
+                   int32_t maxit, float tol, double reject_above, arec_hist *hist, int32_t hist_cap,
+                   int32_t *nhist, float *lam_out) {
     CHECK_ARG(ctx && geom && b_dev && x_dev, "NULL argument");
     CHECK_ARG(hist_cap >= 0 && (hist || hist_cap == 0), "bad history buffer");
-    return sirt_run(geom->g, ctx->stream, b_dev, x_dev, lam, maxit, tol, 1, hist, hist_cap, nhist,
-                    lam_out, use_graphs());
+    return sirt_run(geom->g, ctx->stream, b_dev, x_dev, lam, maxit, tol, 1, reject_above, hist, hist_cap,
+                    nhist, lam_out, use_graphs());
+}
+
+int arec_sirt_batch(arec_ctx *ctx, int32_t nprob, const arec_geom *const *geoms, const float *const *b_dev,
+                    float *const *x_dev, const float *lam, int32_t maxit, float tol, arec_hist *hist,
+                    int32_t hist_cap, int32_t *nhist, float *lam_out) {
+    CHECK_ARG(ctx && nprob >= 0 && (nprob == 0 || (geoms && b_dev && x_dev && lam)), "NULL argument");
+    CHECK_ARG(hist_cap >= 0 && (hist || hist_cap == 0), "bad history buffer");
+    std::vector<SirtProblem> P;
+    for (int k = 0; k < nprob; k++) {
+        CHECK_ARG(geoms[k] && b_dev[k] && x_dev[k], "NULL problem");
+        P.push_back(SirtProblem{&geoms[k]->g, b_dev[k], x_dev[k], lam[k]});
+    }
+    return sirt_batch_run(P, ctx->stream, maxit, tol, 0, 0.0, hist, hist_cap, nhist, lam_out, use_graphs());
+}
+
+int arec_geom_plan(arec_ctx *ctx, arec_geom *geom, int32_t rows_per_lane, int32_t min_groups) {
+    CHECK_ARG(ctx && geom, "NULL argument");
+    CHECK_ARG(min_groups >= 0 && rows_per_lane >= 0, "bad plan");
+    return geometry_replan_forward(geom->g, rows_per_lane, min_groups, ctx->stream);
+}
+
+int arec_geom_get_plan(const arec_geom *geom, int32_t *rows_per_lane, int32_t *min_groups, int32_t *nseg) {
+    CHECK_ARG(geom, "NULL argument");
+    if (rows_per_lane) *rows_per_lane = geom->g.fwd_rpl;
+    if (min_groups) *min_groups = geom->g.fwd_min_groups;
+    if (nseg) *nseg = geom->g.nseg;
+    return 0;
 }
 
 int arec_cgls(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, const float *w_dev,

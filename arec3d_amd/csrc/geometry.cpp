@@ -132,29 +132,71 @@ int host_first_tap(const Geometry &g, int a, int j) {
     return (int)std::nearbyint(xb) - 1;
 }
 
-// Launch shape of the forward projector: rows per lane (4 with 16-byte
-// loads when the row count allows), row chunks of 64 lanes, and enough
-// detector segments per angle that the grid has >= 1024 workgroups of
-// kFwdAngles waves.
-void geometry_plan_forward(Geometry &g) {
-    g.fwd_rpl = (g.ny >= 256 && g.ny % 4 == 0) ? 4 : (g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1;
+bool fwd_rpl_allowed(int ny, int rpl) {
+    if (rpl == 1) return true;
+    if (rpl == 2) return ny % 2 == 0;
+    if (rpl == 4) return ny % 4 == 0;
+    return false;
+}
+
+// Launch shape of the forward projector: rows per lane (rpl; 0 picks 4 with
+// 16-byte loads when the row count allows), row chunks of 64 lanes, and
+// enough detector segments per angle that the grid has >= min_groups
+// workgroups of kFwdAngles waves (0 picks 1024).  Returns false for an rpl
+// the row count does not allow.
+bool geometry_plan_forward(Geometry &g, int rpl, int min_groups) {
+    if (rpl == 0) rpl = (g.ny >= 256 && g.ny % 4 == 0) ? 4 : (g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1;
+    if (!fwd_rpl_allowed(g.ny, rpl)) return false;
+    if (min_groups <= 0) min_groups = 1024;
+    g.fwd_rpl = rpl;
+    g.fwd_min_groups = min_groups;
     g.fwd_chunks = (g.ny + 64 * g.fwd_rpl - 1) / (64 * g.fwd_rpl);
     int64_t groups = (int64_t)((g.na + kFwdAngles - 1) / kFwdAngles) * g.fwd_chunks;
-    int nseg = (int)std::max<int64_t>(1, (1024 + groups - 1) / groups);
+    int nseg = (int)std::max<int64_t>(1, (min_groups + groups - 1) / groups);
     nseg = std::min(nseg, std::max(1, g.nx / (4 * kFwdOut)));
     int segw = (g.nx + nseg - 1) / nseg;
     segw = (segw + kFwdOut - 1) / kFwdOut * kFwdOut;
     g.segw = segw;
     g.nseg = (g.nx + segw - 1) / segw;
+    return true;
+}
+
+// Segment table of the current plan from pixstart: segment s (pixels
+// [s*segw, (s+1)*segw)) needs the entries whose first tap lies in
+// [s*segw - 2, (s+1)*segw), and the first tap of the entry before them.
+void geometry_segments(Geometry &g) {
+    const int nv = g.nvalid, npix = g.npix;
     g.seg.assign((size_t)g.na * g.nseg * 4, 0);
-    g.npix = g.nx + 8;
-    g.pixstart.assign((size_t)g.na * g.npix, 0);
+    for (int a = 0; a < g.na; a++) {
+        const int *ps = &g.pixstart[(size_t)a * npix];
+        // first entry whose first tap is >= k
+        auto first_at = [&](int k) {
+            int q = k - kTapBase;
+            return q <= 0 ? 0 : q >= npix ? nv : ps[q];
+        };
+        // The first tap of entry e is the largest k with first_at(k) <= e;
+        // qk tracks it for the (non-decreasing) entries start - 1.
+        int qk = 0;
+        for (int s = 0; s < g.nseg; s++) {
+            int start = first_at(s * g.segw - 2), end = first_at((s + 1) * g.segw);
+            int *e = &g.seg[((size_t)a * g.nseg + s) * 4];
+            e[0] = start;
+            e[1] = end;
+            if (start == 0) {
+                e[2] = kTapBase;
+            } else {
+                while (qk + 1 < npix && ps[qk + 1] <= start - 1) qk++;
+                e[2] = qk + kTapBase;
+            }
+            e[3] = 0;
+        }
+    }
 }
 
 // Stable counting sort of the contributing columns of angles [a0, a1) by
 // their first tap; order receives (a1-a0)*nvalid column indices.  Also
-// fills the angles' segment table: segment s (pixels [s*segw, (s+1)*segw))
-// needs the entries whose first tap lies in [s*segw - 2, (s+1)*segw).
+// fills pixstart: for each q, the first entry whose first tap is >=
+// q + kTapBase.
 void geometry_sort_angles(Geometry &g, int a0, int a1, std::vector<int> &order) {
     const int nv = g.nvalid;
     order.resize((size_t)(a1 - a0) * nv);
@@ -175,19 +217,6 @@ void geometry_sort_angles(Geometry &g, int a0, int a1, std::vector<int> &order) 
             for (size_t k = 1; k < count.size(); k++) count[k] += count[k - 1];
             int *out = order.data() + (size_t)(a - a0) * nv;
             for (int j = 0; j < nv; j++) out[count[key[j] - kmin]++] = j;
-            // segment bounds from the sorted keys
-            int i = 0;
-            for (int sgi = 0; sgi < g.nseg; sgi++) {
-                int lo = sgi * g.segw - 2, hi = (sgi + 1) * g.segw;
-                while (i < nv && key[out[i]] < lo) i++;
-                int start = i, end = start;
-                while (end < nv && key[out[end]] < hi) end++;
-                int *e = &g.seg[((size_t)a * g.nseg + sgi) * 4];
-                e[0] = start;
-                e[1] = end;
-                e[2] = start > 0 ? key[out[start - 1]] : kTapBase;
-                e[3] = 0;
-            }
             int *ps = &g.pixstart[(size_t)a * g.npix];
             int ii = 0;
             for (int q = 0; q < g.npix; q++) {

@@ -156,25 +156,20 @@ constexpr int kFwdBatch = 8;  // table entries whose loads are in flight togethe
 // pixels go to an LDS strip of kFwdOut columns which is written to global
 // memory as 32-byte row segments.  partials (optional) receives the FP64 sum
 // of the FP32 squares of the wave's stored pixels.
+// Workgroup b of one forward application: angle group b % nag (fastest),
+// then detector segment, then row chunk.  Every wave of the workgroup calls
+// this with the same b.
 template <int RPL>
-__global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(
-    const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ table,
-    const int4 *__restrict__ seg, const int *__restrict__ pixstart, int npix, int nvalid, int nseg,
-    int segw, int nx, int ny, int na, int nchunks, double *__restrict__ partials,
-    const int *__restrict__ skip) {
-    if (skip && *skip) return;
+__device__ __forceinline__ void forward_body(const FwdArgs &A, int b) {
+    const float *__restrict__ x = A.x;
+    const uint4 *__restrict__ table = A.table;
+    const int *__restrict__ pixstart = A.pixstart;
+    const int npix = A.npix, nvalid = A.nvalid, nseg = A.nseg, segw = A.segw, nx = A.nx, ny = A.ny,
+              na = A.na, nchunks = A.nchunks;
     constexpr int ROWS = 64 * RPL;
     constexpr int LDW = kFwdOut + 1;  // padded strip row (conflict-free column writes)
     __shared__ float s_out[kFwdAngles][ROWS * LDW];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    // Workgroups with consecutive ids run on different XCDs: give each XCD a
-    // contiguous range of (chunk, segment, angle group) triples, angle group
-    // fastest, so that the workgroups resident on one XCD sweep the same rows
-    // of nearly the same columns (neighbouring angles walk the disk almost
-    // together) and share its L2.
-    const int nblk = gridDim.x;
-    int b = blockIdx.x;
-    if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
     const int nag = (na + kFwdAngles - 1) / kFwdAngles;
     const int ag = b % nag;
     const int rest = b / nag;
@@ -186,9 +181,9 @@ __global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(
     const int u_lo = sg * segw, u_hi = min(nx, u_lo + segw);
     const int ybase = chunk * ROWS;
     const int y0 = ybase + lane * RPL;
-    float *__restrict__ pa = p + (size_t)a * nx * ny;
+    float *__restrict__ pa = A.p + (size_t)a * nx * ny;
     const uint4 *__restrict__ tab = table + (size_t)a * nvalid;
-    const int4 sd = seg[(size_t)a * nseg + sg];
+    const int4 sd = A.seg[(size_t)a * nseg + sg];
     const int i0 = __builtin_amdgcn_readfirstlane(sd.x), i1 = __builtin_amdgcn_readfirstlane(sd.y);
     float *strip = s_out[wave];
 
@@ -321,10 +316,58 @@ __global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(
     // the last window and the pixels above it up to the segment end
     advance(max(base + 3, u_hi));
     while (strip_lo < u_hi) flush_strip();
-    if (partials) {
+    if (A.partials) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
-        if (lane == 0) partials[((size_t)a * nchunks + chunk) * nseg + sg] = sq;
+        if (lane == 0) A.partials[((size_t)a * nchunks + chunk) * nseg + sg] = sq;
+    }
+}
+
+template <int RPL>
+__global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(FwdArgs A, const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    // Workgroups with consecutive ids run on different XCDs: give each XCD a
+    // contiguous range of (chunk, segment, angle group) triples, angle group
+    // fastest, so that the workgroups resident on one XCD sweep the same rows
+    // of nearly the same columns (neighbouring angles walk the disk almost
+    // together) and share its L2.
+    const int nblk = gridDim.x;
+    int b = blockIdx.x;
+    if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+    forward_body<RPL>(A, b);
+}
+
+// Index of the problem that owns work item `it`: the last k with
+// item0(k) <= it (item0 ascending).  Uniform across the workgroup.
+template <typename P>
+__device__ __forceinline__ int owner_of(const P *__restrict__ probs, int nprob, int it) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (probs[mid].item0 <= it) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Persistent forward projector over many problems: workgroups take work
+// items (one forward workgroup's job of one problem) from a queue until it
+// is empty.  *counter must be 0 at launch.
+template <int RPL>
+__global__ __launch_bounds__(64 * kFwdAngles) void forward_batch_kernel(const BatchFwd *__restrict__ probs,
+                                                                       int nprob, int total,
+                                                                       int *__restrict__ counter) {
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();  // every wave has read the previous item
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= total) break;
+        const int k = owner_of(probs, nprob, it);
+        const int *skip = probs[k].skip;
+        if (skip && *skip) continue;
+        forward_body<RPL>(probs[k].a, it - probs[k].item0);
     }
 }
 
@@ -353,22 +396,23 @@ __device__ __forceinline__ double corner_xb(int X, int Z, const AngleConst &A, d
 //  kBpSirtGrad   out = bvec - acc (FP32), partial += sum out*(double)out
 //  kBpStoreNorm  out = acc, partial += sum (double)(acc*acc)
 template <int M, int MODE>
-__global__ __launch_bounds__(kBpThreads) void backward_kernel(
-    const float *__restrict__ p, float *__restrict__ out, const float *__restrict__ bvec,
-    const int *__restrict__ tile_cols, const int *__restrict__ tile_org, const int *__restrict__ coord,
-    const AngleConst *__restrict__ ang, double radius, int xcent, int nx, int ny, int na, int nrowblk,
-    double *__restrict__ partials, int *__restrict__ err, const int *__restrict__ skip) {
-    if (skip && *skip) return;
+__device__ __forceinline__ void backward_body(const BpArgs &B, int b) {
+    const float *__restrict__ p = B.p;
+    const int *__restrict__ coord = B.coord;
+    const AngleConst *__restrict__ ang = B.ang;
+    const double radius = B.radius;
+    const int xcent = B.xcent, nx = B.nx, ny = B.ny, na = B.na, nrowblk = B.nrowblk;
+    int *__restrict__ err = B.err;
     __shared__ __align__(16) float s_p[kBpAngBlk][kBpWin][kBpRowPad];
     __shared__ int s_u0[kBpAngBlk];
     __shared__ double s_red[kBpThreads / 64];
 
     const int tid = threadIdx.x;
-    const int tile = blockIdx.x / nrowblk;
-    const int rb = blockIdx.x % nrowblk;
+    const int tile = b / nrowblk;
+    const int rb = b % nrowblk;
     const int ybase = rb * kBpRows;
-    const int j = tile_cols[tile * kTileCols + tid];
-    const int X0 = tile_org[2 * tile], Z0 = tile_org[2 * tile + 1];
+    const int j = B.tile_cols[tile * kTileCols + tid];
+    const int X0 = B.tile_org[2 * tile], Z0 = B.tile_org[2 * tile + 1];
     int cX = 0, cZ = 0;
     if (j >= 0) {
         cX = coord[2 * j];
@@ -450,8 +494,8 @@ __global__ __launch_bounds__(kBpThreads) void backward_kernel(
 
     double part = 0.0;
     if (j >= 0) {
-        float *o = out + (size_t)j * ny + ybase;
-        const float *bv = bvec ? bvec + (size_t)j * ny + ybase : nullptr;
+        float *o = B.out + (size_t)j * ny + ybase;
+        const float *bv = B.bvec ? B.bvec + (size_t)j * ny + ybase : nullptr;
 #pragma unroll
         for (int q = 0; q < kBpRows / 4; q++) {
             float v[4] = {acc[q].x, acc[q].y, acc[q].z, acc[q].w};
@@ -480,8 +524,33 @@ __global__ __launch_bounds__(kBpThreads) void backward_kernel(
         if (tid == 0) {
             double s = 0.0;
             for (int w = 0; w < kBpThreads / 64; w++) s += s_red[w];
-            partials[blockIdx.x] = s;
+            B.partials[b] = s;
         }
+        __syncthreads();  // s_red is reused by the next item of a persistent workgroup
+    }
+}
+
+template <int M, int MODE>
+__global__ __launch_bounds__(kBpThreads) void backward_kernel(BpArgs B, const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    backward_body<M, MODE>(B, blockIdx.x);
+}
+
+// Persistent back projector over many problems (see forward_batch_kernel).
+template <int M, int MODE>
+__global__ __launch_bounds__(kBpThreads) void backward_batch_kernel(const BatchBp *__restrict__ probs, int nprob,
+                                                                   int total, int *__restrict__ counter) {
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= total) break;
+        const int k = owner_of(probs, nprob, it);
+        const int *skip = probs[k].skip;
+        if (skip && *skip) continue;
+        backward_body<M, MODE>(probs[k].a, it - probs[k].item0);
     }
 }
 
@@ -490,9 +559,54 @@ __global__ __launch_bounds__(kBpThreads) void backward_kernel(
 // ---------------------------------------------------------------------------
 int64_t forward_partial_count(const Geometry &g) { return (int64_t)g.na * g.fwd_chunks * g.nseg; }
 
-int64_t backward_partial_count(const Geometry &g) {
+int64_t backward_partial_count(const Geometry &g) { return backward_items(g); }
+
+int64_t forward_items(const Geometry &g) {
+    return (int64_t)((g.na + kFwdAngles - 1) / kFwdAngles) * g.nseg * g.fwd_chunks;
+}
+
+int64_t backward_items(const Geometry &g) {
     int nrowblk = (g.ny + kBpRows - 1) / kBpRows;
     return (int64_t)g.ntiles * nrowblk;
+}
+
+FwdArgs fwd_args(const Geometry &g, const float *x, float *p, double *partials) {
+    FwdArgs A;
+    A.x = x;
+    A.p = p;
+    A.table = g.d_fwd;
+    A.seg = reinterpret_cast<const int4 *>(g.d_seg);
+    A.pixstart = g.d_pixstart;
+    A.partials = partials;
+    A.npix = g.npix;
+    A.nvalid = g.nvalid;
+    A.nseg = g.nseg;
+    A.segw = g.segw;
+    A.nx = g.nx;
+    A.ny = g.ny;
+    A.na = g.na;
+    A.nchunks = g.fwd_chunks;
+    return A;
+}
+
+BpArgs bp_args(const Geometry &g, const float *p, float *x, const float *bvec, double *partials) {
+    BpArgs B;
+    B.p = p;
+    B.out = x;
+    B.bvec = bvec;
+    B.tile_cols = g.d_tile_cols;
+    B.tile_org = g.d_tile_org;
+    B.coord = g.d_coord;
+    B.ang = g.d_ang;
+    B.partials = partials;
+    B.err = g.d_err;
+    B.radius = g.radius;
+    B.xcent = g.xcent;
+    B.nx = g.nx;
+    B.ny = g.ny;
+    B.na = g.na;
+    B.nrowblk = (g.ny + kBpRows - 1) / kBpRows;
+    return B;
 }
 
 template <typename T>
@@ -533,7 +647,9 @@ int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
     AREC_HIP(hipMemsetAsync(g.d_err, 0, sizeof(int), st));
     size_t entries = (size_t)g.na * std::max(1, g.nvalid);
     AREC_HIP(hipMalloc((void **)&g.d_fwd, sizeof(uint4) * entries));
-    geometry_plan_forward(g);
+    geometry_plan_forward(g, 0, 0);
+    g.npix = g.nx + 8;
+    g.pixstart.assign((size_t)g.na * g.npix, 0);
     // Build the forward table in batches of angles: order on the host (stable
     // counting sort by first tap), weights on the device.
     int batch = (int)std::max<int64_t>(1, std::min<int64_t>(g.na, (int64_t)(64 << 20) / std::max(1, g.nvalid)));
@@ -557,6 +673,7 @@ int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
     }
     AREC_HIP(hipStreamSynchronize(st));
     hipFree(d_order);
+    geometry_segments(g);
     if ((rc = upload(&g.d_seg, g.seg, st))) return rc;
     if ((rc = upload(&g.d_pixstart, g.pixstart, st))) return rc;
     AREC_HIP(hipStreamSynchronize(st));
@@ -570,36 +687,45 @@ int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
     return 0;
 }
 
+int geometry_replan_forward(Geometry &g, int rpl, int min_groups, hipStream_t st) {
+    if (!g.device_ready) {
+        set_error("geometry not uploaded");
+        return AREC_EINVAL;
+    }
+    if (!geometry_plan_forward(g, rpl, min_groups)) {
+        set_error("forward plan: rows per lane must be 1, 2 or 4 and divide the row count");
+        return AREC_EINVAL;
+    }
+    geometry_segments(g);
+    AREC_HIP(hipStreamSynchronize(st));
+    hipFree(g.d_seg);
+    g.d_seg = nullptr;
+    return upload(&g.d_seg, g.seg, st);
+}
+
 int launch_forward(const Geometry &g, const float *x, float *p, double *partials, hipStream_t st,
                    const int *skip) {
     if (g.nvalid == 0) return fill(p, 0.0f, g.nproj, st, skip);
-    int nag = (g.na + kFwdAngles - 1) / kFwdAngles;
-    dim3 grid((unsigned)(nag * g.nseg * g.fwd_chunks));
-    const int4 *seg = reinterpret_cast<const int4 *>(g.d_seg);
-#define AREC_FWD(R)                                                                                  \
-    forward_kernel<R><<<grid, 64 * kFwdAngles, 0, st>>>(x, p, g.d_fwd, seg, g.d_pixstart, g.npix,  \
-                                                        g.nvalid, g.nseg, g.segw, g.nx, g.ny, g.na, \
-                                                        g.fwd_chunks, partials, skip)
+    dim3 grid((unsigned)forward_items(g));
+    FwdArgs A = fwd_args(g, x, p, partials);
     if (g.fwd_rpl == 4)
-        AREC_FWD(4);
+        forward_kernel<4><<<grid, 64 * kFwdAngles, 0, st>>>(A, skip);
     else if (g.fwd_rpl == 2)
-        AREC_FWD(2);
+        forward_kernel<2><<<grid, 64 * kFwdAngles, 0, st>>>(A, skip);
     else
-        AREC_FWD(1);
-#undef AREC_FWD
+        forward_kernel<1><<<grid, 64 * kFwdAngles, 0, st>>>(A, skip);
     AREC_HIP(hipGetLastError());
     return 0;
 }
 
 template <int M>
-static void bp_dispatch(int mode, dim3 grid, hipStream_t st, const Geometry &g, const float *p,
-                        float *x, const float *bvec, int nrowblk, double *partials, const int *skip) {
+static void bp_dispatch(int mode, dim3 grid, hipStream_t st, const BpArgs &B, const int *skip) {
     if (mode == kBpSirtGrad)
-        backward_kernel<M, kBpSirtGrad><<<grid, kBpThreads, 0, st>>>(p, x, bvec, g.d_tile_cols, g.d_tile_org, g.d_coord, g.d_ang, g.radius, g.xcent, g.nx, g.ny, g.na, nrowblk, partials, g.d_err, skip);
+        backward_kernel<M, kBpSirtGrad><<<grid, kBpThreads, 0, st>>>(B, skip);
     else if (mode == kBpStoreNorm)
-        backward_kernel<M, kBpStoreNorm><<<grid, kBpThreads, 0, st>>>(p, x, bvec, g.d_tile_cols, g.d_tile_org, g.d_coord, g.d_ang, g.radius, g.xcent, g.nx, g.ny, g.na, nrowblk, partials, g.d_err, skip);
+        backward_kernel<M, kBpStoreNorm><<<grid, kBpThreads, 0, st>>>(B, skip);
     else
-        backward_kernel<M, kBpStore><<<grid, kBpThreads, 0, st>>>(p, x, bvec, g.d_tile_cols, g.d_tile_org, g.d_coord, g.d_ang, g.radius, g.xcent, g.nx, g.ny, g.na, nrowblk, partials, g.d_err, skip);
+        backward_kernel<M, kBpStore><<<grid, kBpThreads, 0, st>>>(B, skip);
 }
 
 int launch_backward(const Geometry &g, const float *p, float *x, int mode, const float *bvec,
@@ -610,14 +736,84 @@ int launch_backward(const Geometry &g, const float *p, float *x, int mode, const
         if (rc) return rc;
     }
     if (g.ntiles == 0) return 0;
-    int nrowblk = (g.ny + kBpRows - 1) / kBpRows;
-    dim3 grid((unsigned)(g.ntiles * nrowblk));
+    dim3 grid((unsigned)backward_items(g));
+    BpArgs B = bp_args(g, p, x, bvec, partials);
     if (g.method == kSQ)
-        bp_dispatch<kSQ>(mode, grid, st, g, p, x, bvec, nrowblk, partials, skip);
+        bp_dispatch<kSQ>(mode, grid, st, B, skip);
     else if (g.method == kKB)
-        bp_dispatch<kKB>(mode, grid, st, g, p, x, bvec, nrowblk, partials, skip);
+        bp_dispatch<kKB>(mode, grid, st, B, skip);
     else
-        bp_dispatch<kLinear>(mode, grid, st, g, p, x, bvec, nrowblk, partials, skip);
+        bp_dispatch<kLinear>(mode, grid, st, B, skip);
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
+__global__ void zero_counter_kernel(int *c) { *c = 0; }
+
+// Persistent grid: as many workgroups as can be resident on the device.
+template <typename K>
+static int resident_grid(K kernel, int threads, int *out) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    AREC_HIP(hipGetDevice(&dev));
+    AREC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    AREC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+    *out = std::max(1, ncu * std::max(1, per_cu));
+    return 0;
+}
+
+int launch_forward_batch(int rpl, const BatchFwd *d_probs, int nprob, int64_t total, int *d_counter,
+                         hipStream_t st) {
+    if (nprob <= 0 || total <= 0) return 0;
+    if (total > INT32_MAX) {
+        set_error("batched forward: too many work items");
+        return AREC_ESIZE;
+    }
+    zero_counter_kernel<<<1, 1, 0, st>>>(d_counter);
+    int grid = 0, rc;
+#define AREC_FWDB(R)                                                                                         \
+    {                                                                                                        \
+        if ((rc = resident_grid(forward_batch_kernel<R>, 64 * kFwdAngles, &grid))) return rc;               \
+        grid = (int)std::min<int64_t>(grid, total);                                                          \
+        forward_batch_kernel<R><<<grid, 64 * kFwdAngles, 0, st>>>(d_probs, nprob, (int)total, d_counter); \
+    }
+    if (rpl == 4) AREC_FWDB(4)
+    else if (rpl == 2) AREC_FWDB(2)
+    else AREC_FWDB(1)
+#undef AREC_FWDB
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int M, int MODE>
+static int bp_batch(const BatchBp *d_probs, int nprob, int total, int *d_counter, hipStream_t st) {
+    int grid = 0, rc;
+    if ((rc = resident_grid(backward_batch_kernel<M, MODE>, kBpThreads, &grid))) return rc;
+    grid = std::min(grid, total);
+    backward_batch_kernel<M, MODE><<<grid, kBpThreads, 0, st>>>(d_probs, nprob, total, d_counter);
+    return 0;
+}
+
+template <int M>
+static int bp_batch_mode(int mode, const BatchBp *d_probs, int nprob, int total, int *d_counter,
+                         hipStream_t st) {
+    if (mode == kBpSirtGrad) return bp_batch<M, kBpSirtGrad>(d_probs, nprob, total, d_counter, st);
+    if (mode == kBpStoreNorm) return bp_batch<M, kBpStoreNorm>(d_probs, nprob, total, d_counter, st);
+    return bp_batch<M, kBpStore>(d_probs, nprob, total, d_counter, st);
+}
+
+int launch_backward_batch(int method, int mode, const BatchBp *d_probs, int nprob, int64_t total,
+                          int *d_counter, hipStream_t st) {
+    if (nprob <= 0 || total <= 0) return 0;
+    if (total > INT32_MAX) {
+        set_error("batched back projection: too many work items");
+        return AREC_ESIZE;
+    }
+    zero_counter_kernel<<<1, 1, 0, st>>>(d_counter);
+    int rc;
+    if (method == kSQ) rc = bp_batch_mode<kSQ>(mode, d_probs, nprob, (int)total, d_counter, st);
+    else if (method == kKB) rc = bp_batch_mode<kKB>(mode, d_probs, nprob, (int)total, d_counter, st);
+    else rc = bp_batch_mode<kLinear>(mode, d_probs, nprob, (int)total, d_counter, st);
+    if (rc) return rc;
     AREC_HIP(hipGetLastError());
     return 0;
 }

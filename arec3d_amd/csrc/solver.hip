@@ -30,7 +30,7 @@ struct SirtState {
     int iter, restarts, done, first, zero_x, update, skip_all, skip_proj;
     int nhist, hist_cap, maxit, warm;
     float lam, tol;
-    double bnorm2, bnorm, relnrm0;
+    double bnorm2, bnorm, relnrm0, judge;
 };
 
 struct CglsState {
@@ -105,49 +105,72 @@ __device__ __forceinline__ void push_hist(arec_hist *h, int cap, int *n, int ite
 }
 
 // ------------------------------------------------------------------ SIRT --
+// The SIRT kernels run one or many independent problems: problem k is
+// blockIdx.x (control) or blockIdx.y (vector passes), with its own state,
+// vectors and history.  A vector pass of problem k uses v[k].grid
+// workgroups (its single-problem grid), so every problem's partial sums have
+// the same order in a batch as alone.
 
-__global__ void sirt_begin(SirtState *st) {
+struct SirtVec {
+    float *x;
+    const float *bdata;  // A^T b
+    float *grad;
+    const double *bp_part;  // back projector partials, n_bp of them
+    arec_hist *hist;
+    int64_t nnz, n_bp;
+    int grid, pad;
+};
+
+__global__ void sirt_begin(SirtState *S) {
+    SirtState *st = S + blockIdx.x;
     st->skip_all = st->done;
     st->skip_proj = st->done || (st->first && !st->warm);
 }
 
 // First pass after the start or a restart (arecsirt.cpp:96-108):
 // grad = bdata; partials hold sum (double)(float)(b*b) in [0, G) and
-// sum grad*(double)grad in [G, 2G).
-__global__ __launch_bounds__(kVecThreads) void sirt_first_pass(const float *__restrict__ bdata,
-                                                               float *__restrict__ grad, int64_t n,
-                                                               double *__restrict__ part,
-                                                               const SirtState *st) {
-    if (st->skip_all || !st->first) return;
+// sum grad*(double)grad in [G, 2G) (G = the problem's grid).
+__global__ __launch_bounds__(kVecThreads) void sirt_first_pass(const SirtVec *__restrict__ V,
+                                                               double *__restrict__ fp_part,
+                                                               const SirtState *__restrict__ S) {
+    const int k = blockIdx.y;
+    const SirtState *st = S + k;
+    const SirtVec v = V[k];
+    if (st->skip_all || !st->first || (int)blockIdx.x >= v.grid) return;
     const int warm = st->warm;  // warm start: grad comes from the projection pass
     double sb = 0.0, sg = 0.0;
-    GRID_STRIDE(i, n) {
-        float b = bdata[i];
+    for (int64_t i = blockIdx.x * (int64_t)kVecThreads + threadIdx.x; i < v.nnz; i += (int64_t)v.grid * kVecThreads) {
+        float b = v.bdata[i];
         float s = b * b;
         sb += (double)s;
         sg += (double)b * (double)b;
-        if (!warm) grad[i] = b;
+        if (!warm) v.grad[i] = b;
     }
+    double *part = fp_part + (size_t)k * 2 * kVecBlocks;
     block_partial(sb, part);
-    block_partial(sg, part + gridDim.x);
+    block_partial(sg, part + v.grid);
 }
 
-// Decisions of one pass (arecsirt.cpp:121-177).
-__global__ __launch_bounds__(kVecThreads) void sirt_control(SirtState *st,
-                                                            const double *__restrict__ bp_part,
-                                                            int64_t n_bp,
-                                                            const double *__restrict__ fp_part,
-                                                            int n_fp, arec_hist *hist) {
+// Decisions of one pass (arecsirt.cpp:121-177).  A warm start with
+// judge > 0 stops at once (event 3, x untouched) when the given volume's
+// relative residual is not below judge.
+__global__ __launch_bounds__(kVecThreads) void sirt_control(SirtState *S, const SirtVec *__restrict__ V,
+                                                            const double *__restrict__ fp_part) {
+    const int k = blockIdx.x;
+    SirtState *st = S + k;
+    const SirtVec v = V[k];
     if (st->skip_all) return;
     const int first = st->first, warm = st->warm;
+    const double *fp = fp_part + (size_t)k * 2 * kVecBlocks;
     double s_b = 0.0, s_g;
     if (first) {
-        s_b = block_sum(fp_part, n_fp);
-        s_g = warm ? block_sum(bp_part, n_bp) : block_sum(fp_part + n_fp, n_fp);
+        s_b = block_sum(fp, v.grid);
+        s_g = warm ? block_sum(v.bp_part, v.n_bp) : block_sum(fp + v.grid, v.grid);
     } else {
-        s_g = block_sum(bp_part, n_bp);
+        s_g = block_sum(v.bp_part, v.n_bp);
     }
     if (threadIdx.x != 0) return;
+    arec_hist *hist = v.hist;
     if (first) {
         st->bnorm2 += s_b;  // accumulates across restarts (:91,104)
         st->bnorm = sqrt(st->bnorm2);
@@ -157,6 +180,11 @@ __global__ __launch_bounds__(kVecThreads) void sirt_control(SirtState *st,
     double relnrm = rnorm / st->bnorm;
     st->zero_x = 0;
     st->update = 0;
+    if (warm && st->judge > 0.0 && relnrm >= st->judge) {
+        push_hist(hist, st->hist_cap, &st->nhist, st->iter, 3, relnrm, rnorm);
+        st->done = 1;
+        return;
+    }
     if (relnrm > st->relnrm0) {
         if (st->restarts > 20) {
             push_hist(hist, st->hist_cap, &st->nhist, st->iter, 2, relnrm, rnorm);
@@ -186,19 +214,33 @@ __global__ __launch_bounds__(kVecThreads) void sirt_control(SirtState *st,
 }
 
 // x += lam*grad (FP32), or x = 0 after a restart (arecsirt.cpp:145-177).
-__global__ __launch_bounds__(kVecThreads) void sirt_update(float *__restrict__ x,
-                                                           const float *__restrict__ grad, int64_t n,
-                                                           const SirtState *st) {
-    if (st->skip_all) return;
+__global__ __launch_bounds__(kVecThreads) void sirt_update(const SirtVec *__restrict__ V,
+                                                           const SirtState *__restrict__ S) {
+    const int k = blockIdx.y;
+    const SirtState *st = S + k;
+    const SirtVec v = V[k];
+    if (st->skip_all || (int)blockIdx.x >= v.grid) return;
+    const int64_t stride = (int64_t)v.grid * kVecThreads;
     if (st->zero_x) {
-        GRID_STRIDE(i, n) x[i] = 0.0f;
+        for (int64_t i = blockIdx.x * (int64_t)kVecThreads + threadIdx.x; i < v.nnz; i += stride) v.x[i] = 0.0f;
     } else if (st->update) {
         const float lam = st->lam;
-        GRID_STRIDE(i, n) {
-            float t = lam * grad[i];
-            x[i] = x[i] + t;
+        for (int64_t i = blockIdx.x * (int64_t)kVecThreads + threadIdx.x; i < v.nnz; i += stride) {
+            float t = lam * v.grad[i];
+            v.x[i] = v.x[i] + t;
         }
     }
+}
+
+// *all = 1 when every problem is done.
+__global__ void sirt_all_done(const SirtState *__restrict__ S, int nprob, int *all) {
+    __shared__ int s_open;
+    if (threadIdx.x == 0) s_open = 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < nprob; k += blockDim.x)
+        if (!S[k].done) atomicOr(&s_open, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) *all = !s_open;
 }
 
 // ------------------------------------------------------------------ CGLS --
@@ -347,48 +389,178 @@ struct PassRunner {
 
 }  // namespace
 
-int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float lam, int maxit,
-             float tol, int warm, arec_hist *hist, int hist_cap, int *nhist, float *lam_out,
-             bool use_graph) {
-    const int64_t nnz = g.nnz, nproj = g.nproj;
-    DevBuf bdata, grad, proj, bp_part, fp_part, dstate, dhist;
+// SIRT over nprob independent problems (nprob = 1: the reference's solver).
+// One pass = forward of every live problem (persistent launches, one per
+// forward rows-per-lane class; a single problem uses the plain launch), back
+// projection with the gradient epilogue, then the control and update kernels
+// of all problems.  The pass is captured once as a hipGraph and replayed
+// until every problem is done.
+int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit, float tol, int warm,
+                   double judge, arec_hist *hist, int hist_cap, int *nhist, float *lam_out,
+                   bool use_graph) {
+    const int nprob = (int)P.size();
+    if (nprob == 0) return 0;
+    const int method = P[0].g->method;
+    for (const SirtProblem &q : P)
+        if (q.g->method != method) {
+            set_error("batched SIRT: all problems must use the same projector method");
+            return AREC_EINVAL;
+        }
+    // workspace offsets
+    std::vector<int64_t> o_nnz(nprob + 1, 0), o_proj(nprob + 1, 0), o_bp(nprob + 1, 0);
+    for (int k = 0; k < nprob; k++) {
+        o_nnz[k + 1] = o_nnz[k] + P[k].g->nnz;
+        o_proj[k + 1] = o_proj[k] + P[k].g->nproj;
+        o_bp[k + 1] = o_bp[k] + std::max<int64_t>(1, backward_partial_count(*P[k].g));
+    }
+    const int cap = std::max(1, hist_cap);
+    DevBuf bdata, grad, proj, bp_part, fp_part, dstate, dhist, dvec, dall, dfwd, dbp, dbp0, dcnt;
     int rc;
-    if ((rc = alloc(bdata, sizeof(float) * nnz)) || (rc = alloc(grad, sizeof(float) * nnz)) ||
-        (rc = alloc(proj, sizeof(float) * nproj)) ||
-        (rc = alloc(bp_part, sizeof(double) * std::max<int64_t>(1, backward_partial_count(g)))) ||
-        (rc = alloc(fp_part, sizeof(double) * 2 * kVecBlocks)) ||
-        (rc = alloc(dstate, sizeof(SirtState))) ||
-        (rc = alloc(dhist, sizeof(arec_hist) * std::max(1, hist_cap))))
+    if ((rc = alloc(bdata, sizeof(float) * o_nnz[nprob])) || (rc = alloc(grad, sizeof(float) * o_nnz[nprob])) ||
+        (rc = alloc(proj, sizeof(float) * o_proj[nprob])) || (rc = alloc(bp_part, sizeof(double) * o_bp[nprob])) ||
+        (rc = alloc(fp_part, sizeof(double) * 2 * kVecBlocks * nprob)) ||
+        (rc = alloc(dstate, sizeof(SirtState) * nprob)) || (rc = alloc(dhist, sizeof(arec_hist) * cap * nprob)) ||
+        (rc = alloc(dvec, sizeof(SirtVec) * nprob)) || (rc = alloc(dall, sizeof(int))) ||
+        (rc = alloc(dcnt, sizeof(int))))
         return rc;
-    SirtState s0 = {};
-    s0.iter = 1;
-    s0.first = 1;
-    s0.lam = lam;
-    s0.tol = tol;
-    s0.maxit = maxit;
-    s0.hist_cap = hist_cap;
-    s0.relnrm0 = 1.00001;
-    s0.done = maxit < 1;
-    s0.warm = warm != 0;
     SirtState *S = dstate.as<SirtState>();
-    AREC_HIP(hipMemcpyAsync(S, &s0, sizeof s0, hipMemcpyHostToDevice, st));
-    // x = 0 (arecsirt.cpp:53-56) unless warm-started; bdata = A^T b (:98)
-    if (!warm && (rc = fill(x, 0.0f, nnz, st, nullptr))) return rc;
-    if ((rc = launch_backward(g, b, bdata.as<float>(), kBpStore, nullptr, nullptr, st, nullptr))) return rc;
+    std::vector<SirtState> s0(nprob);
+    std::vector<SirtVec> vec(nprob);
+    for (int k = 0; k < nprob; k++) {
+        SirtState &z = s0[k];
+        z = SirtState{};
+        z.iter = 1;
+        z.first = 1;
+        z.lam = P[k].lam;
+        z.tol = tol;
+        z.maxit = maxit;
+        z.hist_cap = hist_cap;
+        z.relnrm0 = 1.00001;
+        z.done = maxit < 1;
+        z.warm = warm != 0;
+        z.judge = judge;
+        SirtVec &v = vec[k];
+        v.x = P[k].x;
+        v.bdata = bdata.as<float>() + o_nnz[k];
+        v.grad = grad.as<float>() + o_nnz[k];
+        v.bp_part = bp_part.as<double>() + o_bp[k];
+        v.hist = dhist.as<arec_hist>() + (size_t)cap * k;
+        v.nnz = P[k].g->nnz;
+        v.n_bp = backward_partial_count(*P[k].g);
+        v.grid = (int)vec_grid(v.nnz).x;
+        v.pad = 0;
+    }
+    AREC_HIP(hipMemcpyAsync(S, s0.data(), sizeof(SirtState) * nprob, hipMemcpyHostToDevice, st));
+    AREC_HIP(hipMemcpyAsync(dvec.p, vec.data(), sizeof(SirtVec) * nprob, hipMemcpyHostToDevice, st));
 
-    const int fpG = (int)vec_grid(nnz).x;
+    // Projector launches of a pass.  Batched: descriptors per forward rpl
+    // class and one back descriptor table, problems in decreasing order of
+    // per-item work so that the long items start first.
+    struct FwdClass {
+        int rpl;
+        int n;
+        int64_t total;
+        BatchFwd *d;
+    };
+    std::vector<FwdClass> fcls;
+    int64_t bp_total = 0;
+    std::vector<BatchBp> hb0;  // A^T b (store)
+    std::vector<BatchBp> hb;   // gradient pass
+    std::vector<BatchFwd> hf_all;
+    if (nprob > 1) {
+        std::vector<int> order(nprob);
+        for (int k = 0; k < nprob; k++) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            const Geometry &ga = *P[a].g, &gb = *P[b].g;
+            return (int64_t)ga.nvalid / ga.nseg > (int64_t)gb.nvalid / gb.nseg;
+        });
+        for (int rpl : {4, 2, 1}) {
+            FwdClass c{rpl, 0, 0, nullptr};
+            for (int k : order) {
+                const Geometry &g = *P[k].g;
+                if (g.fwd_rpl != rpl) continue;
+                BatchFwd f;
+                f.a = fwd_args(g, P[k].x, proj.as<float>() + o_proj[k], nullptr);
+                f.skip = &S[k].skip_proj;
+                f.item0 = (int)c.total;
+                f.pad = 0;
+                hf_all.push_back(f);
+                c.total += forward_items(g);
+                c.n++;
+            }
+            if (c.n) fcls.push_back(c);
+        }
+        if ((rc = alloc(dfwd, sizeof(BatchFwd) * hf_all.size()))) return rc;
+        AREC_HIP(hipMemcpyAsync(dfwd.p, hf_all.data(), sizeof(BatchFwd) * hf_all.size(), hipMemcpyHostToDevice, st));
+        size_t off = 0;
+        for (FwdClass &c : fcls) {
+            c.d = dfwd.as<BatchFwd>() + off;
+            off += c.n;
+        }
+        std::vector<int> border(nprob);
+        for (int k = 0; k < nprob; k++) border[k] = k;
+        std::stable_sort(border.begin(), border.end(), [&](int a, int b) { return P[a].g->na > P[b].g->na; });
+        for (int k : border) {
+            const Geometry &g = *P[k].g;
+            BatchBp e;
+            e.a = bp_args(g, P[k].b, bdata.as<float>() + o_nnz[k], nullptr, nullptr);
+            e.skip = nullptr;
+            e.item0 = (int)bp_total;
+            e.pad = 0;
+            hb0.push_back(e);
+            e.a = bp_args(g, proj.as<float>() + o_proj[k], vec[k].grad, vec[k].bdata, bp_part.as<double>() + o_bp[k]);
+            e.skip = &S[k].skip_proj;
+            hb.push_back(e);
+            bp_total += backward_items(g);
+        }
+        if ((rc = alloc(dbp0, sizeof(BatchBp) * nprob)) || (rc = alloc(dbp, sizeof(BatchBp) * nprob))) return rc;
+        AREC_HIP(hipMemcpyAsync(dbp0.p, hb0.data(), sizeof(BatchBp) * nprob, hipMemcpyHostToDevice, st));
+        AREC_HIP(hipMemcpyAsync(dbp.p, hb.data(), sizeof(BatchBp) * nprob, hipMemcpyHostToDevice, st));
+    }
+
+    // Columns that never contribute (linear pair's last two) stay zero in
+    // every vector the back projector writes.
+    for (int k = 0; k < nprob; k++) {
+        const Geometry &g = *P[k].g;
+        if (g.nvalid < g.nrays) {
+            int64_t n0 = (int64_t)g.nvalid * g.ny, n1 = (int64_t)(g.nrays - g.nvalid) * g.ny;
+            if ((rc = fill(vec[k].grad + n0, 0.0f, n1, st, nullptr)) ||
+                (rc = fill(bdata.as<float>() + o_nnz[k] + n0, 0.0f, n1, st, nullptr)))
+                return rc;
+        }
+        // x = 0 (arecsirt.cpp:53-56) unless warm-started
+        if (!warm && (rc = fill(P[k].x, 0.0f, P[k].g->nnz, st, nullptr))) return rc;
+    }
+    // bdata = A^T b (:98)
+    if (nprob == 1) {
+        if ((rc = launch_backward(*P[0].g, P[0].b, bdata.as<float>(), kBpStore, nullptr, nullptr, st, nullptr)))
+            return rc;
+    } else if ((rc = launch_backward_batch(method, kBpStore, dbp0.as<BatchBp>(), nprob, bp_total, dcnt.as<int>(), st))) {
+        return rc;
+    }
+
+    const SirtVec *V = dvec.as<SirtVec>();
+    int gmax = 1;
+    for (const SirtVec &v : vec) gmax = std::max(gmax, v.grid);
     auto pass = [&]() -> int {
-        sirt_begin<<<1, 1, 0, st>>>(S);
+        sirt_begin<<<nprob, 1, 0, st>>>(S);
         int r;
-        if ((r = launch_forward(g, x, proj.as<float>(), nullptr, st, &S->skip_proj))) return r;
-        if ((r = launch_backward(g, proj.as<float>(), grad.as<float>(), kBpSirtGrad, bdata.as<float>(),
-                                 bp_part.as<double>(), st, &S->skip_proj)))
-            return r;
-        sirt_first_pass<<<fpG, kVecThreads, 0, st>>>(bdata.as<float>(), grad.as<float>(), nnz,
-                                                     fp_part.as<double>(), S);
-        sirt_control<<<1, kVecThreads, 0, st>>>(S, bp_part.as<double>(), backward_partial_count(g),
-                                                fp_part.as<double>(), fpG, dhist.as<arec_hist>());
-        sirt_update<<<vec_grid(nnz), kVecThreads, 0, st>>>(x, grad.as<float>(), nnz, S);
+        if (nprob == 1) {
+            const Geometry &g = *P[0].g;
+            if ((r = launch_forward(g, P[0].x, proj.as<float>(), nullptr, st, &S->skip_proj))) return r;
+            if ((r = launch_backward(g, proj.as<float>(), vec[0].grad, kBpSirtGrad, vec[0].bdata,
+                                     bp_part.as<double>(), st, &S->skip_proj)))
+                return r;
+        } else {
+            for (const FwdClass &c : fcls)
+                if ((r = launch_forward_batch(c.rpl, c.d, c.n, c.total, dcnt.as<int>(), st))) return r;
+            if ((r = launch_backward_batch(method, kBpSirtGrad, dbp.as<BatchBp>(), nprob, bp_total, dcnt.as<int>(), st)))
+                return r;
+        }
+        sirt_first_pass<<<dim3(gmax, nprob), kVecThreads, 0, st>>>(V, fp_part.as<double>(), S);
+        sirt_control<<<nprob, kVecThreads, 0, st>>>(S, V, fp_part.as<double>());
+        sirt_update<<<dim3(gmax, nprob), kVecThreads, 0, st>>>(V, S);
+        sirt_all_done<<<1, 256, 0, st>>>(S, nprob, dall.as<int>());
         AREC_HIP(hipGetLastError());
         return 0;
     };
@@ -425,7 +597,7 @@ int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float 
             }
         }
         passes += batch;
-        hipMemcpyAsync(done_h, &S->done, sizeof(int), hipMemcpyDeviceToHost, st);
+        hipMemcpyAsync(done_h, dall.p, sizeof(int), hipMemcpyDeviceToHost, st);
         hipError_t e = hipStreamSynchronize(st);
         if (e != hipSuccess) {
             hipHostFree(done_h);
@@ -435,13 +607,23 @@ int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float 
         batch = std::min(batch * 2, 32);
     }
     hipHostFree(done_h);
-    SirtState sf;
-    AREC_HIP(hipMemcpy(&sf, S, sizeof sf, hipMemcpyDeviceToHost));
-    int n = std::min(sf.nhist, hist_cap);
-    if (hist && n > 0) AREC_HIP(hipMemcpy(hist, dhist.p, sizeof(arec_hist) * n, hipMemcpyDeviceToHost));
-    if (nhist) *nhist = sf.nhist;
-    if (lam_out) *lam_out = sf.lam;
+    std::vector<SirtState> sf(nprob);
+    AREC_HIP(hipMemcpy(sf.data(), S, sizeof(SirtState) * nprob, hipMemcpyDeviceToHost));
+    for (int k = 0; k < nprob; k++) {
+        int n = std::min(sf[k].nhist, hist_cap);
+        if (hist && n > 0)
+            AREC_HIP(hipMemcpy(hist + (size_t)hist_cap * k, vec[k].hist, sizeof(arec_hist) * n, hipMemcpyDeviceToHost));
+        if (nhist) nhist[k] = sf[k].nhist;
+        if (lam_out) lam_out[k] = sf[k].lam;
+    }
     return 0;
+}
+
+int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float lam, int maxit,
+             float tol, int warm, double judge, arec_hist *hist, int hist_cap, int *nhist, float *lam_out,
+             bool use_graph) {
+    std::vector<SirtProblem> P{SirtProblem{&g, b, x, lam}};
+    return sirt_batch_run(P, st, maxit, tol, warm, judge, hist, hist_cap, nhist, lam_out, use_graph);
 }
 
 int cgls_run(const Geometry &g, hipStream_t st, const float *b, const float *w, float *x, int warm,

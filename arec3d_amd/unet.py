@@ -14,6 +14,8 @@ unet.hip (no counterpart in the reference).
   and judges the network), then continues SIRT to the tolerance.
 """
 import ctypes as C
+import json
+import os
 import time
 
 import numpy as np
@@ -182,50 +184,69 @@ def normal_residual(ctx, g, b, x, bnorm, tmp_p=None, tmp_x=None):
     return np.sqrt(dot(ctx, tmp_x, tmp_x)) / bnorm
 
 
-def warm_sirt(ctx, g, b, x, lam, tol, params, scale, k_in=5, maxit=3000, net=None, log=None):
-    """Warm-started SIRT: k_in SIRT iterations, the network, then SIRT from the
-    better of (network output, k_in iterate) to relnrm <= tol.  Returns
-    (histories, seconds per phase, chosen start)."""
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_sq128.npz")
+
+
+def load_weights(path=None):
+    """(params, scale, meta) of a trained network (numpy, no pickles)."""
+    with np.load(path or DEFAULT_WEIGHTS, allow_pickle=False) as z:
+        return z["params"].astype(np.float32), float(z["scale"]), json.loads(str(z["meta"]))
+
+
+class WarmStart:
+    """The trained network and its buffers for one geometry's volume."""
+
+    def __init__(self, ctx, g, params=None, scale=None):
+        if params is None:
+            params, scale, _ = load_weights()
+        n = g.nx
+        self.ctx, self.g = ctx, g
+        self.cube_in = ctx.empty((g.ny, n, n))
+        self.cube_out = ctx.empty((g.ny, n, n))
+        self.x_keep = ctx.empty(g.vol_shape)
+        self.net = UNet(ctx, g.ny, n, n, training=False, params=params, scale=scale)
+
+    def apply(self, x):
+        """x <- network(x) (in place, cylinder layout)."""
+        cyl_to_cube(self.ctx, x, self.g.radius, self.g.ny, self.cube_in)
+        self.net.forward(self.cube_in, self.cube_out)
+        cube_to_cyl(self.ctx, self.cube_out, self.g.radius, self.g.ny, x)
+
+    def close(self):
+        self.net.close()
+
+
+def warm_sirt(ctx, g, b, x, lam, tol, ws, k_in=5, maxit=3000, timing=False):
+    """Warm-started SIRT: k_in SIRT iterations from zero, the network, then
+    SIRT from its output to relnrm <= tol.  SIRT judges the network: its
+    first pass computes the output's relative residual and rejects it if it
+    is not below the k_in run's last one; SIRT then continues from the k_in
+    iterate instead (the fallback).  Residuals use the fixed denominator
+    ||A^T b|| throughout.  Returns ((hist_in, hist_out), phase seconds,
+    'network' | 'sirt')."""
+    from . import copy as dcopy
+
     t = {}
     t0 = time.perf_counter()
     h1, _ = g.sirt(b, x, lam, k_in, 0.0)
-    ctx.synchronize()
-    t["sirt_in"] = time.perf_counter() - t0
-    n = g.nx
-    t0 = time.perf_counter()
-    cube_in = ctx.empty((g.ny, n, n))
-    cube_out = ctx.empty((g.ny, n, n))
-    cyl_to_cube(ctx, x, g.radius, g.ny, cube_in)
-    own = net is None
-    if own:
-        net = UNet(ctx, g.ny, n, n, training=False, params=params, scale=scale)
-    net.forward(cube_in, cube_out)
-    xn = ctx.empty(g.vol_shape)
-    cube_to_cyl(ctx, cube_out, g.radius, g.ny, xn)
-    ctx.synchronize()
-    t["net"] = time.perf_counter() - t0
-    if own:
-        net.close()
-    t0 = time.perf_counter()
-    # SIRT judges the network: one SIRT pass from its output gives its
-    # residual; if that is not below the k_in iterate's, continue from the
-    # SIRT iterate instead (SIRT remains the fallback).
-    from . import copy as dcopy
-
-    x_keep = ctx.empty(g.vol_shape)
-    dcopy(ctx, x_keep, x)
-    dcopy(ctx, x, xn)
-    hp, _ = g.sirt_warm(b, x, lam, 1, tol)
+    if timing:
+        ctx.synchronize()
+        t["sirt_in"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+    dcopy(ctx, ws.x_keep, x)
+    ws.apply(x)
+    if timing:
+        ctx.synchronize()
+        t["net"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+    h2, _ = g.sirt_warm(b, x, lam, maxit, tol, hist_cap=maxit + 64, reject_above=h1[-1][2])
     chosen = "network"
-    if hp[0][2] >= h1[-1][2]:
-        dcopy(ctx, x, x_keep)
+    if h2 and h2[0][1] == 3:
         chosen = "sirt"
-        hp = []
-    if not hp or hp[-1][1] != 2:
-        h2, _ = g.sirt_warm(b, x, lam, maxit, tol)
-    else:
-        h2 = []
-    h2 = hp + h2
+        dcopy(ctx, x, ws.x_keep)
+        h3, _ = g.sirt_warm(b, x, lam, maxit, tol, hist_cap=maxit + 64)
+        h2 = h2 + h3
     ctx.synchronize()
-    t["sirt_out"] = time.perf_counter() - t0
+    if timing:
+        t["sirt_out"] = time.perf_counter() - t0
     return (h1, h2), t, chosen

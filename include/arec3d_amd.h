@@ -52,7 +52,8 @@ typedef struct arec_geom arec_geom;
 
 /* One line of a solver's residual history, as the reference prints it
  * (arecsirt.cpp:131-132, areccgls.cpp:197-198).  event: 0 iteration,
- * 1 SIRT restart with lam halved, 2 termination. */
+ * 1 SIRT restart with lam halved, 2 termination, 3 warm start rejected
+ * (arec_sirt_warm with reject_above; no counterpart in the reference). */
 typedef struct {
     int32_t iter;
     int32_t event;
@@ -87,6 +88,14 @@ int arec_geom_destroy(arec_geom *geom);
 /* nrays, nnz = nrays*ny, nproj = nx*ny*na; any pointer may be NULL. */
 int arec_geom_info(const arec_geom *geom, int64_t *nrays, int64_t *nnz, int64_t *nproj);
 int64_t arec_geom_device_bytes(const arec_geom *geom);
+/* Launch plan of the forward projector (no counterpart in the reference):
+ * detector rows per lane (1, 2 or 4; 0 = default: 4 if ny >= 256 and
+ * ny % 4 == 0, else 2 if ny >= 128 and even, else 1) and the minimum number
+ * of workgroups (detector segments are added until the grid reaches it;
+ * 0 = 1024).  Results do not depend on the plan.  The autotuner of the
+ * Python package picks it by timing. */
+int arec_geom_plan(arec_ctx *ctx, arec_geom *geom, int32_t rows_per_lane, int32_t min_groups);
+int arec_geom_get_plan(const arec_geom *geom, int32_t *rows_per_lane, int32_t *min_groups, int32_t *nseg);
 
 /* ---- device-resident operators --------------------------------------- */
 /* p = A x   (arecProject2D / _KB / _SQ, arecproject.cpp:19-99,351-437,540-648) */
@@ -106,10 +115,26 @@ int arec_sirt(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x
  * reference, whose SIRT always starts from zero, arecsirt.cpp:53-56).  The
  * relative residual keeps the reference's fixed denominator ||A^T b||; the
  * first printed line is the residual of the given volume; a restart still
- * zeroes x.  Used for the learned warm start. */
+ * zeroes x.  reject_above > 0 makes SIRT judge the given volume: if its
+ * relative residual is >= reject_above the call returns at once with one
+ * history line of event 3 and x_dev unchanged.  Used for the learned warm
+ * start. */
 int arec_sirt_warm(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x_dev, float lam,
-                   int32_t maxit, float tol, arec_hist *hist, int32_t hist_cap, int32_t *nhist,
-                   float *lam_out);
+                   int32_t maxit, float tol, double reject_above, arec_hist *hist, int32_t hist_cap,
+                   int32_t *nhist, float *lam_out);
+
+/* Batched SIRT over nprob independent problems (no counterpart in the
+ * reference; each problem is one cyl_sirt* call, arecsirt.cpp:11-191), all
+ * with the same projector method and the same maxit/tol, each with its own
+ * geometry, data b_dev[k], volume x_dev[k] (zeroed) and lam[k].  One pass
+ * runs every live problem's forward and back projection as persistent
+ * work-queue launches (one forward launch per rows-per-lane class), then
+ * the per-problem control; the pass is one hipGraph.  Every problem's
+ * history equals a single arec_sirt call's bit for bit.  hist: nprob blocks
+ * of hist_cap lines; nhist and lam_out: nprob entries (either may be NULL). */
+int arec_sirt_batch(arec_ctx *ctx, int32_t nprob, const arec_geom *const *geoms, const float *const *b_dev,
+                    float *const *x_dev, const float *lam, int32_t maxit, float tol, arec_hist *hist,
+                    int32_t hist_cap, int32_t *nhist, float *lam_out);
 
 /* CGLS of cyl_cgls / _KB / _SQ (areccgls.cpp:12-243,245-478,
  * areccgls_KB.cpp:12-240).  warm != 0 starts from x_dev (reference:
