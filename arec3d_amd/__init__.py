@@ -76,7 +76,7 @@ def _load():
         "arec_align_shift": ([vp, vp, i32, i32, i32, vp, vp, flt], C.c_int),
         "arec_h_cc_images": ([i32, i32, i32, vp, i32, i32, i32, vp, vp, vp], C.c_int),
         "arec_h_shift_images": ([i32, i32, i32, vp, vp, vp], C.c_int),
-        "arec_sirt_warm": ([vp, vp, vp, vp, flt, i32, flt, dbl, P(Hist), i32, P(i32), P(flt)], C.c_int),
+        "arec_sirt_warm": ([vp, vp, vp, vp, flt, i32, flt, dbl, i32, P(Hist), i32, P(i32), P(flt)], C.c_int),
         "arec_sirt_batch": ([vp, i32, P(vp), P(vp), P(vp), P(flt), i32, flt, P(Hist), i32, P(i32), P(flt)],
                             C.c_int),
         "arec_geom_plan": ([vp, vp, i32, i32], C.c_int),
@@ -275,15 +275,16 @@ class Geometry:
         _check(lib.arec_sirt(self.ctx.h, self.h, b.ptr, x.ptr, lam, maxit, tol, H, cap, C.byref(nh), C.byref(lo)))
         return _hist(H, nh.value), lo.value
 
-    def sirt_warm(self, b, x, lam, maxit, tol, hist_cap=None, reject_above=0.0):
+    def sirt_warm(self, b, x, lam, maxit, tol, hist_cap=None, reject_above=0.0, check_pass=1):
         """SIRT continued from the volume in x (fixed ||A^T b|| denominator).
-        With reject_above > 0 a start whose relative residual is not below it
-        is rejected at once (one history line with event 3, x unchanged)."""
+        With reject_above > 0 the start is rejected (last history line has
+        event 3) if the relative residual printed at pass check_pass is not
+        below reject_above or a restart comes first."""
         cap = hist_cap or (maxit + 64)
         H = (Hist * cap)()
         nh, lo = C.c_int32(), C.c_float()
-        _check(lib.arec_sirt_warm(self.ctx.h, self.h, b.ptr, x.ptr, lam, maxit, tol, float(reject_above), H, cap,
-                                  C.byref(nh), C.byref(lo)))
+        _check(lib.arec_sirt_warm(self.ctx.h, self.h, b.ptr, x.ptr, lam, maxit, tol, float(reject_above),
+                                  int(check_pass), H, cap, C.byref(nh), C.byref(lo)))
         return _hist(H, nh.value), lo.value
 
     def plan(self, rows_per_lane=0, min_groups=0):

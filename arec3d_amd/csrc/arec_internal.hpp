@@ -158,9 +158,11 @@ struct SirtProblem {
     float lam;
 };
 int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit, float tol, int warm,
-                   double judge, arec_hist *hist, int hist_cap, int *nhist, float *lam_out, bool use_graph);
+                   double reject_above, int check_pass, arec_hist *hist, int hist_cap, int *nhist,
+                   float *lam_out, bool use_graph);
 int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float lam, int maxit, float tol,
-             int warm, double judge, arec_hist *hist, int hist_cap, int *nhist, float *lam_out, bool use_graph);
+             int warm, double reject_above, int check_pass, arec_hist *hist, int hist_cap, int *nhist,
+             float *lam_out, bool use_graph);
 
 // Fill with a constant by a kernel (graph-capture safe); skip as above.
 int fill(float *x, float v, int64_t n, hipStream_t st, const int *skip);

@@ -181,17 +181,18 @@ int arec_sirt(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x
               float *lam_out) {
     CHECK_ARG(ctx && geom && b_dev && x_dev, "NULL argument");
     CHECK_ARG(hist_cap >= 0 && (hist || hist_cap == 0), "bad history buffer");
-    return sirt_run(geom->g, ctx->stream, b_dev, x_dev, lam, maxit, tol, 0, 0.0, hist, hist_cap, nhist,
+    return sirt_run(geom->g, ctx->stream, b_dev, x_dev, lam, maxit, tol, 0, 0.0, 1, hist, hist_cap, nhist,
                     lam_out, use_graphs());
 }
 
 int arec_sirt_warm(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x_dev, float lam,
-                   int32_t maxit, float tol, double reject_above, arec_hist *hist, int32_t hist_cap,
-                   int32_t *nhist, float *lam_out) {
+                   int32_t maxit, float tol, double reject_above, int32_t check_pass, arec_hist *hist,
+                   int32_t hist_cap, int32_t *nhist, float *lam_out) {
     CHECK_ARG(ctx && geom && b_dev && x_dev, "NULL argument");
     CHECK_ARG(hist_cap >= 0 && (hist || hist_cap == 0), "bad history buffer");
-    return sirt_run(geom->g, ctx->stream, b_dev, x_dev, lam, maxit, tol, 1, reject_above, hist, hist_cap,
-                    nhist, lam_out, use_graphs());
+    CHECK_ARG(check_pass >= 1, "check_pass must be >= 1");
+    return sirt_run(geom->g, ctx->stream, b_dev, x_dev, lam, maxit, tol, 1, reject_above, check_pass, hist,
+                    hist_cap, nhist, lam_out, use_graphs());
 }
 
 int arec_sirt_batch(arec_ctx *ctx, int32_t nprob, const arec_geom *const *geoms, const float *const *b_dev,
@@ -204,7 +205,7 @@ int arec_sirt_batch(arec_ctx *ctx, int32_t nprob, const arec_geom *const *geoms,
         CHECK_ARG(geoms[k] && b_dev[k] && x_dev[k], "NULL problem");
         P.push_back(SirtProblem{&geoms[k]->g, b_dev[k], x_dev[k], lam[k]});
     }
-    return sirt_batch_run(P, ctx->stream, maxit, tol, 0, 0.0, hist, hist_cap, nhist, lam_out, use_graphs());
+    return sirt_batch_run(P, ctx->stream, maxit, tol, 0, 0.0, 1, hist, hist_cap, nhist, lam_out, use_graphs());
 }
 
 int arec_geom_plan(arec_ctx *ctx, arec_geom *geom, int32_t rows_per_lane, int32_t min_groups) {

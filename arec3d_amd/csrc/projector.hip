@@ -159,13 +159,15 @@ constexpr int kFwdBatch = 8;  // table entries whose loads are in flight togethe
 // Workgroup b of one forward application: angle group b % nag (fastest),
 // then detector segment, then row chunk.  Every wave of the workgroup calls
 // this with the same b.
+// The buffers are __restrict__ parameters (not struct members) so that the
+// compiler may keep the batched column loads ahead of the strip stores.
 template <int RPL>
-__device__ __forceinline__ void forward_body(const FwdArgs &A, int b) {
-    const float *__restrict__ x = A.x;
-    const uint4 *__restrict__ table = A.table;
-    const int *__restrict__ pixstart = A.pixstart;
-    const int npix = A.npix, nvalid = A.nvalid, nseg = A.nseg, segw = A.segw, nx = A.nx, ny = A.ny,
-              na = A.na, nchunks = A.nchunks;
+__device__ __forceinline__ void forward_body(const float *__restrict__ x, float *__restrict__ p,
+                                             const uint4 *__restrict__ table, const int4 *__restrict__ seg,
+                                             const int *__restrict__ pixstart, double *__restrict__ partials,
+                                             const int npix, const int nvalid, const int nseg, const int segw,
+                                             const int nx, const int ny, const int na, const int nchunks,
+                                             int b) {
     constexpr int ROWS = 64 * RPL;
     constexpr int LDW = kFwdOut + 1;  // padded strip row (conflict-free column writes)
     __shared__ float s_out[kFwdAngles][ROWS * LDW];
@@ -181,9 +183,9 @@ __device__ __forceinline__ void forward_body(const FwdArgs &A, int b) {
     const int u_lo = sg * segw, u_hi = min(nx, u_lo + segw);
     const int ybase = chunk * ROWS;
     const int y0 = ybase + lane * RPL;
-    float *__restrict__ pa = A.p + (size_t)a * nx * ny;
+    float *__restrict__ pa = p + (size_t)a * nx * ny;
     const uint4 *__restrict__ tab = table + (size_t)a * nvalid;
-    const int4 sd = A.seg[(size_t)a * nseg + sg];
+    const int4 sd = seg[(size_t)a * nseg + sg];
     const int i0 = __builtin_amdgcn_readfirstlane(sd.x), i1 = __builtin_amdgcn_readfirstlane(sd.y);
     float *strip = s_out[wave];
 
@@ -316,15 +318,26 @@ __device__ __forceinline__ void forward_body(const FwdArgs &A, int b) {
     // the last window and the pixels above it up to the segment end
     advance(max(base + 3, u_hi));
     while (strip_lo < u_hi) flush_strip();
-    if (A.partials) {
+    if (partials) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
-        if (lane == 0) A.partials[((size_t)a * nchunks + chunk) * nseg + sg] = sq;
+        if (lane == 0) partials[((size_t)a * nchunks + chunk) * nseg + sg] = sq;
     }
 }
 
 template <int RPL>
-__global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(FwdArgs A, const int *__restrict__ skip) {
+__device__ __forceinline__ void forward_item(const FwdArgs &A, int b) {
+    forward_body<RPL>(A.x, A.p, A.table, A.seg, A.pixstart, A.partials, A.npix, A.nvalid, A.nseg, A.segw,
+                      A.nx, A.ny, A.na, A.nchunks, b);
+}
+
+// Plain kernel arguments (not a struct): the compiler then keeps the sizes in
+// scalar registers (68 VGPRs instead of 89 at four rows per lane).
+template <int RPL>
+__global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(
+    const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ table,
+    const int4 *__restrict__ seg, const int *__restrict__ pixstart, double *__restrict__ partials, int npix,
+    int nvalid, int nseg, int segw, int nx, int ny, int na, int nchunks, const int *__restrict__ skip) {
     if (skip && *skip) return;
     // Workgroups with consecutive ids run on different XCDs: give each XCD a
     // contiguous range of (chunk, segment, angle group) triples, angle group
@@ -334,7 +347,7 @@ __global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(FwdArgs A, con
     const int nblk = gridDim.x;
     int b = blockIdx.x;
     if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
-    forward_body<RPL>(A, b);
+    forward_body<RPL>(x, p, table, seg, pixstart, partials, npix, nvalid, nseg, segw, nx, ny, na, nchunks, b);
 }
 
 // Index of the problem that owns work item `it`: the last k with
@@ -362,12 +375,14 @@ __global__ __launch_bounds__(64 * kFwdAngles) void forward_batch_kernel(const Ba
         __syncthreads();  // every wave has read the previous item
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
-        const int it = s_item;
+        // uniform: lets the compiler read the problem's arguments into
+        // scalar registers
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
         if (it >= total) break;
-        const int k = owner_of(probs, nprob, it);
+        const int k = __builtin_amdgcn_readfirstlane(owner_of(probs, nprob, it));
         const int *skip = probs[k].skip;
         if (skip && *skip) continue;
-        forward_body<RPL>(probs[k].a, it - probs[k].item0);
+        forward_item<RPL>(probs[k].a, it - probs[k].item0);
     }
 }
 
@@ -396,13 +411,12 @@ __device__ __forceinline__ double corner_xb(int X, int Z, const AngleConst &A, d
 //  kBpSirtGrad   out = bvec - acc (FP32), partial += sum out*(double)out
 //  kBpStoreNorm  out = acc, partial += sum (double)(acc*acc)
 template <int M, int MODE>
-__device__ __forceinline__ void backward_body(const BpArgs &B, int b) {
-    const float *__restrict__ p = B.p;
-    const int *__restrict__ coord = B.coord;
-    const AngleConst *__restrict__ ang = B.ang;
-    const double radius = B.radius;
-    const int xcent = B.xcent, nx = B.nx, ny = B.ny, na = B.na, nrowblk = B.nrowblk;
-    int *__restrict__ err = B.err;
+__device__ __forceinline__ void backward_body(const float *__restrict__ p, float *__restrict__ out,
+                                              const float *__restrict__ bvec, const int *__restrict__ tile_cols,
+                                              const int *__restrict__ tile_org, const int *__restrict__ coord,
+                                              const AngleConst *__restrict__ ang, double *__restrict__ partials,
+                                              int *__restrict__ err, const double radius, const int xcent,
+                                              const int nx, const int ny, const int na, const int nrowblk, int b) {
     __shared__ __align__(16) float s_p[kBpAngBlk][kBpWin][kBpRowPad];
     __shared__ int s_u0[kBpAngBlk];
     __shared__ double s_red[kBpThreads / 64];
@@ -411,8 +425,8 @@ __device__ __forceinline__ void backward_body(const BpArgs &B, int b) {
     const int tile = b / nrowblk;
     const int rb = b % nrowblk;
     const int ybase = rb * kBpRows;
-    const int j = B.tile_cols[tile * kTileCols + tid];
-    const int X0 = B.tile_org[2 * tile], Z0 = B.tile_org[2 * tile + 1];
+    const int j = tile_cols[tile * kTileCols + tid];
+    const int X0 = tile_org[2 * tile], Z0 = tile_org[2 * tile + 1];
     int cX = 0, cZ = 0;
     if (j >= 0) {
         cX = coord[2 * j];
@@ -494,8 +508,8 @@ __device__ __forceinline__ void backward_body(const BpArgs &B, int b) {
 
     double part = 0.0;
     if (j >= 0) {
-        float *o = B.out + (size_t)j * ny + ybase;
-        const float *bv = B.bvec ? B.bvec + (size_t)j * ny + ybase : nullptr;
+        float *o = out + (size_t)j * ny + ybase;
+        const float *bv = bvec ? bvec + (size_t)j * ny + ybase : nullptr;
 #pragma unroll
         for (int q = 0; q < kBpRows / 4; q++) {
             float v[4] = {acc[q].x, acc[q].y, acc[q].z, acc[q].w};
@@ -524,16 +538,22 @@ __device__ __forceinline__ void backward_body(const BpArgs &B, int b) {
         if (tid == 0) {
             double s = 0.0;
             for (int w = 0; w < kBpThreads / 64; w++) s += s_red[w];
-            B.partials[b] = s;
+            partials[b] = s;
         }
         __syncthreads();  // s_red is reused by the next item of a persistent workgroup
     }
 }
 
 template <int M, int MODE>
+__device__ __forceinline__ void backward_item(const BpArgs &B, int b) {
+    backward_body<M, MODE>(B.p, B.out, B.bvec, B.tile_cols, B.tile_org, B.coord, B.ang, B.partials, B.err,
+                           B.radius, B.xcent, B.nx, B.ny, B.na, B.nrowblk, b);
+}
+
+template <int M, int MODE>
 __global__ __launch_bounds__(kBpThreads) void backward_kernel(BpArgs B, const int *__restrict__ skip) {
     if (skip && *skip) return;
-    backward_body<M, MODE>(B, blockIdx.x);
+    backward_item<M, MODE>(B, blockIdx.x);
 }
 
 // Persistent back projector over many problems (see forward_batch_kernel).
@@ -545,12 +565,12 @@ __global__ __launch_bounds__(kBpThreads) void backward_batch_kernel(const BatchB
         __syncthreads();
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
-        const int it = s_item;
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
         if (it >= total) break;
-        const int k = owner_of(probs, nprob, it);
+        const int k = __builtin_amdgcn_readfirstlane(owner_of(probs, nprob, it));
         const int *skip = probs[k].skip;
         if (skip && *skip) continue;
-        backward_body<M, MODE>(probs[k].a, it - probs[k].item0);
+        backward_item<M, MODE>(probs[k].a, it - probs[k].item0);
     }
 }
 
@@ -707,13 +727,18 @@ int launch_forward(const Geometry &g, const float *x, float *p, double *partials
                    const int *skip) {
     if (g.nvalid == 0) return fill(p, 0.0f, g.nproj, st, skip);
     dim3 grid((unsigned)forward_items(g));
-    FwdArgs A = fwd_args(g, x, p, partials);
+    const FwdArgs A = fwd_args(g, x, p, partials);
+#define AREC_FWD(R)                                                                                         \
+    forward_kernel<R><<<grid, 64 * kFwdAngles, 0, st>>>(A.x, A.p, A.table, A.seg, A.pixstart, A.partials, \
+                                                        A.npix, A.nvalid, A.nseg, A.segw, A.nx, A.ny, A.na, \
+                                                        A.nchunks, skip)
     if (g.fwd_rpl == 4)
-        forward_kernel<4><<<grid, 64 * kFwdAngles, 0, st>>>(A, skip);
+        AREC_FWD(4);
     else if (g.fwd_rpl == 2)
-        forward_kernel<2><<<grid, 64 * kFwdAngles, 0, st>>>(A, skip);
+        AREC_FWD(2);
     else
-        forward_kernel<1><<<grid, 64 * kFwdAngles, 0, st>>>(A, skip);
+        AREC_FWD(1);
+#undef AREC_FWD
     AREC_HIP(hipGetLastError());
     return 0;
 }

@@ -29,8 +29,9 @@ constexpr int kVecBlocks = 1024;
 struct SirtState {
     int iter, restarts, done, first, zero_x, update, skip_all, skip_proj;
     int nhist, hist_cap, maxit, warm;
+    int npass, check_pass;  // passes decided so far; pass at which a warm start is checked
     float lam, tol;
-    double bnorm2, bnorm, relnrm0, judge;
+    double bnorm2, bnorm, relnrm0, reject_above;
 };
 
 struct CglsState {
@@ -152,8 +153,9 @@ __global__ __launch_bounds__(kVecThreads) void sirt_first_pass(const SirtVec *__
 }
 
 // Decisions of one pass (arecsirt.cpp:121-177).  A warm start with
-// judge > 0 stops at once (event 3, x untouched) when the given volume's
-// relative residual is not below judge.
+// reject_above > 0 is checked by SIRT's own residual: it is rejected (event
+// 3, solve ends) if the residual printed at pass check_pass is not below
+// reject_above, or if a restart would come first.
 __global__ __launch_bounds__(kVecThreads) void sirt_control(SirtState *S, const SirtVec *__restrict__ V,
                                                             const double *__restrict__ fp_part) {
     const int k = blockIdx.x;
@@ -175,12 +177,15 @@ __global__ __launch_bounds__(kVecThreads) void sirt_control(SirtState *S, const 
         st->bnorm2 += s_b;  // accumulates across restarts (:91,104)
         st->bnorm = sqrt(st->bnorm2);
     }
+    (void)warm;
     st->warm = 0;
     double rnorm = sqrt(s_g);
     double relnrm = rnorm / st->bnorm;
     st->zero_x = 0;
     st->update = 0;
-    if (warm && st->judge > 0.0 && relnrm >= st->judge) {
+    st->npass++;
+    if (st->reject_above > 0.0 && st->npass <= st->check_pass &&
+        ((st->npass == st->check_pass && relnrm >= st->reject_above) || relnrm > st->relnrm0)) {
         push_hist(hist, st->hist_cap, &st->nhist, st->iter, 3, relnrm, rnorm);
         st->done = 1;
         return;
@@ -396,8 +401,8 @@ struct PassRunner {
 // of all problems.  The pass is captured once as a hipGraph and replayed
 // until every problem is done.
 int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit, float tol, int warm,
-                   double judge, arec_hist *hist, int hist_cap, int *nhist, float *lam_out,
-                   bool use_graph) {
+                   double reject_above, int check_pass, arec_hist *hist, int hist_cap, int *nhist,
+                   float *lam_out, bool use_graph) {
     const int nprob = (int)P.size();
     if (nprob == 0) return 0;
     const int method = P[0].g->method;
@@ -438,7 +443,8 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
         z.relnrm0 = 1.00001;
         z.done = maxit < 1;
         z.warm = warm != 0;
-        z.judge = judge;
+        z.reject_above = warm ? reject_above : 0.0;
+        z.check_pass = std::max(1, check_pass);
         SirtVec &v = vec[k];
         v.x = P[k].x;
         v.bdata = bdata.as<float>() + o_nnz[k];
@@ -620,10 +626,11 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
 }
 
 int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float lam, int maxit,
-             float tol, int warm, double judge, arec_hist *hist, int hist_cap, int *nhist, float *lam_out,
-             bool use_graph) {
+             float tol, int warm, double reject_above, int check_pass, arec_hist *hist, int hist_cap, int *nhist,
+             float *lam_out, bool use_graph) {
     std::vector<SirtProblem> P{SirtProblem{&g, b, x, lam}};
-    return sirt_batch_run(P, st, maxit, tol, warm, judge, hist, hist_cap, nhist, lam_out, use_graph);
+    return sirt_batch_run(P, st, maxit, tol, warm, reject_above, check_pass, hist, hist_cap, nhist, lam_out,
+                          use_graph);
 }
 
 int cgls_run(const Geometry &g, hipStream_t st, const float *b, const float *w, float *x, int warm,

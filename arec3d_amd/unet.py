@@ -11,7 +11,7 @@ unet.hip (no counterpart in the reference).
   and no early termination is row-separable).
 * :func:`warm_sirt` runs SIRT for ``k_in`` iterations, applies the network,
   keeps its output only if its residual is lower (SIRT remains the fallback
-  and judges the network), then continues SIRT to the tolerance.
+  and decides whether the network is used), then continues SIRT to the tolerance.
 """
 import ctypes as C
 import json
@@ -111,7 +111,10 @@ def phantom_params(seed, n, height):
 def make_pairs(ctx, n_samples, n=128, na=360, k_in=5, k_out=200, stack=16, seed0=40000, noise=0.01,
                log=print):
     """Training pairs (SIRT after k_in and after k_out iterations) as FP32
-    cubes [n][n][n] on the host."""
+    cubes [n][n][n] on the host.  noise: Gaussian sigma relative to the
+    sinogram's maximum, or a range (lo, hi) from which each stack draws its
+    sigma log-uniformly (seeded by seed0)."""
+    noise_rng = np.random.default_rng(seed0 + 7)
     ang = (np.arange(na) * np.pi / na).astype(np.float32)
     ny = n * stack
     g = Geometry(ctx, "sq", n, ny, ang)
@@ -134,7 +137,11 @@ def make_pairs(ctx, n_samples, n=128, na=360, k_in=5, k_out=200, stack=16, seed0
             tpl[:, k * n:(k + 1) * n] = tmp.download()
         xs.upload(tpl)
         g.project(xs, b)
-        P.add_noise(ctx, b, seed0 + s0, rel_sigma=noise)
+        if np.isscalar(noise):
+            sig = noise
+        else:
+            sig = float(np.exp(noise_rng.uniform(np.log(noise[0]), np.log(noise[1]))))
+        P.add_noise(ctx, b, seed0 + s0, rel_sigma=sig)
         g.sirt(b, x, lam, k_in, 0.0)
         cyl_to_cube(ctx, x, g.radius, ny, cube)
         c_in = cube.download().reshape(stack, n, n, n)
@@ -194,13 +201,27 @@ def load_weights(path=None):
 
 
 class WarmStart:
-    """The trained network and its buffers for one geometry's volume."""
+    """The trained network and its buffers for one geometry's volume.
 
-    def __init__(self, ctx, g, params=None, scale=None):
+    Scale rule.  k SIRT iterations resolve about k cycles per field of view
+    whatever the grid, so at width n the network's input is SIRT after
+    k_in = k_train * n / n_train iterations: that has the training input's
+    resolution in pixel units (5 iterations at 128 -> 20 at 512).
+
+    Check.  SIRT's own residual decides whether the network's output is
+    used: after check_pass SIRT passes from it, the printed relative
+    residual must be below that of the SIRT iterate it replaced.  Its
+    excess at the first pass is mostly low-frequency error, which SIRT
+    removes in a few passes, so the check waits for them."""
+
+    def __init__(self, ctx, g, params=None, scale=None, meta=None, check_pass=5):
         if params is None:
-            params, scale, _ = load_weights()
+            params, scale, meta = load_weights()
+        meta = meta or {"n": g.nx, "k_in": 5}
         n = g.nx
         self.ctx, self.g = ctx, g
+        self.k_in = max(1, int(round(meta["k_in"] * n / meta["n"])))
+        self.check_pass = check_pass
         self.cube_in = ctx.empty((g.ny, n, n))
         self.cube_out = ctx.empty((g.ny, n, n))
         self.x_keep = ctx.empty(g.vol_shape)
@@ -216,17 +237,18 @@ class WarmStart:
         self.net.close()
 
 
-def warm_sirt(ctx, g, b, x, lam, tol, ws, k_in=5, maxit=3000, timing=False):
-    """Warm-started SIRT: k_in SIRT iterations from zero, the network, then
-    SIRT from its output to relnrm <= tol.  SIRT judges the network: its
-    first pass computes the output's relative residual and rejects it if it
-    is not below the k_in run's last one; SIRT then continues from the k_in
-    iterate instead (the fallback).  Residuals use the fixed denominator
-    ||A^T b|| throughout.  Returns ((hist_in, hist_out), phase seconds,
-    'network' | 'sirt')."""
+def warm_sirt(ctx, g, b, x, lam, tol, ws, k_in=None, maxit=3000, timing=False):
+    """Warm-started SIRT: k_in SIRT iterations from zero (default: the
+    WarmStart's scale rule), the network, then SIRT from its output to
+    relnrm <= tol.  SIRT's residual decides on the network: if the residual
+    printed ws.check_pass passes after the network is not below the last
+    one of the k_in run, SIRT continues from the k_in iterate instead (the
+    fallback).  Residuals use the fixed denominator ||A^T b|| throughout.
+    Returns ((hist_in, hist_out), phase seconds, 'network' | 'sirt')."""
     from . import copy as dcopy
 
     t = {}
+    k_in = ws.k_in if k_in is None else k_in
     t0 = time.perf_counter()
     h1, _ = g.sirt(b, x, lam, k_in, 0.0)
     if timing:
@@ -239,9 +261,10 @@ def warm_sirt(ctx, g, b, x, lam, tol, ws, k_in=5, maxit=3000, timing=False):
         ctx.synchronize()
         t["net"] = time.perf_counter() - t0
         t0 = time.perf_counter()
-    h2, _ = g.sirt_warm(b, x, lam, maxit, tol, hist_cap=maxit + 64, reject_above=h1[-1][2])
+    h2, _ = g.sirt_warm(b, x, lam, maxit, tol, hist_cap=maxit + 64, reject_above=h1[-1][2],
+                        check_pass=ws.check_pass)
     chosen = "network"
-    if h2 and h2[0][1] == 3:
+    if h2 and h2[-1][1] == 3:
         chosen = "sirt"
         dcopy(ctx, x, ws.x_keep)
         h3, _ = g.sirt_warm(b, x, lam, maxit, tol, hist_cap=maxit + 64)

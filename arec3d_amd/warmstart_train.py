@@ -4,7 +4,8 @@
 
 Pairs: seeded ellipsoid phantoms at 128^3 (support radius 62.5, 40
 ellipsoids, semi-axes U[2, 30], density U[0.05, 1]), 360 angles on [0, pi),
-SQ projector, 1% Gaussian noise; input = SIRT after 5 iterations, target =
+SQ projector, Gaussian noise (--noise: a fixed level or a range drawn per
+stack of 16 phantoms); input = SIRT after 5 iterations, target =
 SIRT after 200 iterations (lam = 0.9 / sigma_max^2).  Training: Adam on the
 mean squared error, batch of one cube, cosine learning-rate decay, all on
 the GPU through the U-Net kernels.  A held-out set (other seeds) reports the
@@ -48,6 +49,9 @@ def main(argv=None):
     ap.add_argument("--k-in", type=int, default=5)
     ap.add_argument("--k-out", type=int, default=200)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--noise", type=float, nargs="+", default=[0.01],
+                    help="Gaussian noise sigma relative to the sinogram maximum, or a range lo hi "
+                         "drawn log-uniformly per stack of phantoms")
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)
 
@@ -55,12 +59,13 @@ def main(argv=None):
         print(*a, flush=True)
 
     ctx = Context(0)
+    noise = args.noise[0] if len(args.noise) == 1 else (args.noise[0], args.noise[1])
     t0 = time.time()
     xin, xout, lam = make_pairs(ctx, args.samples, n=args.n, na=args.angles, k_in=args.k_in,
-                                k_out=args.k_out, seed0=40000, log=log)
+                                k_out=args.k_out, seed0=40000, noise=noise, log=log)
     t_pairs = time.time() - t0
     hin, hout, _ = make_pairs(ctx, args.heldout, n=args.n, na=args.angles, k_in=args.k_in,
-                              k_out=args.k_out, seed0=48000, log=None)
+                              k_out=args.k_out, seed0=48000, noise=noise, log=None)
     log(f"pairs: {args.samples} in {t_pairs:.1f}s, lam {lam:.4e}")
     t0 = time.time()
     params, scale, losses = train(ctx, xin, xout, epochs=args.epochs, lr=args.lr, seed=args.seed, log=log)
@@ -70,7 +75,7 @@ def main(argv=None):
         f"({e_id / max(e_net, 1e-30):.2f}x lower)")
     meta = {
         "samples": args.samples, "epochs": args.epochs, "lr": args.lr, "n": args.n, "angles": args.angles,
-        "k_in": args.k_in, "k_out": args.k_out, "lam": lam, "projector": "sq", "noise": 0.01,
+        "k_in": args.k_in, "k_out": args.k_out, "lam": lam, "projector": "sq", "noise": args.noise,
         "pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1),
         "heldout_mse_identity": e_id, "heldout_mse_network": e_net,
     }

@@ -115,13 +115,15 @@ int arec_sirt(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x
  * reference, whose SIRT always starts from zero, arecsirt.cpp:53-56).  The
  * relative residual keeps the reference's fixed denominator ||A^T b||; the
  * first printed line is the residual of the given volume; a restart still
- * zeroes x.  reject_above > 0 makes SIRT judge the given volume: if its
- * relative residual is >= reject_above the call returns at once with one
- * history line of event 3 and x_dev unchanged.  Used for the learned warm
- * start. */
+ * zeroes x.  reject_above > 0 makes SIRT's residual decide on the given
+ * volume: if the relative residual printed at pass check_pass (1 = the
+ * given volume's own) is >= reject_above, or a restart would come before
+ * that pass, the solve ends with a last history line of event 3 (x_dev then
+ * holds the volume after check_pass - 1 updates; with check_pass = 1 it is
+ * unchanged).  Used for the learned warm start. */
 int arec_sirt_warm(arec_ctx *ctx, const arec_geom *geom, const float *b_dev, float *x_dev, float lam,
-                   int32_t maxit, float tol, double reject_above, arec_hist *hist, int32_t hist_cap,
-                   int32_t *nhist, float *lam_out);
+                   int32_t maxit, float tol, double reject_above, int32_t check_pass, arec_hist *hist,
+                   int32_t hist_cap, int32_t *nhist, float *lam_out);
 
 /* Batched SIRT over nprob independent problems (no counterpart in the
  * reference; each problem is one cyl_sirt* call, arecsirt.cpp:11-191), all

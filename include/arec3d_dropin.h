@@ -9,7 +9,8 @@
  * 64 bytes on LP64) because callers pass it by value.  MPI_Comm comes from
  * the caller's MPI (MPICH: an int), as in the reference's headers.  One GPU
  * per process: a communicator with more than one rank is rejected (status
- * -2) -- the per-rank Y-slab design is in DESIGN.md.
+ * -2); several ranks would each hold one Y-slab on their own GPU, which
+ * this library does not do.
  */
 #ifndef AREC3D_DROPIN_H
 #define AREC3D_DROPIN_H

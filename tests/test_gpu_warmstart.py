@@ -4,9 +4,9 @@ On held-out seeded phantoms at the training size (128^3 cylinder, 360
 angles, SQ, 1% noise; phantom seeds disjoint from the training seeds), SIRT
 from zero and the warm start (5 SIRT iterations, the network, SIRT from its
 output) both run to relnrm <= 1e-3 with the same fixed denominator
-||A^T b||.  The network must be accepted by SIRT's judge and must cut the
+||A^T b||.  SIRT's residual check must accept the network, which must cut the
 projector passes to the tolerance by at least 2x.  A start whose residual is
-not below the judge's bound is rejected with x untouched, and the solver
+not below the bound is rejected with x untouched, and the solver
 falls back to the SIRT iterate.
 """
 import numpy as np
@@ -53,9 +53,9 @@ def test_warm_start_beats_plain_sirt(ctx, setup, seed):
     assert warm_passes * 2 <= plain_passes
 
 
-def test_judge_rejects_a_bad_start(ctx, setup):
+def test_residual_check_rejects_a_bad_start(ctx, setup):
     A, P, U, g, lam, ws = setup
-    print("\n[case] warm start judge", flush=True)
+    print("\n[case] warm start residual check", flush=True)
     rng = np.random.default_rng(1)
     b = ctx.to_device(rng.random(g.proj_shape).astype(np.float32))
     bad = (10.0 * rng.standard_normal(g.vol_shape)).astype(np.float32)
@@ -63,3 +63,6 @@ def test_judge_rejects_a_bad_start(ctx, setup):
     h, _ = g.sirt_warm(b, x, lam, 50, 1e-3, reject_above=0.5)
     assert len(h) == 1 and h[0][1] == 3 and h[0][2] >= 0.5
     assert np.array_equal(x.download(), bad)
+    # checked after 5 passes: rejected at the latest there
+    h, _ = g.sirt_warm(b, x, lam, 50, 1e-3, reject_above=1e-9, check_pass=5)
+    assert 1 <= len(h) <= 5 and h[-1][1] == 3 and all(e[1] == 0 for e in h[:-1])

@@ -1,5 +1,5 @@
 """CPU: the FP64 convolution oracle of the U-Net agrees with scipy's
-correlate and with brute force (pins it before it judges the MFMA kernels)."""
+correlate and with brute force (pins it before it checks the MFMA kernels)."""
 import numpy as np
 from scipy.ndimage import correlate
 
