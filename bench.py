@@ -4,13 +4,16 @@ Headline workload (default): a seeded 512 x 512 x 512 ellipsoid phantom in
 the compact cylinder (support radius 250, 40 ellipsoids, semi-axes U[8,120],
 density U[0.05,1]), 720 parallel-beam angles on [0, pi), detector 512 x 512,
 SQ projector, sinogram = A x + Gaussian noise (sigma = 1% of max).  One step
-is one warm-started SIRT solve: 5 SIRT iterations from zero (the reference's
-Landweber, arecsirt.cpp:193-373), the trained U-Net (arec3d_amd/data), then
-SIRT from the network's output to a relative normal-equation residual
-||A^T(b - A x)|| / ||A^T b|| <= 1e-3 (fixed denominator), lam = 0.9 /
-sigma_max^2 from power iteration.  Plain SIRT from zero to the same
-tolerance is timed beside it (median of as many solves) and reported in
-config; --plain-only makes it the step.
+is one warm-started SIRT solve: SIRT from zero (the reference's Landweber,
+arecsirt.cpp:193-373) for 5 iterations per 128 pixels of width (20 at 512:
+the network's training input had 5 at 128), the trained U-Net
+(arec3d_amd/data, trained on 128^3/360 pairs), then SIRT from the network's
+output to a relative normal-equation residual ||A^T(b - A x)|| / ||A^T b||
+<= 1e-3 (fixed denominator), lam = 0.9 / sigma_max^2 from power iteration;
+SIRT's residual 5 passes after the network decides whether its output is
+kept (else SIRT continues from its own iterate).  Plain SIRT from zero to
+the same tolerance is timed beside it (median of as many solves) and
+reported in config; --plain-only makes it the step.
 
 Prints one JSON line (rank 0) with the roofline of the dominant kernel and a
 CPU baseline (the oracle, test infrastructure, timed on a bounded sub-slab).
@@ -28,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time to 1e-3 rel. residual, 512^3/720 angles: warm-start vs plain SIRT"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_FLOPS = 157.3e12  # FP32 vector
 
 
 def log(*a):
@@ -49,44 +53,65 @@ def power_iteration(A, ctx, g, iters=20, seed=0):
     return lam
 
 
-def cpu_baseline(nx, na, rows, iters_gpu, ny_full):
-    """Oracle SQ forward + back on `rows` detector rows of the same geometry
-    (rows are independent and cost the same), scaled to the full height."""
+def cpu_baseline(nx, na, iters_gpu, ny_full, rows_1t=2, rows_per_thread=1, threads=None):
+    """The FP32 oracle (the reference's loop order: angle -> column -> row)
+    timed on a sub-slab of rows of the same geometry -- rows are independent
+    and cost the same, so the time scales exactly with the row count.  One
+    thread on rows_1t rows (the reference's single-rank path), then T threads
+    on rows_per_thread rows each (the reference's MPI Y-slab split, as
+    threads: ctypes releases the GIL).  Reported at T threads, scaled to the
+    full height and to the GPU's plain-SIRT iteration count."""
+    import concurrent.futures as cf
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
+    T = threads or max(1, min(16, os.cpu_count() or 1))
     ang = (np.arange(na) * np.pi / na).astype(np.float32)
     nr = len(O.cyl_geometry(0.5 * (nx - 1)))
-    x = np.random.default_rng(1).random((nr, rows)).astype(np.float32)
+    rng = np.random.default_rng(1)
+
+    def one(rows):
+        x = rng.random((nr, rows)).astype(np.float32)
+        p = O.project("sq", x, ang, nx, rows)
+        O.backproject("sq", p, ang, nx, rows)
+
     t0 = time.perf_counter()
-    p = O.project("sq", x, ang, nx, rows)
-    O.backproject("sq", p, ang, nx, rows)
-    dt = time.perf_counter() - t0
-    per_iter_full = dt * ny_full / rows
-    vu = 2 * nr * rows * na
+    one(rows_1t)
+    dt1 = time.perf_counter() - t0
+    iter_1t = dt1 * ny_full / rows_1t
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=T) as ex:
+        list(ex.map(one, [rows_per_thread] * T))
+    dtT = time.perf_counter() - t0
+    iter_T = dtT * ny_full / (rows_per_thread * T)
+    vu_row = 2 * nr * na  # forward + back, one row
     return {
-        "value": per_iter_full * iters_gpu,
+        "value": iter_T * iters_gpu,
         "unit": "s",
-        "cores": 1,
+        "cores": T,
         "kind": "port",
-        "sample": f"oracle SQ forward+back (1 SIRT iteration) on {rows} of {ny_full} rows at "
-                  f"{nx} wide, {na} angles: {dt:.2f} s = {vu / dt / 1e6:.1f} MVU/s; "
-                  f"scaled x{ny_full // rows} rows x {iters_gpu} iterations (the GPU's count)",
-        "vu_per_s": vu / dt,
+        "sample": f"oracle SQ forward+back (one SIRT iteration) at {nx} wide, {na} angles: 1 thread on {rows_1t} "
+                  f"rows {dt1:.2f} s ({vu_row * rows_1t / dt1 / 1e6:.1f} MVU/s, {iter_1t * iters_gpu:.0f} s scaled); "
+                  f"{T} threads on {rows_per_thread * T} rows {dtT:.2f} s ({vu_row * rows_per_thread * T / dtT / 1e6:.1f} "
+                  f"MVU/s); scaled to {ny_full} rows x {iters_gpu} iterations (plain SIRT's count on the GPU)",
+        "seconds_per_iteration_1_thread": iter_1t,
+        "seconds_per_iteration": iter_T,
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=512, help="detector width = volume width")
     ap.add_argument("--rows", type=int, default=512)
     ap.add_argument("--angles", type=int, default=720)
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--maxit", type=int, default=3000)
-    ap.add_argument("--cpu-rows", type=int, default=2)
+    ap.add_argument("--plan", type=int, nargs=2, metavar=("ROWS_PER_LANE", "MIN_GROUPS"),
+                    help="forward projector plan (default: autotuned)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=None, help="plain SIRT solves (default: --steps)")
     ap.add_argument("--plain-only", action="store_true", help="time plain SIRT only (no network)")
@@ -109,7 +134,10 @@ def main():
     t0 = time.perf_counter()
     g = A.Geometry(ctx, "sq", nx, ny, ang)
     t_geom = time.perf_counter() - t0
-    if args.no_autotune:
+    if args.plan:
+        g.plan(*args.plan)
+        plan = g.get_plan()[:2]
+    elif args.no_autotune:
         plan = g.get_plan()[:2]
     else:
         from arec3d_amd.autotune import autotune
@@ -146,6 +174,7 @@ def main():
         from arec3d_amd import unet as U
 
         ws = U.WarmStart(ctx, g)
+        ws_k_in = ws.k_in
         for _ in range(args.warmup):
             U.warm_sirt(ctx, g, b, x, lam, args.tol, ws, maxit=3)
         ctx.synchronize()
@@ -194,7 +223,7 @@ def main():
         workload = f"plain SIRT (SQ) to relnrm {args.tol:g}"
     else:
         value, n_steps = float(np.median(w_times)), len(w_times)
-        workload = f"warm-started SIRT (SQ; 5 SIRT iterations, U-Net, SIRT) to relnrm {args.tol:g}"
+        workload = f"warm-started SIRT (SQ; SIRT, U-Net, SIRT) to relnrm {args.tol:g}"
     cfg = {
         "workload": f"{workload}, {nx}x{nx}x{ny} volume, {na} angles, detector {nx}x{ny}",
         "plain_sirt_s": round(t_plain, 4),
@@ -207,6 +236,8 @@ def main():
         "forward_ms": round(ms_f, 3),
         "backward_ms": round(ms_b, 3),
         "voxel_updates_per_s": round(2 * vu / ((ms_f + ms_b) * 1e-3), 1),
+        "valu_frac_forward": round(6.0 * vu / (ms_f * 1e-3) / VALU_PEAK_FLOPS, 4),
+        "valu_frac_backward": round(6.0 * vu / (ms_b * 1e-3) / VALU_PEAK_FLOPS, 4),
     }
     if not args.plain_only:
         cfg.update({
@@ -216,6 +247,7 @@ def main():
             "warm_start_used": w_chosen,
             "warm_phase_s": {k: round(v, 4) for k, v in w_phase.items()},
             "network": "3-level 3-D U-Net 16/32/64, BF16 MFMA, trained on 2000 SIRT-5 -> SIRT-200 pairs at 128^3/360",
+            "sirt_iterations_before_network": ws_k_in,
         })
     out = {
         "metric": METRIC,
@@ -242,13 +274,16 @@ def main():
             "model": "algorithmic bytes per launch 4*na*(nnz + nx*ny) (per-view streaming)",
         },
     }
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of the
+    # same kernel, workload and forward plan), when such a summary exists
     pmc = os.path.join(ROOT, "profiles", f"pmc_sq_{nx}x{nx}x{ny}_{na}.json")
     if os.path.exists(pmc):
-        t = json.load(open(pmc)).get(dom)
-        if t:
+        summary = json.load(open(pmc))
+        t = summary.get(dom)
+        if t and list(summary.get("plan", [])) == list(plan):
             out["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(nx, na, args.cpu_rows, n_it, ny)
+        out["cpu_baseline"] = cpu_baseline(nx, na, n_it, ny)
     print(json.dumps(out), flush=True)
 
 
