@@ -187,32 +187,31 @@ __global__ void real_part_kernel(const float2 *__restrict__ c, float *__restrict
         out[i] = c[i].x / div;
 }
 
-// quad2dfit (align2dstack.cpp:1125-1166), FP64 on FP32 samples.
-__device__ void quad2dfit_dev(const float *Zp, float *xsh, float *ysh) {
-#define Z(i, j) ((double)Zp[((j)-1) * 3 + (i)-1])
-    double C2 = (8. * Z(1, 1) - 8. * Z(1, 2) + 5. * Z(2, 1) - 8. * Z(2, 2) + 3. * Z(2, 3) +
-                 2. * Z(3, 1) - 8. * Z(3, 2) + 6. * Z(3, 3)) /
-                (-6.);
-    double C3 = (Z(1, 1) - 2. * Z(1, 2) + Z(1, 3) + Z(2, 1) - 2. * Z(2, 2) + Z(2, 3) + Z(3, 1) -
-                 2. * Z(3, 2) + Z(3, 3)) /
-                6.0;
-    double C4 = (8. * Z(1, 1) + 5. * Z(1, 2) + 2. * Z(1, 3) - 8. * Z(2, 1) - 8. * Z(2, 2) -
-                 8. * Z(2, 3) + 3. * Z(3, 2) + 6. * Z(3, 3)) /
-                (-6.0);
-    double C5 = (Z(1, 1) - Z(1, 3) - Z(3, 1) + Z(3, 3)) / 4.0;
-    double C6 = (Z(1, 1) + Z(1, 2) + Z(1, 3) - 2. * Z(2, 1) - 2. * Z(2, 2) - 2. * Z(2, 3) +
-                 Z(3, 1) + Z(3, 2) + Z(3, 3)) /
-                6.0;
-#undef Z
-    float ys = 0.0f, xs = 0.0f;
-    double denom = 4. * C3 * C6 - C5 * C5;
-    if (denom != 0.0) {
-        ys = (float)((C4 * C5 - 2. * C2 * C6) / denom - 2.0);
-        xs = (float)((C2 * C5 - 2. * C4 * C3) / denom - 2.0);
-        if (ys < -1.0f) ys = -1.0f;
-        if (ys > 1.0f) ys = 1.0f;
-        if (xs < -1.0f) xs = -1.0f;
-        if (xs > 1.0f) xs = 1.0f;
+// Sub-pixel peak of a 3x3 neighbourhood (quad2dfit, align2dstack.cpp:1125-
+// 1166), FP64 on FP32 samples z[k], k = 3*row + column.  Least-squares fit
+// of q(u,v) = a + b u + c v + d u^2 + e u v + f v^2 on u, v in {-1, 0, 1}
+// (u along x): the grid's moments give b = Su/6, c = Sv/6, e = Suv/4,
+// d + f = (Suu + Svv)/2 - 2 S0/3 and d - f = (Suu - Svv)/2; the stationary
+// point solves [2d e; e 2f] (u, v) = -(b, c) and is clamped to [-1, 1].
+__device__ void quad2dfit_dev(const float *z, float *xsh, float *ysh) {
+    double s0 = 0, su = 0, sv = 0, suu = 0, svv = 0, suv = 0;
+    for (int k = 0; k < 9; k++) {
+        double u = (double)(k % 3 - 1), v = (double)(k / 3 - 1), w = (double)z[k];
+        s0 += w;
+        su += u * w;
+        sv += v * w;
+        suu += u * u * w;
+        svv += v * v * w;
+        suv += u * v * w;
+    }
+    double b = su / 6.0, c = sv / 6.0, e = suv / 4.0;
+    double dpf = 0.5 * (suu + svv) - 2.0 * s0 / 3.0, dmf = 0.5 * (suu - svv);
+    double d = 0.5 * (dpf + dmf), f = 0.5 * (dpf - dmf);
+    double det = 4.0 * d * f - e * e;
+    float xs = 0.0f, ys = 0.0f;
+    if (det != 0.0) {
+        xs = (float)fmin(1.0, fmax(-1.0, (e * c - 2.0 * f * b) / det));
+        ys = (float)fmin(1.0, fmax(-1.0, (e * b - 2.0 * d * c) / det));
     }
     *xsh = xs;
     *ysh = ys;

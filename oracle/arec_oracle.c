@@ -332,79 +332,73 @@ void orc_ccorr2d(int nx, int ny, const float *x, const float *y, double *c) {
     free(yi);
 }
 
-/* quad2dfit, align2dstack.cpp:1125-1166: least-squares paraboloid through a
- * 3x3 neighbourhood Z(i,j) = Z[(j-1)*3 + i-1], offsets clamped to [-1, 1]. */
-void orc_quad2dfit(const float *Zp, float *xsh, float *ysh, float *peakv) {
-#define Z(i, j) ((double)Zp[((j)-1) * 3 + (i)-1])
-    double C1 = (26.0 * Z(1, 1) - Z(1, 2) + 2.0 * Z(1, 3) - Z(2, 1) - 19.0 * Z(2, 2) -
-                 7.0 * Z(2, 3) + 2.0 * Z(3, 1) - 7.0 * Z(3, 2) + 14.0 * Z(3, 3)) /
-                9.0;
-    double C2 = (8. * Z(1, 1) - 8. * Z(1, 2) + 5. * Z(2, 1) - 8. * Z(2, 2) + 3. * Z(2, 3) +
-                 2. * Z(3, 1) - 8. * Z(3, 2) + 6. * Z(3, 3)) /
-                (-6.);
-    double C3 = (Z(1, 1) - 2. * Z(1, 2) + Z(1, 3) + Z(2, 1) - 2. * Z(2, 2) + Z(2, 3) + Z(3, 1) -
-                 2. * Z(3, 2) + Z(3, 3)) /
-                6.0;
-    double C4 = (8. * Z(1, 1) + 5. * Z(1, 2) + 2. * Z(1, 3) - 8. * Z(2, 1) - 8. * Z(2, 2) -
-                 8. * Z(2, 3) + 3. * Z(3, 2) + 6. * Z(3, 3)) /
-                (-6.0);
-    double C5 = (Z(1, 1) - Z(1, 3) - Z(3, 1) + Z(3, 3)) / 4.0;
-    double C6 = (Z(1, 1) + Z(1, 2) + Z(1, 3) - 2. * Z(2, 1) - 2. * Z(2, 2) - 2. * Z(2, 3) +
-                 Z(3, 1) + Z(3, 2) + Z(3, 3)) /
-                6.0;
-#undef Z
-    *ysh = 0.0f;
+/* quad2dfit, align2dstack.cpp:1125-1166: the sub-pixel peak of a 3x3
+ * neighbourhood z[k], k = 3*row + column (row along y, column along x).
+ * Least-squares fit of q(u,v) = a + b u + c v + d u^2 + e u v + f v^2 on the
+ * grid u, v in {-1, 0, 1} (u along x).  With the moments S0 = sum z,
+ * Su = sum u z, Sv, Suu, Svv, Suv over the grid, the normal equations give
+ * b = Su/6, c = Sv/6, e = Suv/4, d + f = (Suu + Svv)/2 - 2 S0/3,
+ * d - f = (Suu - Svv)/2 and a = (S0 - 6 (d + f))/9.  The stationary point
+ * solves [2d e; e 2f] (u, v) = -(b, c); peakv = q there, then the offsets
+ * are clamped to [-1, 1].  A singular system leaves (0, 0) and peakv
+ * unchanged, as the reference does. */
+void orc_quad2dfit(const float *z, float *xsh, float *ysh, float *peakv) {
+    double s0 = 0, su = 0, sv = 0, suu = 0, svv = 0, suv = 0;
+    for (int k = 0; k < 9; k++) {
+        double u = (double)(k % 3 - 1), v = (double)(k / 3 - 1), w = (double)z[k];
+        s0 += w;
+        su += u * w;
+        sv += v * w;
+        suu += u * u * w;
+        svv += v * v * w;
+        suv += u * v * w;
+    }
+    double b = su / 6.0, c = sv / 6.0, e = suv / 4.0;
+    double dpf = 0.5 * (suu + svv) - 2.0 * s0 / 3.0, dmf = 0.5 * (suu - svv);
+    double d = 0.5 * (dpf + dmf), f = 0.5 * (dpf - dmf), a = (s0 - 6.0 * dpf) / 9.0;
+    double det = 4.0 * d * f - e * e;
     *xsh = 0.0f;
-    double denom = 4. * C3 * C6 - C5 * C5;
-    if (denom != 0.0) {
-        *ysh = (float)((C4 * C5 - 2. * C2 * C6) / denom - 2.0);
-        *xsh = (float)((C2 * C5 - 2. * C4 * C3) / denom - 2.0);
-        float pv = (float)(4. * C1 * C3 * C6 - C1 * C5 * C5 - C2 * C2 * C6 + C2 * C4 * C5 -
-                           C4 * C4 * C3);
-        *peakv = (float)(pv / denom);
-        if (*ysh < -1.0) *ysh = -1.0f;
-        if (*ysh > 1.0) *ysh = 1.0f;
-        if (*xsh < -1.0) *xsh = -1.0f;
-        if (*xsh > 1.0) *xsh = 1.0f;
+    *ysh = 0.0f;
+    if (det != 0.0) {
+        double u = (e * c - 2.0 * f * b) / det, v = (e * b - 2.0 * d * c) / det;
+        *peakv = (float)(a + b * u + c * v + d * u * u + e * u * v + f * v * v);
+        *xsh = (float)fmin(1.0, fmax(-1.0, u));
+        *ysh = (float)fmin(1.0, fmax(-1.0, v));
     }
 }
 
-/* speak, align2dstack.cpp:1059-1105: x-major scan (i outer, j inner) for
- * |v| > pmax with pmax holding the signed value, circular 3x3 gather,
- * quad2dfit, then wrap to (-n/2, n/2].  With no element above 0 the
- * reference reads uninitialised indices; the peak is taken at (1,1) here. */
+/* speak, align2dstack.cpp:1059-1105.  The scan runs x outer, y inner; an
+ * element wins when its magnitude exceeds the running best, and the running
+ * best keeps the winner's signed value (after a negative winner any element
+ * whose magnitude exceeds that negative number wins next: the reference's
+ * quirk).  The 3x3 neighbourhood wraps circularly, the fitted offset is added
+ * to the 0-based peak and the result is wrapped to (-n/2, n/2] with integer
+ * n/2.  With no element above 0 the reference reads uninitialised indices;
+ * the peak is taken at (0, 0) here. */
 void orc_speak(int nx, int ny, const float *data, float *xt, float *yt) {
-#define D(i, j) data[((j)-1) * nx + (i)-1]
-    int ix = 1, iy = 1;
-    float pmax = 0.0f;
-    for (int i = 1; i <= nx; i++)
-        for (int j = 1; j <= ny; j++)
-            if (fabsf(D(i, j)) > pmax) {
-                ix = i;
-                iy = j;
-                pmax = D(i, j);
+    int px = 0, py = 0;
+    float best = 0.0f;
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++) {
+            float v = data[(size_t)y * nx + x];
+            if (fabsf(v) > best) {
+                best = v;
+                px = x;
+                py = y;
             }
-    float rsq[9];
-    for (int j = -1; j <= 1; j++) {
-        int jcol = iy + j;
-        if (jcol < 1) jcol += ny;
-        if (jcol > ny) jcol -= ny;
-        for (int i = -1; i <= 1; i++) {
-            int irow = ix + i;
-            if (irow < 1) irow += nx;
-            if (irow > nx) irow -= nx;
-            rsq[(j + 1) * 3 + i + 1] = D(irow, jcol);
         }
+    float nb[9];
+    for (int k = 0; k < 9; k++) {
+        int x = (px + k % 3 - 1 + nx) % nx, y = (py + k / 3 - 1 + ny) % ny;
+        nb[k] = data[(size_t)y * nx + x];
     }
-#undef D
-    float xsh, ysh, peakv;
-    orc_quad2dfit(rsq, &xsh, &ysh, &peakv);
-    ix--;
-    iy--;
-    *xt = xsh + (float)ix;
-    *yt = ysh + (float)iy;
-    if (*xt > nx / 2) *xt = *xt - (float)nx;
-    if (*yt > ny / 2) *yt = *yt - (float)ny;
+    float fx, fy, pv;
+    orc_quad2dfit(nb, &fx, &fy, &pv);
+    float sxv = fx + (float)px, syv = fy + (float)py;
+    if (sxv > nx / 2) sxv -= (float)nx;
+    if (syv > ny / 2) syv -= (float)ny;
+    *xt = sxv;
+    *yt = syv;
 }
 
 /* fshift2d, align2dstack.cpp:566-630: complex DFT, phase ramp with the FP32

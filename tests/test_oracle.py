@@ -271,6 +271,24 @@ def test_quad2dfit_paraboloid(x0, y0):
     assert ys == pytest.approx(y0, abs=1e-5)
 
 
+def test_quad2dfit_is_the_least_squares_paraboloid():
+    # against numpy's least squares on the 6-term quadratic, random samples
+    rng = np.random.default_rng(11)
+    U = np.array([k % 3 - 1 for k in range(9)], float)
+    V = np.array([k // 3 - 1 for k in range(9)], float)
+    M = np.stack([np.ones(9), U, V, U * U, U * V, V * V], 1)
+    for _ in range(200):
+        z = rng.standard_normal(9).astype(np.float32)
+        a, b, c, d, e, f = np.linalg.lstsq(M, z.astype(np.float64), rcond=None)[0]
+        det = 4 * d * f - e * e
+        u, v = (e * c - 2 * f * b) / det, (e * b - 2 * d * c) / det
+        xs, ys, pv = O.quad2dfit(z)
+        assert xs == pytest.approx(np.clip(u, -1, 1), abs=1e-5)
+        assert ys == pytest.approx(np.clip(v, -1, 1), abs=1e-5)
+        q = a + b * u + c * v + d * u * u + e * u * v + f * v * v
+        assert pv == pytest.approx(q, rel=1e-5, abs=1e-5)
+
+
 def test_speak_integer_and_subpixel():
     ny, nx = 16, 32
     yy, xx = np.mgrid[0:ny, 0:nx]
