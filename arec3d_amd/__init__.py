@@ -67,6 +67,7 @@ def _load():
         "arec_dot": ([vp, vp, vp, i64, P(dbl)], C.c_int),
         "arec_max": ([vp, vp, i64, P(flt)], C.c_int),
         "arec_copy": ([vp, vp, vp, i64], C.c_int),
+        "arec_scale": ([vp, vp, i64, flt], C.c_int),
         "arec_time_operator": ([vp, vp, C.c_int, vp, vp, i32, P(flt)], C.c_int),
         "arec_phantom": ([vp, dbl, i64, dbl, vp, i32, vp], C.c_int),
         "arec_add_noise": ([vp, vp, i64, C.c_uint64, i32, dbl, dbl, dbl], C.c_int),
@@ -80,6 +81,7 @@ def _load():
         "arec_sirt_batch": ([vp, i32, P(vp), P(vp), P(vp), P(flt), i32, flt, P(Hist), i32, P(i32), P(flt)],
                             C.c_int),
         "arec_geom_plan": ([vp, vp, i32, i32], C.c_int),
+        "arec_match": ([vp, vp, vp, vp, flt, i32, i32, flt, i32, i32, vp, vp, P(Hist)], C.c_int),
         "arec_geom_get_plan": ([vp, P(i32), P(i32), P(i32)], C.c_int),
         "arec_unet_create": ([vp, P(vp), i32, i32, i32, i32, C.c_uint64], C.c_int),
         "arec_unet_destroy": ([vp], C.c_int),
@@ -118,7 +120,7 @@ EXPORTED = [
     "arec_unet_destroy", "arec_unet_nparams", "arec_unet_get_params", "arec_unet_set_params",
     "arec_unet_set_scale", "arec_unet_forward", "arec_unet_train_step", "arec_unet_gradient",
     "arec_conv3_check", "arec_conv3_wgrad_check", "arec_cyl_to_cube", "arec_cube_to_cyl", "arec_h_project", "arec_h_backproject",
-    "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan",
+    "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan", "arec_match", "arec_scale",
 ]
 
 
@@ -347,6 +349,17 @@ def sirt_batch(ctx, geoms, bs, xs, lams, maxit, tol, hist_cap=None):
     return out
 
 
+def match(ctx, g, views, x, lam, outer=5, maxit=20, tol=0.0, cc_passes=3, method="sirt"):
+    """Projection matching (arec_match): returns (sx, sy, last history line
+    of each outer loop's reconstruction)."""
+    sx = np.zeros(g.na, np.float32)
+    sy = np.zeros(g.na, np.float32)
+    H = (Hist * max(1, outer))()
+    m = {"sirt": 1, "cgls": 2, "sirt_warm": 3}[method]
+    _check(lib.arec_match(ctx.h, g.h, views.ptr, x.ptr, lam, outer, maxit, tol, cc_passes, m, _ptr(sx), _ptr(sy), H))
+    return sx, sy, _hist(H, outer)
+
+
 def align_ccorr(ctx, x, y, c):
     """Correlation maps of view stacks x, y ([nv, ny, nx] DeviceArrays) into c."""
     nv, ny, nx = x.shape
@@ -398,6 +411,11 @@ def arecShiftImages(images, sx, sy):
 
 def copy(ctx, dst, src):
     _check(lib.arec_copy(ctx.h, dst.ptr, src.ptr, src.size))
+
+
+def scale(ctx, x, a):
+    """x *= a on the device."""
+    _check(lib.arec_scale(ctx.h, x.ptr, x.size, float(a)))
 
 
 def reduce_max(ctx, a):

@@ -164,6 +164,24 @@ int sirt_run(const Geometry &g, hipStream_t st, const float *b, float *x, float 
              int warm, double reject_above, int check_pass, arec_hist *hist, int hist_cap, int *nhist,
              float *lam_out, bool use_graph);
 
+int cgls_run(const Geometry &g, hipStream_t st, const float *b, const float *w, float *x, int warm,
+             int maxit, float tol, arec_hist *hist, int hist_cap, int *nhist, bool use_graph);
+int init_weights(const float *b, float *w, int64_t nx, int64_t ny, int64_t nz, hipStream_t st);
+int soft_noneg(float *x, int64_t n, hipStream_t st);
+int dot(const float *a, const float *b, int64_t n, double *out, hipStream_t st);
+int reduce_max(const float *x, int64_t n, float *out, hipStream_t st);
+int copy(float *dst, const float *src, int64_t n, hipStream_t st);
+int scale(float *x, float a, int64_t n, hipStream_t st);
+// hipGraph replay of solver passes unless AREC_NO_GRAPH is set (capi.cpp)
+bool use_graphs();
+
+// Alignment (align.hip): per-view cross-correlation maps and/or sub-pixel
+// peaks; Fourier shift of every view by sgn*(sx, sy).
+int cc_views(const float *proj, const float *meas, int nx, int ny, int nv, float *cmap_out, float *sx,
+             float *sy, hipStream_t st);
+int shift_views(float *data, int nx, int ny, int nv, const float *sx, const float *sy, float sgn,
+                hipStream_t st);
+
 // Fill with a constant by a kernel (graph-capture safe); skip as above.
 int fill(float *x, float v, int64_t n, hipStream_t st, const int *skip);
 

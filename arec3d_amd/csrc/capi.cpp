@@ -19,16 +19,7 @@ int hip_fail(hipError_t e, const char *what) {
     return AREC_EHIP;
 }
 
-// solver.hip
-int cgls_run(const Geometry &g, hipStream_t st, const float *b, const float *w, float *x, int warm,
-             int maxit, float tol, arec_hist *hist, int hist_cap, int *nhist, bool use_graph);
-int init_weights(const float *b, float *w, int64_t nx, int64_t ny, int64_t nz, hipStream_t st);
-int soft_noneg(float *x, int64_t n, hipStream_t st);
-int dot(const float *a, const float *b, int64_t n, double *out, hipStream_t st);
-int reduce_max(const float *x, int64_t n, float *out, hipStream_t st);
-int copy(float *dst, const float *src, int64_t n, hipStream_t st);
-
-static bool use_graphs() {
+bool use_graphs() {
     const char *e = getenv("AREC_NO_GRAPH");
     return !(e && *e && e[0] != '0');
 }
@@ -274,6 +265,11 @@ int arec_time_operator(arec_ctx *ctx, const arec_geom *geom, int which, const fl
 int arec_copy(arec_ctx *ctx, float *dst_dev, const float *src_dev, int64_t n) {
     CHECK_ARG(ctx && dst_dev && src_dev && n >= 0, "bad argument");
     return copy(dst_dev, src_dev, n, ctx->stream);
+}
+
+int arec_scale(arec_ctx *ctx, float *x_dev, int64_t n, float a) {
+    CHECK_ARG(ctx && x_dev && n >= 0, "bad argument");
+    return scale(x_dev, a, n, ctx->stream);
 }
 
 int arec_max(arec_ctx *ctx, const float *x_dev, int64_t n, float *out) {

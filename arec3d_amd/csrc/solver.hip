@@ -782,6 +782,15 @@ __global__ void soft_noneg_kernel(float *__restrict__ x, int64_t n) {
     }
 }
 
+__global__ void scale_kernel(float *__restrict__ x, float a, int64_t n) { GRID_STRIDE(i, n) x[i] = a * x[i]; }
+
+int scale(float *x, float a, int64_t n, hipStream_t st) {
+    if (n <= 0) return 0;
+    scale_kernel<<<vec_grid(n), kVecThreads, 0, st>>>(x, a, n);
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
 int soft_noneg(float *x, int64_t n, hipStream_t st) {
     soft_noneg_kernel<<<vec_grid(n), kVecThreads, 0, st>>>(x, n);
     AREC_HIP(hipGetLastError());

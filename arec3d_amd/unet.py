@@ -20,7 +20,7 @@ import time
 
 import numpy as np
 
-from . import (ArecError, Context, DeviceArray, Geometry, Hist, _check, _hist, _ptr, dot, lib,
+from . import (ArecError, Context, DeviceArray, Geometry, Hist, _check, _hist, _ptr, dot, lib, scale,
                reduce_max)
 from . import phantom as P
 
@@ -86,13 +86,15 @@ def cube_to_cyl(ctx, cube, radius, ny, x):
 
 def sigma_max2(ctx, g, iters=20, seed=0):
     """Largest eigenvalue of A^T A by power iteration (FP64 dots)."""
-    v = ctx.to_device(np.random.default_rng(seed).standard_normal(g.vol_shape).astype(np.float32))
+    # random start on the device (SplitMix64 Gaussian), no host-sized array
+    v = ctx.empty(g.vol_shape).zero()
+    P.add_noise(ctx, v, 777 + seed, sigma=1.0)
     w = ctx.empty(g.proj_shape)
     v2 = ctx.empty(g.vol_shape)
     lam = 0.0
     for _ in range(iters):
         n = np.sqrt(dot(ctx, v, v))
-        v.upload(v.download() / n)
+        scale(ctx, v, 1.0 / n)
         g.project(v, w)
         g.backproject(w, v2)
         lam = dot(ctx, v, v2)

@@ -166,6 +166,8 @@ int arec_time_operator(arec_ctx *ctx, const arec_geom *geom, int which, const fl
 /* dst = src, n floats, by a streaming copy kernel (also the reference
  * kernel for calibrating HBM byte counters). */
 int arec_copy(arec_ctx *ctx, float *dst_dev, const float *src_dev, int64_t n);
+/* x *= a (FP32) */
+int arec_scale(arec_ctx *ctx, float *x_dev, int64_t n, float a);
 
 /* Largest value of x (reduction), e.g. to scale noise to 1% of max(b). */
 int arec_max(arec_ctx *ctx, const float *x_dev, int64_t n, float *out);
@@ -188,6 +190,22 @@ int arec_align_peak(arec_ctx *ctx, const float *c_dev, int32_t nx, int32_t ny, i
  * sign = -1. */
 int arec_align_shift(arec_ctx *ctx, float *data_dev, int32_t nx, int32_t ny, int32_t nviews,
                      const float *sx_dev, const float *sy_dev, float sign);
+
+/* Projection matching: the reference driver's alignment loop (AREC3D.cpp:
+ * 140-165 initial reconstruction, :185-325 outer loops, translational
+ * alignment only) with every buffer on the device.  views_dev [na][ny][nx]
+ * are the measured views (never modified); x_dev receives the volume.  Per
+ * outer loop: reproject, cc_passes rounds of arecCCImages + arecShiftImages
+ * on a copy of the views, shift of the fresh views by the loop's summed
+ * shifts, reconstruction (method 1: SIRT from zero, lam/maxit/tol as
+ * cyl_sirt*; 2: CGLS continued from x, as cyl_cgls*; 3: SIRT continued from
+ * x, no counterpart in the reference), soft_noneg.  sx, sy
+ * (host, na floats each, may be NULL) receive the last loop's summed
+ * shifts; loop_hist (host, outer lines, may be NULL) each reconstruction's
+ * last history line. */
+int arec_match(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *x_dev, float lam,
+               int32_t outer, int32_t maxit, float tol, int32_t cc_passes, int32_t method, float *sx,
+               float *sy, arec_hist *loop_hist);
 
 /* ---- synthetic inputs (no counterpart in the reference) --------------- */
 /* Sum of ellipsoids at the voxel centres of the compact cylinder of the

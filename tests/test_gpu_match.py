@@ -1,0 +1,104 @@
+"""Projection matching on the GPU (arec_match, the reference driver's
+alignment loop AREC3D.cpp:140-325 with translational alignment).
+
+* Parity: the loop restated with the oracle's functions (SIRT, reprojection,
+  arecCCImages, arecShiftImages, soft_noneg, same order) gives the same
+  per-view shifts to 2e-3 px and the same volume to 1e-3 relative (the GPU
+  correlates with FP32 FFTs, the oracle with FP64 DFTs; SIRT differs by FP32
+  summation order).
+* Recovery: views of a seeded ellipsoid phantom shifted by seeded sub-pixel
+  offsets in [-3, 3] px plus 1% noise; after 5 outer loops of 20 SIRT
+  iterations every view's recovered shift is within 0.1 px of the truth,
+  after removing what projection matching cannot observe: a common y shift
+  (the volume moves along the axis) and an x pattern a cos(t) + b sin(t)
+  (the volume moves in the plane).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_match(views, ang, nx, ny, lam, outer, maxit, cc_passes):
+    _, x, _, _ = O.sirt("sq", views, ang, lam, maxit, 0.0)
+    for _ in range(outer):
+        prj = O.project("sq", x, ang, nx, ny)
+        aligned = views.copy()
+        sx = np.zeros(len(ang), np.float32)
+        sy = np.zeros(len(ang), np.float32)
+        for _ in range(cc_passes):
+            tx, ty = O.cc_images(prj, aligned)
+            aligned = O.shift_images(aligned, tx, ty)
+            sx += tx
+            sy += ty
+        aligned = O.shift_images(views, sx, sy)
+        _, x, _, _ = O.sirt("sq", aligned, ang, lam, maxit, 0.0)
+        x = O.soft_noneg(x)
+    return sx, sy, x
+
+
+def shifted_views(ctx, g, nx, ny, na, seed, amp, ell=12, noise=0.01):
+    import arec3d_amd as A
+    from arec3d_amd import phantom as P
+
+    sup = 0.45 * nx
+    prm = P.ellipsoid_params(seed, ell, sup, ny, axes=(0.06 * nx, 0.3 * nx), dens=(0.1, 1.0))
+    xt = ctx.empty(g.vol_shape)
+    P.phantom_device(ctx, xt, g.radius, ny, sup, prm)
+    b = ctx.empty(g.proj_shape)
+    g.project(xt, b)
+    rng = np.random.default_rng(seed)
+    dx = rng.uniform(-amp, amp, na).astype(np.float32)
+    dy = rng.uniform(-amp, amp, na).astype(np.float32)
+    A.align_shift(ctx, b, ctx.to_device(dx), ctx.to_device(dy), sign=1.0)
+    if noise:
+        P.add_noise(ctx, b, seed + 1, rel_sigma=noise)
+    return b, dx, dy
+
+
+def test_match_vs_oracle(ctx):
+    import arec3d_amd as A
+
+    print("\n[case] match vs oracle", flush=True)
+    nx, ny, na = 33, 8, 24
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    g = A.Geometry(ctx, "sq", nx, ny, ang)
+    b, dx, dy = shifted_views(ctx, g, nx, ny, na, 7, 1.5, ell=4)
+    views = b.download()
+    lam = 1.0 / (na * nx)
+    x = ctx.empty(g.vol_shape)
+    sx, sy, hl = A.match(ctx, g, b, x, lam, outer=2, maxit=6, cc_passes=3)
+    assert np.array_equal(b.download(), views)  # the measured views are not modified
+    osx, osy, ox = oracle_match(views, ang, nx, ny, lam, 2, 6, 3)
+    assert np.abs(sx - osx).max() <= 2e-3
+    assert np.abs(sy - osy).max() <= 2e-3
+    xg = x.download().astype(np.float64)
+    assert np.linalg.norm(xg - ox) <= 1e-3 * np.linalg.norm(ox)
+
+
+def gauge_free_error(sx, sy, dx, dy, ang):
+    ex, ey = (sx - dx).astype(np.float64), (sy - dy).astype(np.float64)
+    M = np.stack([np.cos(ang), np.sin(ang)], 1).astype(np.float64)
+    ex = ex - M @ np.linalg.lstsq(M, ex, rcond=None)[0]
+    ey = ey - ey.mean()
+    return ex, ey
+
+
+def test_match_recovers_shifts(ctx):
+    import arec3d_amd as A
+    from arec3d_amd import unet as U
+
+    print("\n[case] match recovers shifts 128x64, 180 angles", flush=True)
+    nx, ny, na = 128, 64, 180
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    g = A.Geometry(ctx, "sq", nx, ny, ang)
+    b, dx, dy = shifted_views(ctx, g, nx, ny, na, 50000, 3.0)
+    lam = 0.9 / U.sigma_max2(ctx, g)
+    x = ctx.empty(g.vol_shape)
+    sx, sy, hl = A.match(ctx, g, b, x, lam, outer=5, maxit=20, cc_passes=3)
+    ex, ey = gauge_free_error(sx, sy, dx, dy, ang)
+    print(f"max |error| x {np.abs(ex).max():.4f} px, y {np.abs(ey).max():.4f} px; "
+          f"raw x {np.abs(sx - dx).max():.3f}, y {np.abs(sy - dy).max():.3f}")
+    assert np.abs(ex).max() <= 0.1 and np.abs(ey).max() <= 0.1
