@@ -88,6 +88,8 @@ void geometry_sort_angles(Geometry &g, int a0, int a1, std::vector<int> &order);
 bool fwd_rpl_allowed(int ny, int rpl);
 bool geometry_plan_forward(Geometry &g, int rpl, int min_groups);
 void geometry_segments(Geometry &g);
+// Segment table of segments of width segw (nseg of them) for g's forward table.
+void segment_table(const Geometry &g, int segw, int nseg, std::vector<int> &seg);
 
 // Arguments of one forward / back application (device code reads them).
 struct FwdArgs {

@@ -164,9 +164,11 @@ bool geometry_plan_forward(Geometry &g, int rpl, int min_groups) {
 // Segment table of the current plan from pixstart: segment s (pixels
 // [s*segw, (s+1)*segw)) needs the entries whose first tap lies in
 // [s*segw - 2, (s+1)*segw), and the first tap of the entry before them.
-void geometry_segments(Geometry &g) {
+void geometry_segments(Geometry &g) { segment_table(g, g.segw, g.nseg, g.seg); }
+
+void segment_table(const Geometry &g, int segw, int nseg, std::vector<int> &seg) {
     const int nv = g.nvalid, npix = g.npix;
-    g.seg.assign((size_t)g.na * g.nseg * 4, 0);
+    seg.assign((size_t)g.na * nseg * 4, 0);
     for (int a = 0; a < g.na; a++) {
         const int *ps = &g.pixstart[(size_t)a * npix];
         // first entry whose first tap is >= k
@@ -177,9 +179,9 @@ void geometry_segments(Geometry &g) {
         // The first tap of entry e is the largest k with first_at(k) <= e;
         // qk tracks it for the (non-decreasing) entries start - 1.
         int qk = 0;
-        for (int s = 0; s < g.nseg; s++) {
-            int start = first_at(s * g.segw - 2), end = first_at((s + 1) * g.segw);
-            int *e = &g.seg[((size_t)a * g.nseg + s) * 4];
+        for (int s = 0; s < nseg; s++) {
+            int start = first_at(s * segw - 2), end = first_at((s + 1) * segw);
+            int *e = &seg[((size_t)a * nseg + s) * 4];
             e[0] = start;
             e[1] = end;
             if (start == 0) {

@@ -419,7 +419,7 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
         o_bp[k + 1] = o_bp[k] + std::max<int64_t>(1, backward_partial_count(*P[k].g));
     }
     const int cap = std::max(1, hist_cap);
-    DevBuf bdata, grad, proj, bp_part, fp_part, dstate, dhist, dvec, dall, dfwd, dbp, dbp0, dcnt;
+    DevBuf bdata, grad, proj, bp_part, fp_part, dstate, dhist, dvec, dall, dfwd, dbp, dbp0, dcnt, dseg1;
     int rc;
     if ((rc = alloc(bdata, sizeof(float) * o_nnz[nprob])) || (rc = alloc(grad, sizeof(float) * o_nnz[nprob])) ||
         (rc = alloc(proj, sizeof(float) * o_proj[nprob])) || (rc = alloc(bp_part, sizeof(double) * o_bp[nprob])) ||
@@ -474,11 +474,28 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
     std::vector<BatchBp> hb;   // gradient pass
     std::vector<BatchFwd> hf_all;
     if (nprob > 1) {
+        // One detector segment per angle group: a batch has parallelism
+        // across problems, and fewer, longer items cost less queue overhead
+        // than the many short items a single problem's plan asks for.
+        // Results do not depend on the plan; the geometries keep theirs.
+        std::vector<int> segw1(nprob);
+        std::vector<int64_t> o_seg(nprob + 1, 0);
+        std::vector<int> hseg;
+        for (int k = 0; k < nprob; k++) {
+            const Geometry &g = *P[k].g;
+            segw1[k] = (g.nx + kFwdOut - 1) / kFwdOut * kFwdOut;
+            std::vector<int> t;
+            segment_table(g, segw1[k], 1, t);
+            hseg.insert(hseg.end(), t.begin(), t.end());
+            o_seg[k + 1] = (int64_t)hseg.size();
+        }
+        if ((rc = alloc(dseg1, sizeof(int) * hseg.size()))) return rc;
+        AREC_HIP(hipMemcpyAsync(dseg1.p, hseg.data(), sizeof(int) * hseg.size(), hipMemcpyHostToDevice, st));
         std::vector<int> order(nprob);
         for (int k = 0; k < nprob; k++) order[k] = k;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
             const Geometry &ga = *P[a].g, &gb = *P[b].g;
-            return (int64_t)ga.nvalid / ga.nseg > (int64_t)gb.nvalid / gb.nseg;
+            return (int64_t)ga.nvalid * ga.fwd_rpl > (int64_t)gb.nvalid * gb.fwd_rpl;
         });
         for (int rpl : {4, 2, 1}) {
             FwdClass c{rpl, 0, 0, nullptr};
@@ -487,11 +504,14 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
                 if (g.fwd_rpl != rpl) continue;
                 BatchFwd f;
                 f.a = fwd_args(g, P[k].x, proj.as<float>() + o_proj[k], nullptr);
+                f.a.seg = reinterpret_cast<const int4 *>(dseg1.as<int>() + o_seg[k]);
+                f.a.nseg = 1;
+                f.a.segw = segw1[k];
                 f.skip = &S[k].skip_proj;
                 f.item0 = (int)c.total;
                 f.pad = 0;
                 hf_all.push_back(f);
-                c.total += forward_items(g);
+                c.total += (int64_t)((g.na + kFwdAngles - 1) / kFwdAngles) * g.fwd_chunks;
                 c.n++;
             }
             if (c.n) fcls.push_back(c);

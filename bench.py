@@ -308,8 +308,15 @@ def cgls_workload(args):
     t1 = time.perf_counter()
     _, x64, h64 = O.cgls("sq", bsub, ang, iters, 0.0, dtype=np.float64)
     t_cpu = time.perf_counter() - t1
+    log(f"FP64 oracle on rows {r0}..{r0 + nr}: {t_cpu:.1f} s")
+    _, x32, h32 = O.cgls("sq", bsub, ang, iters, 0.0, dtype=np.float32)
+    log("FP32 oracle done")
     xg = xs.download().astype(np.float64)
-    err = float(np.linalg.norm(xg - x64) / np.linalg.norm(x64))
+
+    def rel(a, b):
+        return float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))
+
+    err = rel(xg, x64)
     hist_err = float(max(abs(a[2] - c[2]) / c[2] for a, c in zip(hs, h64)))
     vu = 2.0 * g.nnz * na * iters
     out = {
@@ -328,7 +335,11 @@ def cgls_workload(args):
             "fp64_check": {
                 "rows": [r0, r0 + nr],
                 "x_rel_error_vs_fp64_oracle": err,
-                "max_rel_relnrm_deviation": hist_err,
+                "fp32_reference_semantics_x_rel_error_vs_fp64": rel(x32, x64),
+                "x_rel_difference_vs_fp32_oracle": rel(xg, x32),
+                "max_rel_relnrm_deviation_vs_fp64": hist_err,
+                "max_rel_relnrm_deviation_fp32_oracle_vs_fp64": float(
+                    max(abs(a[2] - c[2]) / c[2] for a, c in zip(h32, h64))),
                 "oracle_seconds": round(t_cpu, 1),
             },
         },
