@@ -40,6 +40,8 @@ constexpr int kTapBase = -4;  // first tap of the virtual entry before entry 0
 constexpr int kFwdAngles = 8;  // angles (wavefronts) per forward workgroup
 constexpr int kFwdOut = 8;     // pixels per staged output segment (32 bytes)
 constexpr int kFwdPhase = 2;   // detector pixels per lockstep phase of a forward workgroup
+// The back projector's tap table is built when it takes at most this much.
+constexpr int64_t kMaxBackTableBytes = 16ll << 30;
 
 struct Geometry {
     int method = kSQ;
@@ -63,6 +65,9 @@ struct Geometry {
     int *d_tile_cols = nullptr;
     int *d_tile_org = nullptr;
     uint4 *d_fwd = nullptr;       // na * nvalid entries
+    // back projector's taps in tile order, ((tile * na) + a) * kTileCols + slot:
+    // {first tap, w0, w1, w2}; nullptr when too large (taps computed on the fly)
+    uint4 *d_btab = nullptr;
     // forward sweep split into detector segments: per (angle, segment)
     // {first entry, end entry, first tap of the entry before the first}
     int nseg = 1, segw = 0, fwd_rpl = 1, fwd_chunks = 1, fwd_min_groups = 1024;
@@ -105,6 +110,7 @@ struct BpArgs {
     const float *p;
     float *out;
     const float *bvec;
+    const uint4 *btab;
     const int *tile_cols, *tile_org, *coord;
     const AngleConst *ang;
     double *partials;

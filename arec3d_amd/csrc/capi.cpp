@@ -153,8 +153,10 @@ int arec_geom_info(const arec_geom *geom, int64_t *nrays, int64_t *nnz, int64_t 
 int64_t arec_geom_device_bytes(const arec_geom *geom) {
     if (!geom) return 0;
     const Geometry &g = geom->g;
-    return (int64_t)sizeof(uint4) * g.na * g.nvalid + (int64_t)sizeof(AngleConst) * g.na +
-           (int64_t)sizeof(int) * (g.coord.size() + g.tile_cols.size() + g.tile_org.size());
+    const int64_t btab = g.d_btab ? (int64_t)sizeof(uint4) * g.ntiles * g.na * kTileCols : 0;
+    return (int64_t)sizeof(uint4) * g.na * g.nvalid + btab + (int64_t)sizeof(AngleConst) * g.na +
+           (int64_t)sizeof(int) * (g.coord.size() + g.tile_cols.size() + g.tile_org.size() + g.seg.size() +
+                                   g.pixstart.size());
 }
 
 int arec_project(arec_ctx *ctx, const arec_geom *geom, const float *x_dev, float *p_dev) {

@@ -14,15 +14,17 @@
 //    three detector pixels the column can touch in registers.  A pixel is
 //    complete once the walk has passed it and is stored then; every output
 //    pixel is written exactly once.
-//  * Back (voxel-driven gather, no atomics): one workgroup owns a tile of 64
-//    voxel columns inside an 8x8 box times 64 rows.  Per block of angles it
-//    stages the tile's detector footprint (<= 20 pixels x 64 rows per angle)
-//    in LDS, computes the 64 columns' taps once for the 4 wavefronts, and
-//    accumulates 16 rows per thread in registers.
+//  * Back (voxel-driven gather, no atomics): one workgroup owns a tile of
+//    256 voxel columns inside a 16x16 box times 16 rows, one column per
+//    thread.  Per block of 4 angles it stages the tile's detector footprint
+//    (<= 32 pixels x 16 rows per angle) in LDS, reads each column's taps from
+//    the geometry's tap table (or computes them), and accumulates the 16 rows
+//    in registers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "arec_internal.hpp"
 
@@ -141,6 +143,28 @@ __global__ void build_fwd_table(const int *__restrict__ order, int a0, int na_ba
         e.z = __float_as_uint(t.w1);
         e.w = __float_as_uint(t.w2);
         table[(size_t)a * nvalid + i] = e;
+    }
+}
+
+// Back projector's tap table: entry ((tile * na) + a) * kTileCols + slot holds
+// the taps of the tile's slot-th column at angle a (zeros for an empty slot),
+// computed by the same tap_weights<> as everywhere else.
+template <int M>
+__global__ void build_back_table(const int *__restrict__ tile_cols, int ntiles, int na,
+                                 const int *__restrict__ coord, const AngleConst *__restrict__ ang,
+                                 double radius, int xcent, uint4 *__restrict__ btab) {
+    const int64_t n = (int64_t)ntiles * na * kTileCols;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int slot = (int)(e % kTileCols);
+        const int64_t ta = e / kTileCols;
+        const int a = (int)(ta % na), tile = (int)(ta / na);
+        const int j = tile_cols[(int64_t)tile * kTileCols + slot];
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (j >= 0) {
+            Taps t = tap_weights<M>(coord[2 * j], coord[2 * j + 1], ang[a], radius, xcent);
+            v = make_uint4((unsigned)t.p0, __float_as_uint(t.w0), __float_as_uint(t.w1), __float_as_uint(t.w2));
+        }
+        btab[e] = v;
     }
 }
 
@@ -390,7 +414,9 @@ __global__ __launch_bounds__(64 * kFwdAngles) void forward_batch_kernel(const Ba
 // Back projector
 // ---------------------------------------------------------------------------
 constexpr int kBpThreads = kTileCols;  // one voxel column per thread
-constexpr int kBpRows = 64;            // rows per workgroup (registers per thread)
+// Rows per workgroup: 16 keeps the kernel at 94 VGPRs (5 waves per SIMD);
+// measured at 512^3/720: 64 rows 27.5 ms, 32 rows 22.8 ms, 16 rows 20.8 ms.
+constexpr int kBpRows = 16;
 constexpr int kBpRowPad = kBpRows + 4; // LDS row stride (16-byte aligned)
 constexpr int kBpAngBlk = 4;           // angles staged per LDS pass
 constexpr int kBpWin = 32;             // staged pixels per (angle)
@@ -404,15 +430,16 @@ __device__ __forceinline__ double corner_xb(int X, int Z, const AngleConst &A, d
 }
 
 // x[j][y] = sum_a sum_k w_k(a,j) p[a][y][p0+k] for the tile's columns (one
-// per thread) and 64 rows.  Per block of angles the tile's footprint (<= 32
-// pixels x 64 rows per angle) is staged in LDS with rows fastest, so one
+// per thread) and kBpRows rows.  Per block of angles the tile's footprint (<= 32
+// pixels x kBpRows rows per angle) is staged in LDS with rows fastest, so one
 // 16-byte LDS read gives 4 rows of one tap.  Epilogue by mode:
 //  kBpStore      out = acc
 //  kBpSirtGrad   out = bvec - acc (FP32), partial += sum out*(double)out
 //  kBpStoreNorm  out = acc, partial += sum (double)(acc*acc)
 template <int M, int MODE>
 __device__ __forceinline__ void backward_body(const float *__restrict__ p, float *__restrict__ out,
-                                              const float *__restrict__ bvec, const int *__restrict__ tile_cols,
+                                              const float *__restrict__ bvec, const uint4 *__restrict__ btab,
+                                              const int *__restrict__ tile_cols,
                                               const int *__restrict__ tile_org, const int *__restrict__ coord,
                                               const AngleConst *__restrict__ ang, double *__restrict__ partials,
                                               int *__restrict__ err, const double radius, const int xcent,
@@ -455,12 +482,20 @@ __device__ __forceinline__ void backward_body(const float *__restrict__ p, float
             if ((int)floor(hi) + 3 - u0 >= kBpWin) atomicOr(err, 2);
             s_u0[tid] = u0;
         }
-        // this column's taps for the staged angles (registers)
+        // this column's taps for the staged angles (registers): from the tap
+        // table (one coalesced 16-byte load per thread and angle) or computed
         Taps t[kBpAngBlk];
 #pragma unroll
         for (int ab = 0; ab < kBpAngBlk; ab++) {
             t[ab] = Taps{0, 0.f, 0.f, 0.f};
-            if (j >= 0 && ab < nb) t[ab] = tap_weights<M>(cX, cZ, ang[a0 + ab], radius, xcent);
+            if (j >= 0 && ab < nb) {
+                if (btab) {
+                    const uint4 e = btab[((size_t)tile * na + a0 + ab) * kTileCols + tid];
+                    t[ab] = Taps{(int)e.x, __uint_as_float(e.y), __uint_as_float(e.z), __uint_as_float(e.w)};
+                } else {
+                    t[ab] = tap_weights<M>(cX, cZ, ang[a0 + ab], radius, xcent);
+                }
+            }
         }
         __syncthreads();
         // stage the footprints, rows fastest in LDS
@@ -546,8 +581,8 @@ __device__ __forceinline__ void backward_body(const float *__restrict__ p, float
 
 template <int M, int MODE>
 __device__ __forceinline__ void backward_item(const BpArgs &B, int b) {
-    backward_body<M, MODE>(B.p, B.out, B.bvec, B.tile_cols, B.tile_org, B.coord, B.ang, B.partials, B.err,
-                           B.radius, B.xcent, B.nx, B.ny, B.na, B.nrowblk, b);
+    backward_body<M, MODE>(B.p, B.out, B.bvec, B.btab, B.tile_cols, B.tile_org, B.coord, B.ang, B.partials,
+                           B.err, B.radius, B.xcent, B.nx, B.ny, B.na, B.nrowblk, b);
 }
 
 template <int M, int MODE>
@@ -614,6 +649,7 @@ BpArgs bp_args(const Geometry &g, const float *p, float *x, const float *bvec, d
     B.p = p;
     B.out = x;
     B.bvec = bvec;
+    B.btab = g.d_btab;
     B.tile_cols = g.d_tile_cols;
     B.tile_org = g.d_tile_org;
     B.coord = g.d_coord;
@@ -643,6 +679,8 @@ void geometry_free_device(Geometry &g) {
     hipFree(g.d_tile_cols);
     hipFree(g.d_tile_org);
     hipFree(g.d_fwd);
+    hipFree(g.d_btab);
+    g.d_btab = nullptr;
     hipFree(g.d_err);
     hipFree(g.d_seg);
     hipFree(g.d_pixstart);
@@ -693,6 +731,20 @@ int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
     }
     AREC_HIP(hipStreamSynchronize(st));
     hipFree(d_order);
+    // AREC_NO_BACK_TABLE forces the on-the-fly taps (same results; tests use it)
+    const int64_t btab_entries = (int64_t)g.ntiles * g.na * kTileCols;
+    if (g.ntiles > 0 && btab_entries * (int64_t)sizeof(uint4) <= kMaxBackTableBytes &&
+        !getenv("AREC_NO_BACK_TABLE")) {
+        AREC_HIP(hipMalloc((void **)&g.d_btab, sizeof(uint4) * btab_entries));
+        dim3 bg((unsigned)std::min<int64_t>((btab_entries + 255) / 256, 65536));
+        if (g.method == kSQ)
+            build_back_table<kSQ><<<bg, 256, 0, st>>>(g.d_tile_cols, g.ntiles, g.na, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_btab);
+        else if (g.method == kKB)
+            build_back_table<kKB><<<bg, 256, 0, st>>>(g.d_tile_cols, g.ntiles, g.na, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_btab);
+        else
+            build_back_table<kLinear><<<bg, 256, 0, st>>>(g.d_tile_cols, g.ntiles, g.na, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_btab);
+        AREC_HIP(hipGetLastError());
+    }
     geometry_segments(g);
     if ((rc = upload(&g.d_seg, g.seg, st))) return rc;
     if ((rc = upload(&g.d_pixstart, g.pixstart, st))) return rc;

@@ -155,3 +155,24 @@ def test_reference_known_answers_dropin(ctx):
     bad = A.arecAllocateCBImage(5, 2, 1)
     assert A.arecProject2D_SQ(cyl, np.zeros(1, np.float32), 1, bad) == 2
     assert A.arecBackProject2D_SQ(bad, np.zeros(1, np.float32), 1, cyl) == -1
+
+
+@pytest.mark.parametrize("method", ["sq", "kb", "linear"])
+def test_back_tap_table_and_computed_taps_agree(ctx, method, monkeypatch):
+    """The back projector reads its taps from a per-geometry table when that
+    fits (kMaxBackTableBytes), else computes them; both give the same bits."""
+    import arec3d_amd as A
+
+    print(f"\n[case] back taps table vs computed {method}", flush=True)
+    nx, ny, na = 97, 70, 37
+    rng = np.random.default_rng(9)
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    p = ctx.to_device(rng.random((na, ny, nx)).astype(np.float32))
+    g1 = A.Geometry(ctx, method, nx, ny, ang)
+    monkeypatch.setenv("AREC_NO_BACK_TABLE", "1")
+    g2 = A.Geometry(ctx, method, nx, ny, ang)
+    assert g2.device_bytes() < g1.device_bytes()
+    x1, x2 = ctx.empty(g1.vol_shape), ctx.empty(g2.vol_shape)
+    g1.backproject(p, x1)
+    g2.backproject(p, x2)
+    assert np.array_equal(x1.download(), x2.download())

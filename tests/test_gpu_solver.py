@@ -183,3 +183,35 @@ def test_graph_and_eager_agree(ctx, monkeypatch):
     h2, _ = g.sirt(db, x2, 1.0 / (na * nx), 25, 1e-9)
     assert h1 == h2
     assert np.array_equal(x1.download(), x2.download())
+
+
+def test_sirt_config1_shape_rows_vs_oracle(ctx):
+    """The CPU reference case at its own shape: 128 wide, 64 rows, 180 angles,
+    12 seeded ellipsoids (support 60, semi-axes U[4,30], density
+    U[0.1,1]), 1% noise, 50 SIRT iterations from zero.  With a fixed lam, no
+    restart and no tolerance stop, SIRT is separable by rows, so the oracle
+    solved on 4 of the rows must give the GPU volume's same 4 rows (1e-4
+    relative: FP32 summation order over 50 iterations)."""
+    import arec3d_amd as A
+    from arec3d_amd import phantom as P
+    from arec3d_amd import unet as U
+
+    print("\n[case] sirt 128x64x180, 50 iterations, rows vs oracle", flush=True)
+    nx, ny, na = 128, 64, 180
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    g = A.Geometry(ctx, "sq", nx, ny, ang)
+    prm = P.ellipsoid_params(1, 12, 60.0, ny, axes=(4.0, 30.0), dens=(0.1, 1.0))
+    xt = ctx.empty(g.vol_shape)
+    P.phantom_device(ctx, xt, g.radius, ny, 60.0, prm)
+    b = ctx.empty(g.proj_shape)
+    g.project(xt, b)
+    P.add_noise(ctx, b, 1, rel_sigma=0.01)
+    lam = 0.9 / U.sigma_max2(ctx, g)
+    x = ctx.empty(g.vol_shape)
+    hg, _ = g.sirt(b, x, lam, 50, 0.0)
+    assert len(hg) == 50 and all(h[1] == 0 for h in hg)
+    r0 = 30
+    bsub = np.ascontiguousarray(b.download()[:, r0:r0 + 4, :])
+    st, xo, ho, lo = O.sirt("sq", bsub, ang, lam, 50, 0.0)
+    assert st == 0 and all(h[1] == 0 for h in ho)
+    assert relnorm(x.download()[:, r0:r0 + 4], xo) <= 1e-4
