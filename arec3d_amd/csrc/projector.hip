@@ -588,6 +588,10 @@ __device__ __forceinline__ void backward_item(const BpArgs &B, int b) {
 template <int M, int MODE>
 __global__ __launch_bounds__(kBpThreads) void backward_kernel(BpArgs B, const int *__restrict__ skip) {
     if (skip && *skip) return;
+    // Workgroup ids go round-robin over the XCDs, so each XCD takes the same
+    // row blocks of neighbouring tiles, which share detector pixels (cheaper
+    // than grouping a tile's row blocks on one XCD: 42 vs 61 GB fetched per
+    // launch at 512^3/720).
     backward_item<M, MODE>(B, blockIdx.x);
 }
 
