@@ -81,6 +81,8 @@ def _load():
         "arec_sirt_batch": ([vp, i32, P(vp), P(vp), P(vp), P(flt), i32, flt, P(Hist), i32, P(i32), P(flt)],
                             C.c_int),
         "arec_geom_plan": ([vp, vp, i32, i32], C.c_int),
+        "arec_geom_set_forward": ([vp, vp, i32, i32, i32], C.c_int),
+        "arec_geom_forward_design": ([vp], i32),
         "arec_match": ([vp, vp, vp, vp, flt, i32, i32, flt, i32, i32, vp, vp, P(Hist)], C.c_int),
         "arec_geom_get_plan": ([vp, P(i32), P(i32), P(i32)], C.c_int),
         "arec_unet_create": ([vp, P(vp), i32, i32, i32, i32, C.c_uint64], C.c_int),
@@ -120,7 +122,8 @@ EXPORTED = [
     "arec_unet_destroy", "arec_unet_nparams", "arec_unet_get_params", "arec_unet_set_params",
     "arec_unet_set_scale", "arec_unet_forward", "arec_unet_train_step", "arec_unet_gradient",
     "arec_conv3_check", "arec_conv3_wgrad_check", "arec_cyl_to_cube", "arec_cube_to_cyl", "arec_h_project", "arec_h_backproject",
-    "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan", "arec_match", "arec_scale",
+    "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan", "arec_match", "arec_scale", "arec_geom_set_forward",
+    "arec_geom_forward_design",
 ]
 
 
@@ -292,6 +295,14 @@ class Geometry:
     def plan(self, rows_per_lane=0, min_groups=0):
         """Set the forward projector's launch plan (results do not depend on it)."""
         _check(lib.arec_geom_plan(self.ctx.h, self.h, int(rows_per_lane), int(min_groups)))
+
+    def set_forward(self, design, rows_per_lane=0, min_groups=0):
+        """Forward design 1 (detector sweep) or 2 (depth strips) and its plan."""
+        _check(lib.arec_geom_set_forward(self.ctx.h, self.h, int(design), int(rows_per_lane), int(min_groups)))
+
+    @property
+    def forward_design(self):
+        return int(lib.arec_geom_forward_design(self.h))
 
     def get_plan(self):
         r, m, s = C.c_int32(), C.c_int32(), C.c_int32()

@@ -1,21 +1,27 @@
-"""Autotuner of the forward projector's launch plan (no counterpart in the
-reference).
+"""Autotuner of the forward projector's design and launch plan (no
+counterpart in the reference).
 
-The forward projector's results do not depend on its plan; its speed does.
-A plan is (detector rows per lane, minimum workgroup count): rows per lane
-sets how many rows one wavefront carries (1, 2 or 4: one, two or four FP32
-per lane per column read), the workgroup count sets how finely each angle's
-detector is split into segments.  :func:`autotune` times every allowed plan
-on the geometry with HIP events and keeps the fastest; results are cached in
-a JSON file keyed by (method, nx, ny, na) so a shape is timed once.
+The forward projector's results do not depend on its plan and agree between
+designs to FP32 summation order; its speed depends on both.  Design 1 (the
+detector sweep) has plans (detector rows per lane, minimum workgroup count):
+rows per lane sets how many rows one wavefront carries (1, 2 or 4: one, two
+or four FP32 per lane per column read), the workgroup count how finely each
+angle's detector is split into segments.  Design 2 (depth strips) has only
+rows per lane; its table is built while it is timed and released again when
+design 1 wins.  :func:`autotune` times every candidate on the geometry with
+HIP events and keeps the fastest; results are cached in a JSON file keyed by
+(method, nx, ny, na) so a shape is timed once.
 """
 import json
 import os
 
 import numpy as np
 
+import arec3d_amd as A
+
 ROWS_PER_LANE = (4, 2, 1)
 MIN_GROUPS = (512, 1024, 2048, 4096)
+STRIP_ROWS_PER_LANE = (4, 2)
 
 
 def default_cache():
@@ -24,7 +30,7 @@ def default_cache():
 
 
 def _key(g):
-    return f"{g.method}:{g.nx}:{g.ny}:{g.na}"
+    return f"designs:{g.method}:{g.nx}:{g.ny}:{g.na}"
 
 
 def _load(path):
@@ -35,31 +41,48 @@ def _load(path):
         return {}
 
 
-def autotune(ctx, g, cache=None, reps=3, log=None):
-    """Pick and set the fastest forward plan for geometry g; returns
-    (rows_per_lane, min_groups, ms)."""
+def candidates(g, designs=(1, 2)):
+    """(design, rows_per_lane, min_groups) plans allowed for geometry g."""
+    out = []
+    if 1 in designs:
+        out += [(1, r, m) for r in ROWS_PER_LANE if g.ny % r == 0 for m in MIN_GROUPS]
+    if 2 in designs:
+        out += [(2, r, 0) for r in STRIP_ROWS_PER_LANE if g.ny % r == 0]
+    return out
+
+
+def apply(g, plan):
+    design, r, m = plan
+    g.set_forward(design, r, m)
+
+
+def autotune(ctx, g, cache=None, reps=3, log=None, designs=(1, 2)):
+    """Pick and set the fastest forward design and plan for geometry g;
+    returns (design, rows_per_lane, min_groups, ms)."""
     path = cache or default_cache()
     table = _load(path) if path else {}
     k = _key(g)
     if k in table:
-        r, m, ms = table[k]
-        g.plan(r, m)
-        return r, m, ms
+        d, r, m, ms = table[k]
+        apply(g, (d, r, m))
+        return d, r, m, ms
     x = ctx.to_device(np.ones(g.vol_shape, np.float32))
     p = ctx.empty(g.proj_shape)
     best = None
-    for r in ROWS_PER_LANE:
-        if g.ny % r:
-            continue
-        for m in MIN_GROUPS:
-            g.plan(r, m)
-            g.time_operator(0, x, p, 1)  # warm
-            ms = g.time_operator(0, x, p, reps)
+    for plan in candidates(g, designs):
+        try:
+            apply(g, plan)
+        except A.ArecError as e:  # e.g. the depth-strip table does not fit
             if log:
-                log(f"plan rows/lane {r} groups>={m}: {ms:.3f} ms")
-            if best is None or ms < best[2]:
-                best = (r, m, ms)
-    g.plan(best[0], best[1])
+                log(f"design {plan[0]} rows/lane {plan[1]}: skipped ({e})")
+            continue
+        g.time_operator(0, x, p, 1)  # warm
+        ms = g.time_operator(0, x, p, reps)
+        if log:
+            log(f"design {plan[0]} rows/lane {plan[1]} groups>={plan[2]}: {ms:.3f} ms")
+        if best is None or ms < best[3]:
+            best = (*plan, ms)
+    apply(g, best[:3])
     if path:
         table = _load(path)
         table[k] = list(best)

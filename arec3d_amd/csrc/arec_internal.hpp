@@ -40,8 +40,19 @@ constexpr int kTapBase = -4;  // first tap of the virtual entry before entry 0
 constexpr int kFwdAngles = 8;  // angles (wavefronts) per forward workgroup
 constexpr int kFwdOut = 8;     // pixels per staged output segment (32 bytes)
 constexpr int kFwdPhase = 2;   // detector pixels per lockstep phase of a forward workgroup
+// Depth-strip forward design: detector strips of kStripW pixels, entries of
+// a strip walked in depth bins of kDepthBin pixels along the ray, and inside
+// a bin ordered by their first tap relative to the strip (kStripGroups
+// values: a column's three taps start at most 2 pixels before the strip).
+constexpr int kStripW = 8;
+constexpr int kDepthBin = 16;
+constexpr int kStripGroups = kStripW + 2;
+constexpr int kStripAngles = 4;  // angles (wavefronts) per depth-strip workgroup
+constexpr int kStripBatch = 8;   // depth-strip table entries whose loads are in flight together
 // The back projector's tap table is built when it takes at most this much.
 constexpr int64_t kMaxBackTableBytes = 16ll << 30;
+// The depth-strip forward table is built (on request) up to this size.
+constexpr int64_t kMaxStripTableBytes = 32ll << 30;
 
 struct Geometry {
     int method = kSQ;
@@ -79,6 +90,21 @@ struct Geometry {
     std::vector<int> pixstart;    // na * npix
     int *d_pixstart = nullptr;
     int *d_err = nullptr;         // geometry self-check flag
+    // Forward design: 1 = detector sweep (the table above, pixstart, seg);
+    // 2 = depth strips (built on request).  Per angle a, the strip table
+    // holds, for every strip s (pixels [s*kStripW, (s+1)*kStripW)), the
+    // entries whose first tap lies in [s*kStripW - 2, (s+1)*kStripW),
+    // ordered by depth bin b along the ray and inside a bin by d = first
+    // tap - (s*kStripW - 2) in [0, kStripGroups); entry = {column | d << 22,
+    // w0, w1, w2}.  strip_off[a][s*nbins + b] is the first entry of (s, b)
+    // relative to strip_base[a].
+    int fwd_design = 1;
+    int nstrips = 0, nbins = 0, strip_rpl = 1;
+    std::vector<int64_t> strip_base;  // na + 1
+    std::vector<int> strip_off;       // na * (nstrips * nbins + 1)
+    uint4 *d_strip = nullptr;
+    int64_t *d_strip_base = nullptr;
+    int *d_strip_off = nullptr;
     bool device_ready = false;
 };
 
@@ -93,6 +119,11 @@ void geometry_sort_angles(Geometry &g, int a0, int a1, std::vector<int> &order);
 bool fwd_rpl_allowed(int ny, int rpl);
 bool geometry_plan_forward(Geometry &g, int rpl, int min_groups);
 void geometry_segments(Geometry &g);
+// Depth-strip ordering (design 2): strip_count fills nstrips, nbins and
+// strip_base; strip_angles fills strip_off for angles [a0, a1) and order
+// with (column | strip << 22) per entry, angle after angle.
+void geometry_strip_count(Geometry &g);
+void geometry_strip_angles(Geometry &g, int a0, int a1, std::vector<uint32_t> &order);
 // Segment table of segments of width segw (nseg of them) for g's forward table.
 void segment_table(const Geometry &g, int segw, int nseg, std::vector<int> &seg);
 
@@ -137,6 +168,9 @@ void geometry_free_device(Geometry &g);
 // Re-plan the forward launch (rows per lane, minimum workgroup count) of an
 // uploaded geometry: recomputes and uploads the segment table.
 int geometry_replan_forward(Geometry &g, int rpl, int min_groups, hipStream_t st);
+// Select the forward design (1 sweep, 2 depth strips; 2 builds its table on
+// first use) and rows per lane.
+int geometry_set_forward(Geometry &g, int design, int rpl, int min_groups, hipStream_t st);
 // Modes of the back projector's epilogue.
 enum BpMode { kBpStore = 0, kBpSirtGrad = 1, kBpStoreNorm = 2 };
 int launch_forward(const Geometry &g, const float *x, float *p, double *partials, hipStream_t st,

@@ -195,6 +195,88 @@ void segment_table(const Geometry &g, int segw, int nseg, std::vector<int> &seg)
     }
 }
 
+// Depth along the ray of column j at angle a (pixels, centred), the same
+// coordinates as host_first_tap.
+static double host_depth(const Geometry &g, int a, int j) {
+    const AngleConst &A = g.ang[a];
+    double c = g.method == kLinear ? (double)g.xcent : g.radius;
+    double xc = (double)g.coord[2 * j] - c, zc = (double)g.coord[2 * j + 1] - c;
+    return zc * A.c - xc * A.s;
+}
+
+static int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+// The strips whose entry ranges hold a column with first tap p0: s0 = p0
+// div kStripW and, when p0 is within 2 pixels of the next strip, s0 + 1.
+static int strips_of(int p0, int nstrips, int *s) {
+    int n = 0, s0 = floor_div(p0, kStripW);
+    if (s0 >= 0 && s0 < nstrips) s[n++] = s0;
+    if (p0 - s0 * kStripW >= kStripW - 2 && s0 + 1 >= 0 && s0 + 1 < nstrips) s[n++] = s0 + 1;
+    return n;
+}
+
+void geometry_strip_count(Geometry &g) {
+    g.nstrips = (g.nx + kStripW - 1) / kStripW;
+    double c = g.method == kLinear ? (double)g.xcent : g.radius;
+    g.nbins = (int)std::ceil((2 * c + 4) / kDepthBin) + 1;
+    g.strip_base.assign((size_t)g.na + 1, 0);
+    std::vector<int64_t> cnt(g.na, 0);
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    nth = std::min<unsigned>(nth, (unsigned)g.na);
+    auto work = [&](unsigned t) {
+        int s[2];
+        for (int a = (int)t; a < g.na; a += (int)nth) {
+            int64_t n = 0;
+            for (int j = 0; j < g.nvalid; j++) n += strips_of(host_first_tap(g, a, j), g.nstrips, s);
+            cnt[a] = n;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; t++) pool.emplace_back(work, t);
+    for (auto &th : pool) th.join();
+    for (int a = 0; a < g.na; a++) g.strip_base[a + 1] = g.strip_base[a] + cnt[a];
+    g.strip_off.assign((size_t)g.na * ((size_t)g.nstrips * g.nbins + 1), 0);
+}
+
+void geometry_strip_angles(Geometry &g, int a0, int a1, std::vector<uint32_t> &order) {
+    const int nb = g.nstrips * g.nbins, nk = nb * kStripGroups;
+    const int64_t e0 = g.strip_base[a0];
+    order.resize((size_t)(g.strip_base[a1] - e0));
+    double c = g.method == kLinear ? (double)g.xcent : g.radius;
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    nth = std::min<unsigned>(nth, (unsigned)(a1 - a0));
+    auto work = [&](unsigned t) {
+        std::vector<int> cnt(nk + 1);
+        std::vector<int> nstr(g.nvalid);
+        std::vector<int> str(2 * (size_t)g.nvalid), key(2 * (size_t)g.nvalid);
+        for (int a = a0 + (int)t; a < a1; a += (int)nth) {
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (int j = 0; j < g.nvalid; j++) {
+                int b = (int)std::floor((host_depth(g, a, j) + c + 1.0) / kDepthBin);
+                b = std::min(std::max(b, 0), g.nbins - 1);
+                const int p0 = host_first_tap(g, a, j);
+                nstr[j] = strips_of(p0, g.nstrips, &str[2 * (size_t)j]);
+                for (int k = 0; k < nstr[j]; k++) {
+                    const int sj = str[2 * (size_t)j + k], d = p0 - (sj * kStripW - 2);
+                    key[2 * (size_t)j + k] = (sj * g.nbins + b) * kStripGroups + d;
+                    cnt[key[2 * (size_t)j + k] + 1]++;
+                }
+            }
+            for (int i = 1; i <= nk; i++) cnt[i] += cnt[i - 1];
+            int *off = &g.strip_off[(size_t)a * (nb + 1)];
+            for (int i = 0; i <= nb; i++) off[i] = cnt[(size_t)i * kStripGroups];
+            uint32_t *out = order.data() + (g.strip_base[a] - e0);
+            for (int j = 0; j < g.nvalid; j++)
+                for (int k = 0; k < nstr[j]; k++)
+                    out[cnt[key[2 * (size_t)j + k]]++] =
+                        (uint32_t)j | ((uint32_t)str[2 * (size_t)j + k] << kColBits);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; t++) pool.emplace_back(work, t);
+    for (auto &th : pool) th.join();
+}
+
 // Stable counting sort of the contributing columns of angles [a0, a1) by
 // their first tap; order receives (a1-a0)*nvalid column indices.  Also
 // fills pixstart: for each q, the first entry whose first tap is >=

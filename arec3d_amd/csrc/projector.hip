@@ -168,6 +168,33 @@ __global__ void build_back_table(const int *__restrict__ tile_cols, int ntiles, 
     }
 }
 
+// Depth-strip table (design 2) for angles [a0, a0 + nab): entry e of angle a
+// (order[e] = column | strip << 22) -> {column | d << 22, w0, w1, w2} with
+// d = first tap - (strip * kStripW - 2), which must lie in [0, kStripW + 1].
+template <int M>
+__global__ void build_strip_table(const uint32_t *__restrict__ order, const int64_t *__restrict__ base, int a0,
+                                  int nab, const int *__restrict__ coord, const AngleConst *__restrict__ ang,
+                                  double radius, int xcent, uint4 *__restrict__ tab, int *__restrict__ err) {
+    const int ab = blockIdx.y;
+    if (ab >= nab) return;
+    const int a = a0 + ab;
+    const int64_t lo = base[a] - base[a0], n = base[a + 1] - base[a];
+    const AngleConst A = ang[a];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = order[lo + i];
+        const int j = (int)(o & (kMaxCols - 1)), sidx = (int)(o >> kColBits);
+        Taps t = tap_weights<M>(coord[2 * j], coord[2 * j + 1], A, radius, xcent);
+        int d = t.p0 - (sidx * kStripW - 2);
+        if (d < 0 || d > kStripW + 1) {
+            atomicOr(err, 8);
+            d = 0;
+            t.w0 = t.w1 = t.w2 = 0.0f;
+        }
+        tab[base[a] + i] = make_uint4((unsigned)j | ((unsigned)d << kColBits), __float_as_uint(t.w0),
+                                      __float_as_uint(t.w1), __float_as_uint(t.w2));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Forward projector
 // ---------------------------------------------------------------------------
@@ -372,6 +399,229 @@ __global__ __launch_bounds__(64 * kFwdAngles) void forward_kernel(
     int b = blockIdx.x;
     if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
     forward_body<RPL>(x, p, table, seg, pixstart, partials, npix, nvalid, nseg, segw, nx, ny, na, nchunks, b);
+}
+
+// acc[D..D+2] += w * v for one entry of the depth-strip walk (D is the
+// entry's first tap relative to the strip start - 2, a constant per group).
+template <int D, int RPL>
+__device__ __forceinline__ void strip_add(float (&acc)[kStripW + 4][RPL], float w0, float w1, float w2,
+                                          const float (&v)[RPL]) {
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+        acc[D][r] = fmaf(w0, v[r], acc[D][r]);
+        acc[D + 1][r] = fmaf(w1, v[r], acc[D + 1][r]);
+        acc[D + 2][r] = fmaf(w2, v[r], acc[D + 2][r]);
+    }
+}
+
+// The wave's strip table entries [s0, last] stream through a ring of 128
+// entries in LDS (slot = entry index mod 128), kStripAhead entries ahead of
+// the walk.  Every batch issues the same memory operations in the same
+// order: it writes the previous batch's fetched entries to the ring, loads
+// its columns' rows, then fetches (lanes 0-7, one entry each) the 8 entries
+// kStripAhead past its own first entry.  The fetch is the youngest load when
+// the batch waits for its rows, so it stays in flight; it is written one
+// batch later, and read about kStripAhead entries later still.  A write
+// replaces entries at least 128 - kStripAhead - 8 before the batch, which no
+// batch reads again.
+constexpr int kStripAhead = 64;
+
+struct StripRing {
+    uint4 *ring;  // this wave's 128 entries in LDS
+    const uint4 *__restrict__ t;
+    int s0, last;
+    int fetched;  // first entry of the fetch in flight
+    uint4 pend;   // that fetch (lane l: entry fetched + l, l < 8)
+};
+
+__device__ __forceinline__ uint4 strip_entry_load(const StripRing &R, int e) { return R.t[min(e, R.last)]; }
+
+// Word q of entry idx (s0 <= idx <= last, in the ring), wave-uniform.
+__device__ __forceinline__ unsigned strip_ring_word(const StripRing &R, const int idx, const int q) {
+    const unsigned *r = reinterpret_cast<const unsigned *>(R.ring);
+    return __builtin_amdgcn_readfirstlane(r[4 * (idx & 127) + q]);
+}
+
+// acc[D..D+2] += w * v for the batch's entries whose first tap is the
+// strip start - 2 + D (weights of the other entries are zero).
+template <int D, int RPL>
+__device__ __forceinline__ void strip_add_batch(float (&acc)[kStripW + 4][RPL], const unsigned (&dk)[kStripBatch],
+                                                const float (&w)[kStripBatch][3],
+                                                const float (&v)[kStripBatch][RPL]) {
+#pragma unroll
+    for (int k = 0; k < kStripBatch; k++) {
+        const bool on = dk[k] == (unsigned)D;
+        strip_add<D, RPL>(acc, on ? w[k][0] : 0.0f, on ? w[k][1] : 0.0f, on ? w[k][2] : 0.0f, v[k]);
+    }
+}
+
+template <int D, int RPL>
+__device__ __forceinline__ void strip_add_present(float (&acc)[kStripW + 4][RPL], const unsigned present,
+                                                  const unsigned (&dk)[kStripBatch],
+                                                  const float (&w)[kStripBatch][3],
+                                                  const float (&v)[kStripBatch][RPL]) {
+    if (present & (1u << D)) strip_add_batch<D, RPL>(acc, dk, w, v);
+    if constexpr (D + 1 < kStripGroups) strip_add_present<D + 1, RPL>(acc, present, dk, w, v);
+}
+
+// One batch: entries i .. i + kStripBatch - 1 of the strip (indices clamped
+// to last so every load is valid; entries past last add zero).  Every entry
+// is accumulated for each first-tap offset present in the batch, with the
+// weights of the entries at other offsets zeroed, so the batch needs no
+// alignment to the offsets (entries are ordered by offset within a bin, so
+// a batch usually spans one or two).  FULL: all of the wave's rows lie
+// inside the volume.
+template <int RPL, bool FULL>
+__device__ __forceinline__ void strip_batch(StripRing &R, const int i, const float *__restrict__ x, const int ny,
+                                            const int y0, const int lane, float (&acc)[kStripW + 4][RPL]) {
+    static_assert(kStripBatch <= 8 && kStripAhead >= 2 * kStripBatch && kStripAhead + 8 <= 128,
+                  "strip ring sizes");
+    if (lane < 8) R.ring[(R.fetched + lane) & 127] = R.pend;
+    float v[kStripBatch][RPL];
+#pragma unroll
+    for (int k = 0; k < kStripBatch; k++) {
+        const int col = (int)(strip_ring_word(R, min(i + k, R.last), 0) & (kMaxCols - 1));
+        const float *__restrict__ xj = x + (size_t)col * ny + y0;
+        if constexpr (FULL && RPL == 4) {
+            const float4 q = *reinterpret_cast<const float4 *>(xj);
+            v[k][0] = q.x;
+            v[k][RPL > 1 ? 1 : 0] = q.y;
+            v[k][RPL > 2 ? 2 : 0] = q.z;
+            v[k][RPL > 3 ? 3 : 0] = q.w;
+        } else if constexpr (FULL && RPL == 2) {
+            const float2 q = *reinterpret_cast<const float2 *>(xj);
+            v[k][0] = q.x;
+            v[k][RPL > 1 ? 1 : 0] = q.y;
+        } else if constexpr (FULL) {
+            v[k][0] = xj[0];
+        } else {
+#pragma unroll
+            for (int r = 0; r < RPL; r++) v[k][r] = (y0 + r < ny) ? xj[r] : 0.0f;
+        }
+    }
+    R.fetched = i + kStripAhead;
+    R.pend = strip_entry_load(R, R.fetched + (lane & 7));
+    unsigned dk[kStripBatch], present = 0;
+    float w[kStripBatch][3];
+#pragma unroll
+    for (int k = 0; k < kStripBatch; k++) {
+        const int e = min(i + k, R.last);
+        dk[k] = i + k <= R.last ? strip_ring_word(R, e, 0) >> kColBits : (unsigned)kStripGroups;
+        present |= 1u << dk[k];
+#pragma unroll
+        for (int q = 0; q < 3; q++) w[k][q] = __uint_as_float(strip_ring_word(R, e, 1 + q));
+    }
+    strip_add_present<0, RPL>(acc, present, dk, w, v);
+}
+
+// The wave's walk over its strip's entries [o[0], o[nbins]) in batches, with
+// a workgroup barrier whenever the walk has passed the end of a depth bin
+// (every wave meets nbins barriers; the waves share no data, only the
+// lockstep).  The ring starts with entries [s0, s0 + kStripAhead) and the
+// fetch of the 8 after them.
+template <int RPL, bool FULL>
+__device__ __forceinline__ void strip_walk(uint4 *ring, const float *__restrict__ x, const uint4 *__restrict__ t,
+                                           const int *__restrict__ o, const int nbins, const bool active,
+                                           const int ny, const int y0, const int lane,
+                                           float (&acc)[kStripW + 4][RPL]) {
+    StripRing R;
+    R.ring = ring;
+    R.t = t;
+    R.s0 = __builtin_amdgcn_readfirstlane(o[0]);
+    R.last = __builtin_amdgcn_readfirstlane(o[nbins]) - 1;
+    const bool any = active && R.last >= R.s0;
+    if (any) {
+        static_assert(kStripAhead <= 64, "one entry per lane at the start");
+        if (lane < kStripAhead) R.ring[(R.s0 + lane) & 127] = strip_entry_load(R, R.s0 + lane);
+        R.fetched = R.s0 + kStripAhead;
+        R.pend = strip_entry_load(R, R.fetched + (lane & 7));
+    }
+    int i = R.s0;
+    for (int bin = 0; bin < nbins; bin++) {
+        if (any) {
+            const int bend = __builtin_amdgcn_readfirstlane(o[bin + 1]);
+            for (; i < bend; i += kStripBatch) strip_batch<RPL, FULL>(R, i, x, ny, y0, lane, acc);
+        }
+        __builtin_amdgcn_s_barrier();
+    }
+}
+
+// Forward projection, depth-strip design.  Workgroup b: angle group b % nag
+// (kStripAngles waves, one angle each), then detector strip, then row
+// chunk.  Each wave walks its angle's entries of the strip bin by bin in
+// depth along the ray; the waves meet at a barrier after every bin, so
+// neighbouring angles read nearly the same columns at nearly the same time.
+// The strip's pixels stay in registers (an entry's first-tap offset selects
+// a fixed register set); every pixel of the strip is written once at the
+// end.
+template <int RPL>
+__device__ __forceinline__ void forward_strip_body(const float *__restrict__ x, float *__restrict__ p,
+                                                   const uint4 *__restrict__ tab,
+                                                   const int64_t *__restrict__ base,
+                                                   const int *__restrict__ off, double *__restrict__ partials,
+                                                   const int nstrips, const int nbins, const int nx, const int ny,
+                                                   const int na, const int nchunks, int b) {
+    constexpr int ROWS = 64 * RPL;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int nag = (na + kStripAngles - 1) / kStripAngles;
+    const int ag = b % nag;
+    const int rest = b / nag;
+    const int st = rest % nstrips;
+    const int chunk = rest / nstrips;
+    const int a_raw = ag * kStripAngles + wave;
+    const bool active = a_raw < na;
+    const int a = active ? a_raw : na - 1;
+    const int ybase = chunk * ROWS;
+    const int y0 = ybase + lane * RPL;
+    const bool full_rows = (ybase + ROWS <= ny);
+    const uint4 *__restrict__ t = tab + base[a];
+    const int *__restrict__ o = off + (size_t)a * ((size_t)nstrips * nbins + 1) + (size_t)st * nbins;
+    float acc[kStripW + 4][RPL];
+#pragma unroll
+    for (int i = 0; i < kStripW + 4; i++)
+#pragma unroll
+        for (int r = 0; r < RPL; r++) acc[i][r] = 0.0f;
+    __shared__ uint4 s_ring[kStripAngles][2 * 64];
+    if (full_rows)
+        strip_walk<RPL, true>(s_ring[wave], x, t, o, nbins, active, ny, y0, lane, acc);
+    else
+        strip_walk<RPL, false>(s_ring[wave], x, t, o, nbins, active, ny, y0, lane, acc);
+    if (!active) return;
+    // pixels st*kStripW + q, q < kStripW, are acc[q + 2]
+    double sq = 0.0;
+    const int u0 = st * kStripW;
+    float *__restrict__ pa = p + (size_t)a * nx * ny;
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+        const int y = y0 + r;
+        if (y >= ny) continue;
+#pragma unroll
+        for (int q = 0; q < kStripW; q++) {
+            if (u0 + q < nx) {
+                const float val = acc[q + 2][r];
+                pa[(size_t)y * nx + u0 + q] = val;
+                const float s2 = val * val;
+                sq += (double)s2;
+            }
+        }
+    }
+    if (partials) {
+#pragma unroll
+        for (int off2 = 32; off2 > 0; off2 >>= 1) sq += __shfl_xor(sq, off2, 64);
+        if (lane == 0) partials[((size_t)a * nchunks + chunk) * nstrips + st] = sq;
+    }
+}
+
+template <int RPL>
+__global__ __launch_bounds__(64 * kStripAngles) void forward_strip_kernel(
+    const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ tab,
+    const int64_t *__restrict__ base, const int *__restrict__ off, double *__restrict__ partials, int nstrips,
+    int nbins, int nx, int ny, int na, int nchunks, const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    const int nblk = gridDim.x;
+    int b = blockIdx.x;
+    if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+    forward_strip_body<RPL>(x, p, tab, base, off, partials, nstrips, nbins, nx, ny, na, nchunks, b);
 }
 
 // Index of the problem that owns work item `it`: the last k with
@@ -616,7 +866,10 @@ __global__ __launch_bounds__(kBpThreads) void backward_batch_kernel(const BatchB
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-int64_t forward_partial_count(const Geometry &g) { return (int64_t)g.na * g.fwd_chunks * g.nseg; }
+int64_t forward_partial_count(const Geometry &g) {
+    if (g.fwd_design == 2) return (int64_t)g.na * ((g.ny + 64 * g.strip_rpl - 1) / (64 * g.strip_rpl)) * g.nstrips;
+    return (int64_t)g.na * g.fwd_chunks * g.nseg;
+}
 
 int64_t backward_partial_count(const Geometry &g) { return backward_items(g); }
 
@@ -685,6 +938,12 @@ void geometry_free_device(Geometry &g) {
     hipFree(g.d_fwd);
     hipFree(g.d_btab);
     g.d_btab = nullptr;
+    hipFree(g.d_strip);
+    hipFree(g.d_strip_base);
+    hipFree(g.d_strip_off);
+    g.d_strip = nullptr;
+    g.d_strip_base = nullptr;
+    g.d_strip_off = nullptr;
     hipFree(g.d_err);
     hipFree(g.d_seg);
     hipFree(g.d_pixstart);
@@ -779,9 +1038,126 @@ int geometry_replan_forward(Geometry &g, int rpl, int min_groups, hipStream_t st
     return upload(&g.d_seg, g.seg, st);
 }
 
+static int build_strip_tables(Geometry &g, hipStream_t st) {
+    geometry_strip_count(g);
+    const int64_t total = g.strip_base[g.na];
+    if (total * (int64_t)sizeof(uint4) > kMaxStripTableBytes) {
+        set_error("depth-strip forward table too large for this geometry");
+        return AREC_ESIZE;
+    }
+    int rc;
+    AREC_HIP(hipMalloc((void **)&g.d_strip, sizeof(uint4) * std::max<int64_t>(1, total)));
+    if ((rc = upload(&g.d_strip_base, g.strip_base, st))) return rc;
+    AREC_HIP(hipMemsetAsync(g.d_err, 0, sizeof(int), st));
+    // angles in batches of <= 64 M entries: order on the host, weights on the device
+    std::vector<uint32_t> order;
+    uint32_t *d_order = nullptr;
+    int64_t cap = 0;
+    for (int a0 = 0; a0 < g.na;) {
+        int a1 = a0 + 1;
+        while (a1 < g.na && g.strip_base[a1 + 1] - g.strip_base[a0] <= (int64_t)(64 << 20)) a1++;
+        geometry_strip_angles(g, a0, a1, order);
+        AREC_HIP(hipStreamSynchronize(st));  // previous batch done with d_order
+        if ((int64_t)order.size() > cap) {
+            hipFree(d_order);
+            cap = (int64_t)order.size();
+            AREC_HIP(hipMalloc((void **)&d_order, sizeof(uint32_t) * std::max<int64_t>(1, cap)));
+        }
+        AREC_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(uint32_t) * order.size(), hipMemcpyHostToDevice, st));
+        int64_t most = 0;
+        for (int a = a0; a < a1; a++) most = std::max(most, g.strip_base[a + 1] - g.strip_base[a]);
+        dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((most + 255) / 256, 1024)), a1 - a0);
+        if (g.method == kSQ)
+            build_strip_table<kSQ><<<grid, 256, 0, st>>>(d_order, g.d_strip_base, a0, a1 - a0, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_strip, g.d_err);
+        else if (g.method == kKB)
+            build_strip_table<kKB><<<grid, 256, 0, st>>>(d_order, g.d_strip_base, a0, a1 - a0, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_strip, g.d_err);
+        else
+            build_strip_table<kLinear><<<grid, 256, 0, st>>>(d_order, g.d_strip_base, a0, a1 - a0, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_strip, g.d_err);
+        AREC_HIP(hipGetLastError());
+        a0 = a1;
+    }
+    AREC_HIP(hipStreamSynchronize(st));
+    hipFree(d_order);
+    if ((rc = upload(&g.d_strip_off, g.strip_off, st))) return rc;
+    AREC_HIP(hipStreamSynchronize(st));
+    int herr = 0;
+    AREC_HIP(hipMemcpy(&herr, g.d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) {
+        set_error("depth-strip table: device taps disagree with the host ordering");
+        return AREC_EGEOM;
+    }
+    return 0;
+}
+
+// Frees the depth-strip tables (device and host).
+static void release_strip_tables(Geometry &g) {
+    hipFree(g.d_strip);
+    hipFree(g.d_strip_base);
+    hipFree(g.d_strip_off);
+    g.d_strip = nullptr;
+    g.d_strip_base = nullptr;
+    g.d_strip_off = nullptr;
+    std::vector<int64_t>().swap(g.strip_base);
+    std::vector<int>().swap(g.strip_off);
+}
+
+// Selecting design 1 releases design 2's table (rebuilt when design 2 is
+// selected again).
+int geometry_set_forward(Geometry &g, int design, int rpl, int min_groups, hipStream_t st) {
+    if (design == 1) {
+        int rc = geometry_replan_forward(g, rpl, min_groups, st);
+        if (!rc) {
+            g.fwd_design = 1;
+            if (g.d_strip) {
+                AREC_HIP(hipStreamSynchronize(st));
+                release_strip_tables(g);
+            }
+        }
+        return rc;
+    }
+    if (design != 2 || !g.device_ready) {
+        set_error("forward design must be 1 (sweep) or 2 (depth strips) on an uploaded geometry");
+        return AREC_EINVAL;
+    }
+    if (rpl == 0) rpl = (g.ny >= 256 && g.ny % 4 == 0) ? 4 : (g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1;
+    if (!fwd_rpl_allowed(g.ny, rpl)) {
+        set_error("forward plan: rows per lane must be 1, 2 or 4 and divide the row count");
+        return AREC_EINVAL;
+    }
+    if (!g.d_strip) {
+        int rc = build_strip_tables(g, st);
+        if (rc) {
+            hipStreamSynchronize(st);
+            release_strip_tables(g);
+            return rc;
+        }
+    }
+    // design 1's plan keeps its own rows per lane and chunks
+    g.strip_rpl = rpl;
+    g.fwd_design = 2;
+    return 0;
+}
+
 int launch_forward(const Geometry &g, const float *x, float *p, double *partials, hipStream_t st,
                    const int *skip) {
     if (g.nvalid == 0) return fill(p, 0.0f, g.nproj, st, skip);
+    if (g.fwd_design == 2) {
+        const int rpl = g.strip_rpl, nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
+        dim3 grid((unsigned)((int64_t)((g.na + kStripAngles - 1) / kStripAngles) * g.nstrips * nchunks));
+#define AREC_FWDS(R)                                                                                       \
+    forward_strip_kernel<R><<<grid, 64 * kStripAngles, 0, st>>>(x, p, g.d_strip, g.d_strip_base, g.d_strip_off, \
+                                                              partials, g.nstrips, g.nbins, g.nx, g.ny, g.na,  \
+                                                              nchunks, skip)
+        if (rpl == 4)
+            AREC_FWDS(4);
+        else if (rpl == 2)
+            AREC_FWDS(2);
+        else
+            AREC_FWDS(1);
+#undef AREC_FWDS
+        AREC_HIP(hipGetLastError());
+        return 0;
+    }
     dim3 grid((unsigned)forward_items(g));
     const FwdArgs A = fwd_args(g, x, p, partials);
 #define AREC_FWD(R)                                                                                         \

@@ -122,7 +122,7 @@ def align_workload(args):
     t0 = time.perf_counter()
     g = A.Geometry(ctx, "sq", nx, ny, ang)
     if args.plan:
-        g.plan(*args.plan)
+        g.set_forward(*args.plan)
     t_geom = time.perf_counter() - t0
     log(f"geometry {nx}x{ny}x{na}: {t_geom:.1f} s, {g.device_bytes() / 1e9:.1f} GB of tables")
     support = 1000.0 * f
@@ -210,7 +210,7 @@ def batch_workload(args):
         ne = int(rng.integers(8, 31))
         ang = (np.arange(na) * np.pi / na).astype(np.float32)
         g = A.Geometry(ctx, "sq", nx, ny, ang)
-        autotune(ctx, g)
+        autotune(ctx, g, designs=(1,))  # the persistent kernels use the sweep's tables
         sup = 0.47 * nx
         f = nx / 128.0
         prm = P.ellipsoid_params(3000 + k, ne, sup, ny, axes=(4.0 * f, 30.0 * f), dens=(0.1, 1.0))
@@ -357,8 +357,8 @@ def main():
     ap.add_argument("--angles", type=int, default=None, help="(720; align: 1800)")
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--maxit", type=int, default=3000)
-    ap.add_argument("--plan", type=int, nargs=2, metavar=("ROWS_PER_LANE", "MIN_GROUPS"),
-                    help="forward projector plan (default: autotuned)")
+    ap.add_argument("--plan", type=int, nargs=3, metavar=("DESIGN", "ROWS_PER_LANE", "MIN_GROUPS"),
+                    help="forward projector design (1 sweep, 2 depth strips) and plan (default: autotuned)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=None, help="plain SIRT solves (default: --steps)")
     ap.add_argument("--plain-only", action="store_true", help="time plain SIRT only (no network)")
@@ -407,15 +407,15 @@ def main():
     g = A.Geometry(ctx, "sq", nx, ny, ang)
     t_geom = time.perf_counter() - t0
     if args.plan:
-        g.plan(*args.plan)
-        plan = g.get_plan()[:2]
-    elif args.no_autotune:
-        plan = g.get_plan()[:2]
-    else:
+        g.set_forward(*args.plan)
+    elif not args.no_autotune:
         from arec3d_amd.autotune import autotune
 
-        plan = autotune(ctx, g, log=lambda m: log("autotune " + m))[:2]
-    log(f"forward plan: {plan[0]} rows/lane, >= {plan[1]} workgroups, {g.get_plan()[2]} segments")
+        autotune(ctx, g, log=lambda m: log("autotune " + m))
+    plan = [g.forward_design, *g.get_plan()[:2]]
+    if plan[0] == 2:
+        plan[2] = 0
+    log(f"forward design {plan[0]}: {plan[1]} rows/lane, >= {plan[2]} workgroups")
     support = 250.0 * nx / 512
     x_true = ctx.empty(g.vol_shape)
     prm = P.ellipsoid_params(49000, 40, support, ny, axes=(8.0 * nx / 512, 120.0 * nx / 512), dens=(0.05, 1.0))
@@ -486,7 +486,8 @@ def main():
     p_tmp = ctx.empty(g.proj_shape)
     ms_f = g.time_operator(0, x, p_tmp, 5)
     ms_b = g.time_operator(1, p_tmp, x, 5)
-    dom, ms = ("forward_kernel", ms_f) if ms_f >= ms_b else ("backward_kernel", ms_b)
+    fwd_name = "forward_strip_kernel" if plan[0] == 2 else "forward_kernel"
+    dom, ms = (fwd_name, ms_f) if ms_f >= ms_b else ("backward_kernel", ms_b)
     achieved = B / (ms * 1e-3) / 1e9
     log(f"forward {ms_f:.2f} ms ({vu / ms_f / 1e6:.1f} GVU/s), back {ms_b:.2f} ms ({vu / ms_b / 1e6:.1f} GVU/s)")
 
@@ -503,7 +504,8 @@ def main():
         "plain_final_relnrm": final[2],
         "ms_per_iteration": round(t_plain * 1e3 / max(1, n_it), 3),
         "lam": lam,
-        "forward_plan": {"rows_per_lane": plan[0], "min_groups": plan[1], "segments": g.get_plan()[2]},
+        "forward_plan": {"design": {1: "detector sweep", 2: "depth strips"}[plan[0]], "rows_per_lane": plan[1],
+                         "min_groups": plan[2]},
         "geometry_setup_s": round(t_geom, 3),
         "forward_ms": round(ms_f, 3),
         "backward_ms": round(ms_b, 3),

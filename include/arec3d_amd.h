@@ -96,6 +96,16 @@ int64_t arec_geom_device_bytes(const arec_geom *geom);
  * Python package picks it by timing. */
 int arec_geom_plan(arec_ctx *ctx, arec_geom *geom, int32_t rows_per_lane, int32_t min_groups);
 int arec_geom_get_plan(const arec_geom *geom, int32_t *rows_per_lane, int32_t *min_groups, int32_t *nseg);
+/* Forward projector design (no counterpart in the reference): 1 = detector
+ * sweep (the default; arec_geom_plan sets its plan), 2 = depth strips (8-pixel
+ * detector strips walked in depth order by 4 neighbouring angles in
+ * lockstep; its table, ~1.25 x the sweep table, is built when design 2 is
+ * selected, refused above 32 GiB, and released when design 1 is selected).
+ * Results agree to FP32 summation order.  arec_geom_get_plan reports the
+ * selected design's rows per lane. */
+int arec_geom_set_forward(arec_ctx *ctx, arec_geom *geom, int32_t design, int32_t rows_per_lane,
+                          int32_t min_groups);
+int32_t arec_geom_forward_design(const arec_geom *geom);
 
 /* ---- device-resident operators --------------------------------------- */
 /* p = A x   (arecProject2D / _KB / _SQ, arecproject.cpp:19-99,351-437,540-648) */

@@ -176,3 +176,31 @@ def test_back_tap_table_and_computed_taps_agree(ctx, method, monkeypatch):
     g1.backproject(p, x1)
     g2.backproject(p, x2)
     assert np.array_equal(x1.download(), x2.download())
+
+
+@pytest.mark.parametrize("method", ["sq", "kb", "linear"])
+@pytest.mark.parametrize("nx,ny,na,rpl", [(33, 5, 17, 1), (64, 70, 45, 2), (129, 132, 90, 4), (256, 3, 7, 1)])
+def test_forward_depth_strips_vs_oracle(ctx, method, nx, ny, na, rpl):
+    """Forward design 2 (depth strips): the same parity bar as the sweep
+    against the oracle, and the sweep's projections to FP32 summation order."""
+    import arec3d_amd as A
+
+    print(f"\n[case] depth strips {method} nx={nx} ny={ny} na={na} rpl={rpl}", flush=True)
+    rng = np.random.default_rng(nx * 7 + ny + na)
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    g = A.Geometry(ctx, method, nx, ny, ang)
+    x = rng.random(g.vol_shape).astype(np.float32)
+    if method == "linear":
+        x[-2:] = 0.0
+    dx, dp = ctx.to_device(x), ctx.empty(g.proj_shape)
+    g.project(dx, dp)
+    sweep = dp.download()
+    g.set_forward(2, rpl)
+    assert g.forward_design == 2
+    g.project(dx, dp)
+    strips = dp.download()
+    op = O.project(method, x, ang, nx, ny)
+    assert rel_views(strips, op).max() <= TOL
+    assert rel_views(strips, sweep).max() <= TOL
+    g.set_forward(1)
+    assert g.forward_design == 1

@@ -215,3 +215,64 @@ def test_sirt_config1_shape_rows_vs_oracle(ctx):
     st, xo, ho, lo = O.sirt("sq", bsub, ang, lam, 50, 0.0)
     assert st == 0 and all(h[1] == 0 for h in ho)
     assert relnorm(x.download()[:, r0:r0 + 4], xo) <= 1e-4
+
+
+@pytest.mark.parametrize("method", ["sq", "kb", "linear"])
+def test_solvers_with_depth_strip_forward_vs_oracle(ctx, method):
+    """SIRT and CGLS with forward design 2 (depth strips, whose kernel also
+    delivers the residual-norm partials): the same bar against the oracle
+    as with the sweep."""
+    import arec3d_amd as A
+
+    nx, ny, na = 65, 40, 60
+    print(f"\n[case] solvers with depth strips {method}", flush=True)
+    b, ang = problem(method, nx, ny, na, 1)
+    lam = 1.0 / (na * nx)
+    _, xo, ho, lo = O.sirt(method, b, ang, lam, 30, 1e-6)
+    _, _, ho64, _ = O.sirt(method, b, ang, lam, 30, 1e-6, dtype=np.float64)
+    g = A.Geometry(ctx, method, nx, ny, ang)
+    g.set_forward(2, 0)
+    assert g.forward_design == 2
+    db, dx = ctx.to_device(b), ctx.empty(g.vol_shape)
+    hg, lg = g.sirt(db, dx, lam, 30, 1e-6)
+    check_hist(hg, ho, ho64)
+    assert lg == lo
+    assert relnorm(dx.download(), xo) <= 1e-4
+    _, xc, hc = O.cgls(method, b, ang, 15, 1e-6)
+    _, xc64, hc64 = O.cgls(method, b, ang, 15, 1e-6, dtype=np.float64)
+    dx = ctx.empty(g.vol_shape)
+    hg = g.cgls(db, dx, 15, 1e-6)
+    check_hist(hg, hc, hc64)
+    assert relnorm(dx.download(), xc) <= max(1e-4, 4 * relnorm(xc, xc64))
+
+
+def test_autotune_designs(ctx, tmp_path):
+    """The autotuner times both forward designs, sets the fastest, caches it
+    and applies the cached plan on the next geometry of the same shape;
+    selecting design 1 releases the depth-strip table."""
+    import arec3d_amd as A
+    from arec3d_amd.autotune import autotune, candidates
+
+    print("\n[case] autotune designs", flush=True)
+    nx, ny, na = 96, 64, 90
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    g = A.Geometry(ctx, "sq", nx, ny, ang)
+    g.set_forward(1, 4, 0)
+    base = g.device_bytes()
+    assert {c[0] for c in candidates(g)} == {1, 2}
+    cache = str(tmp_path / "plans.json")
+    d, r, m, ms = autotune(ctx, g, cache=cache, reps=1)
+    assert g.forward_design == d and g.get_plan()[0] == r and ms > 0
+    g2 = A.Geometry(ctx, "sq", nx, ny, ang)
+    assert autotune(ctx, g2, cache=cache)[:3] == (d, r, m)
+    assert g2.forward_design == d
+    g2.set_forward(2, 4)
+    assert g2.device_bytes() > base
+    g2.set_forward(1, 4, 0)
+    assert g2.device_bytes() == base
+    rng = np.random.default_rng(3)
+    x = rng.random(g.vol_shape).astype(np.float32)
+    p = ctx.empty(g.proj_shape)
+    g.project(ctx.to_device(x), p)
+    op = O.project("sq", x, ang, nx, ny)
+    assert np.abs(p.download() - op).max() <= 1e-5 * np.abs(op).max()
