@@ -84,6 +84,8 @@ def _load():
         "arec_geom_set_forward": ([vp, vp, i32, i32, i32], C.c_int),
         "arec_geom_forward_design": ([vp], i32),
         "arec_match": ([vp, vp, vp, vp, flt, i32, i32, flt, i32, i32, vp, vp, P(Hist)], C.c_int),
+        "arec_match_ex": ([vp, vp, vp, vp, flt, i32, i32, flt, i32, i32, i32, vp, vp, P(Hist)], C.c_int),
+        "arec_align_centroid": ([vp, vp, vp, vp], C.c_int),
         "arec_geom_get_plan": ([vp, P(i32), P(i32), P(i32)], C.c_int),
         "arec_unet_create": ([vp, P(vp), i32, i32, i32, i32, C.c_uint64], C.c_int),
         "arec_unet_destroy": ([vp], C.c_int),
@@ -122,7 +124,7 @@ EXPORTED = [
     "arec_unet_destroy", "arec_unet_nparams", "arec_unet_get_params", "arec_unet_set_params",
     "arec_unet_set_scale", "arec_unet_forward", "arec_unet_train_step", "arec_unet_gradient",
     "arec_conv3_check", "arec_conv3_wgrad_check", "arec_cyl_to_cube", "arec_cube_to_cyl", "arec_h_project", "arec_h_backproject",
-    "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan", "arec_match", "arec_scale", "arec_geom_set_forward",
+    "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan", "arec_match", "arec_match_ex", "arec_align_centroid", "arec_scale", "arec_geom_set_forward",
     "arec_geom_forward_design",
 ]
 
@@ -360,15 +362,26 @@ def sirt_batch(ctx, geoms, bs, xs, lams, maxit, tol, hist_cap=None):
     return out
 
 
-def match(ctx, g, views, x, lam, outer=5, maxit=20, tol=0.0, cc_passes=3, method="sirt"):
-    """Projection matching (arec_match): returns (sx, sy, last history line
-    of each outer loop's reconstruction)."""
+def match(ctx, g, views, x, lam, outer=5, maxit=20, tol=0.0, cc_passes=3, method="sirt", x_from="cc"):
+    """Projection matching (arec_match / arec_match_ex): returns (sx, sy,
+    last history line of each outer loop's reconstruction).  x_from="cc"
+    takes every shift from the cross-correlation passes (the reference's
+    loop); x_from="centroid" takes the x shifts from the views' centroids."""
     sx = np.zeros(g.na, np.float32)
     sy = np.zeros(g.na, np.float32)
     H = (Hist * max(1, outer))()
     m = {"sirt": 1, "cgls": 2, "sirt_warm": 3}[method]
-    _check(lib.arec_match(ctx.h, g.h, views.ptr, x.ptr, lam, outer, maxit, tol, cc_passes, m, _ptr(sx), _ptr(sy), H))
+    flags = {"cc": 0, "centroid": 1}[x_from]
+    _check(lib.arec_match_ex(ctx.h, g.h, views.ptr, x.ptr, lam, outer, maxit, tol, cc_passes, m, flags, _ptr(sx),
+                             _ptr(sy), H))
     return sx, sy, _hist(H, outer)
+
+
+def align_centroid(ctx, g, views):
+    """x shift of every view from its centroid (arec_align_centroid)."""
+    sx = np.zeros(g.na, np.float32)
+    _check(lib.arec_align_centroid(ctx.h, g.h, views.ptr, _ptr(sx)))
+    return sx
 
 
 def align_ccorr(ctx, x, y, c):

@@ -416,6 +416,46 @@ int shift_views(float *data, int nx, int ny, int nv, const float *sx, const floa
     return rc;
 }
 
+// Zeroth and first moments of every view of a stack [nv][ny][nx], in FP64:
+// mom[3v..3v+2] = {sum p, sum u*p, sum y*p} over the view's pixels (u, y).
+// One workgroup per view; each thread sums a fixed set of columns of every
+// row, then a fixed-order tree: the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void view_moments_kernel(const float *__restrict__ views, int nx, int ny,
+                                                           double *__restrict__ mom) {
+    __shared__ double red[3][256];
+    const int v = blockIdx.x, tid = threadIdx.x;
+    const float *__restrict__ p = views + (int64_t)v * nx * ny;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int y = 0; y < ny; y++) {
+        const float *__restrict__ row = p + (int64_t)y * nx;
+        double r0 = 0.0, r1 = 0.0;
+        for (int u = tid; u < nx; u += 256) {
+            const double q = row[u];
+            r0 += q;
+            r1 += q * u;
+        }
+        s0 += r0;
+        s1 += r1;
+        s2 += r0 * y;
+    }
+    red[0][tid] = s0;
+    red[1][tid] = s1;
+    red[2][tid] = s2;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h)
+            for (int k = 0; k < 3; k++) red[k][tid] += red[k][tid + h];
+        __syncthreads();
+    }
+    if (tid < 3) mom[3 * (int64_t)v + tid] = red[tid][0];
+}
+
+int view_moments(const float *views, int nx, int ny, int nv, double *mom_dev, hipStream_t st) {
+    view_moments_kernel<<<nv, 256, 0, st>>>(views, nx, ny, mom_dev);
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
 }  // namespace arec
 
 using namespace arec;

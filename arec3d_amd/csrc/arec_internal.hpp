@@ -223,6 +223,16 @@ int cc_views(const float *proj, const float *meas, int nx, int ny, int nv, float
              float *sy, hipStream_t st);
 int shift_views(float *data, int nx, int ny, int nv, const float *sx, const float *sy, float sgn,
                 hipStream_t st);
+// Per-view moments {sum p, sum u p, sum y p} (FP64, deterministic) into
+// mom_dev[3 * nv].
+int view_moments(const float *views, int nx, int ny, int nv, double *mom_dev, hipStream_t st);
+// Projection matching (matching.cpp).  flags bit 0: x shifts from the views'
+// centroids (centroid_x_shifts) instead of cross-correlation.
+constexpr int kMatchXCentroid = 1;
+int centroid_x_shifts(const Geometry &g, hipStream_t st, const float *views, float *sx_host);
+int match_run(const Geometry &g, hipStream_t st, const float *views0, float *x, float lam, int outer,
+              int maxit, float tol, int cc_passes, int method, int flags, float *sx_host, float *sy_host,
+              arec_hist *loop_hist);
 
 // Fill with a constant by a kernel (graph-capture safe); skip as above.
 int fill(float *x, float v, int64_t n, hipStream_t st, const int *skip);

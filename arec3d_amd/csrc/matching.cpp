@@ -13,6 +13,19 @@
 // preceded by the initial reconstruction from views0 (:140-165).  The
 // reference moves views and slabs between these steps through MRC files
 // (:197-202,249,293); here they never leave the device.
+//
+// Option (no counterpart in the reference): x shifts from the views'
+// centroids.  For parallel-beam views of an object inside the field of
+// view, a view's centroid along the detector is the object's centroid
+// projected at the view's angle -- a cos(t) + b sin(t) about the rotation
+// axis -- plus the view's own x shift, so the shifts follow from one pass
+// over the views, without the reconstruction's part-absorption of in-plane
+// misalignment that slows the reprojection loop in x.  The axis position is
+// the centroid of the projected uniform cylinder (symmetric about the axis
+// under every method's detector convention); a and b are not observable and
+// are removed by least squares, as any alignment leaves them free.  y shifts
+// still come from the cross-correlation passes (the y centroid is not used:
+// objects that fill the detector rows break it).
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -68,36 +81,99 @@ struct Progress {
 
 }  // namespace
 
+int centroid_x_shifts(const Geometry &g, hipStream_t st, const float *views, float *sx_host) {
+    const int na = g.na;
+    Dev ones, prj;
+    double *mom = nullptr;
+    int rc;
+    if ((rc = dalloc(ones, g.nnz)) || (rc = dalloc(prj, g.nproj))) return rc;
+    AREC_HIP(hipMalloc((void **)&mom, sizeof(double) * 6 * (size_t)na));
+    std::vector<double> m(6 * (size_t)na);
+    rc = fill(ones.p, 1.0f, g.nnz, st, nullptr);
+    if (!rc) rc = launch_forward(g, ones.p, prj.p, nullptr, st, nullptr);
+    if (!rc) rc = view_moments(prj.p, g.nx, g.ny, na, mom, st);
+    if (!rc) rc = view_moments(views, g.nx, g.ny, na, mom + 3 * (size_t)na, st);
+    if (!rc && hipMemcpyAsync(m.data(), mom, sizeof(double) * m.size(), hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = hip_fail(hipGetLastError(), "centroid_x_shifts");
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = hip_fail(hipGetLastError(), "centroid_x_shifts");
+    hipFree(mom);
+    if (rc) return rc;
+    // d_a = view centroid - axis; fit d ~ a cos + b sin (2x2 normal equations)
+    std::vector<double> d(na);
+    double scc = 0, sss = 0, scs = 0, scd = 0, ssd = 0;
+    for (int a = 0; a < na; a++) {
+        const double *ax = &m[3 * (size_t)a], *vw = &m[3 * ((size_t)na + a)];
+        if (!(ax[0] > 0.0) || !(vw[0] > 0.0)) {
+            set_error("centroid x shifts: a view (or the projected cylinder) has no positive mass");
+            return AREC_EINVAL;
+        }
+        d[a] = vw[1] / vw[0] - ax[1] / ax[0];
+        const double c = g.ang[a].c, s = g.ang[a].s;
+        scc += c * c;
+        sss += s * s;
+        scs += c * s;
+        scd += c * d[a];
+        ssd += s * d[a];
+    }
+    const double det = scc * sss - scs * scs;
+    const double ca = det > 0.0 ? (scd * sss - ssd * scs) / det : 0.0;
+    const double cb = det > 0.0 ? (ssd * scc - scd * scs) / det : 0.0;
+    for (int a = 0; a < na; a++) sx_host[a] = (float)(d[a] - ca * g.ang[a].c - cb * g.ang[a].s);
+    return 0;
+}
+
 int match_run(const Geometry &g, hipStream_t st, const float *views0, float *x, float lam, int outer,
-              int maxit, float tol, int cc_passes, int method, float *sx_host, float *sy_host,
+              int maxit, float tol, int cc_passes, int method, int flags, float *sx_host, float *sy_host,
               arec_hist *loop_hist) {
     const int64_t nproj = g.nproj, nnz = g.nnz;
     const int nx = g.nx, ny = g.ny, na = g.na;
+    const bool xc = (flags & kMatchXCentroid) != 0;
     Dev aligned, prj, s, t;
     int rc;
     if ((rc = dalloc(aligned, nproj)) || (rc = dalloc(prj, nproj)) || (rc = dalloc(s, 2 * (size_t)na)) ||
         (rc = dalloc(t, 2 * (size_t)na)))
         return rc;
-    std::vector<float> hs(2 * (size_t)na), ht(2 * (size_t)na);
+    std::vector<float> hs(2 * (size_t)na), ht(2 * (size_t)na), cx(na, 0.0f);
     const Progress progress;
-    // initial reconstruction from the unshifted views
-    if ((rc = reconstruct(g, st, method, views0, x, lam, maxit, tol, true, nullptr))) return rc;
+    // views0 shifted by the host shifts h (x from h[0..na), y from h[na..2na))
+    auto shifted_views = [&](const std::vector<float> &h) -> int {
+        int r;
+        AREC_HIP(hipMemcpyAsync(s.p, h.data(), sizeof(float) * 2 * na, hipMemcpyHostToDevice, st));
+        if ((r = copy(aligned.p, views0, nproj, st))) return r;
+        return shift_views(aligned.p, nx, ny, na, s.p, s.p + na, -1.0f, st);
+    };
+    // initial reconstruction from the unshifted views (or, with centroid x
+    // shifts, from the views shifted in x)
+    if (xc) {
+        if ((rc = centroid_x_shifts(g, st, views0, cx.data()))) return rc;
+        std::copy(cx.begin(), cx.end(), hs.begin());
+        if ((rc = shifted_views(hs))) return rc;
+        progress("centroid x shifts done", -1);
+    }
+    if ((rc = reconstruct(g, st, method, xc ? aligned.p : views0, x, lam, maxit, tol, true, nullptr))) return rc;
     progress("initial reconstruction done", -1);
     for (int it = 0; it < outer; it++) {
         if ((rc = launch_forward(g, x, prj.p, nullptr, st, nullptr))) return rc;
-        if ((rc = copy(aligned.p, views0, nproj, st))) return rc;
         std::fill(hs.begin(), hs.end(), 0.0f);
+        if (xc) {
+            std::copy(cx.begin(), cx.end(), hs.begin());
+            if ((rc = shifted_views(hs))) return rc;
+        } else if ((rc = copy(aligned.p, views0, nproj, st))) {
+            return rc;
+        }
         for (int pass = 0; pass < cc_passes; pass++) {
             if ((rc = cc_views(prj.p, aligned.p, nx, ny, na, nullptr, t.p, t.p + na, st))) return rc;
-            if ((rc = shift_views(aligned.p, nx, ny, na, t.p, t.p + na, -1.0f, st))) return rc;
             AREC_HIP(hipMemcpyAsync(ht.data(), t.p, sizeof(float) * 2 * na, hipMemcpyDeviceToHost, st));
             AREC_HIP(hipStreamSynchronize(st));
+            if (xc) {  // x stays at the centroid shifts
+                std::fill(ht.begin(), ht.begin() + na, 0.0f);
+                AREC_HIP(hipMemcpyAsync(t.p, ht.data(), sizeof(float) * na, hipMemcpyHostToDevice, st));
+            }
+            if ((rc = shift_views(aligned.p, nx, ny, na, t.p, t.p + na, -1.0f, st))) return rc;
             for (size_t i = 0; i < hs.size(); i++) hs[i] += ht[i];  // FP32 sums (:220-223)
         }
         // the fresh views shifted by the loop's total
-        AREC_HIP(hipMemcpyAsync(s.p, hs.data(), sizeof(float) * 2 * na, hipMemcpyHostToDevice, st));
-        if ((rc = copy(aligned.p, views0, nproj, st))) return rc;
-        if ((rc = shift_views(aligned.p, nx, ny, na, s.p, s.p + na, -1.0f, st))) return rc;
+        if ((rc = shifted_views(hs))) return rc;
         progress("reprojection and alignment done", it);
         arec_hist last;
         if ((rc = reconstruct(g, st, method, aligned.p, x, lam, maxit, tol, false, &last))) return rc;
@@ -115,14 +191,29 @@ int match_run(const Geometry &g, hipStream_t st, const float *views0, float *x, 
 
 using namespace arec;
 
+extern "C" int arec_match_ex(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *x_dev,
+                             float lam, int32_t outer, int32_t maxit, float tol, int32_t cc_passes,
+                             int32_t method, int32_t flags, float *sx, float *sy, arec_hist *loop_hist) {
+    if (!ctx || !geom || !views_dev || !x_dev || outer < 0 || maxit < 1 || cc_passes < 0 || method < 1 ||
+        method > 3 || (flags & ~kMatchXCentroid)) {
+        set_error("arec_match: bad argument (method 1 = SIRT, 2 = CGLS, 3 = SIRT continued; flags 0 or 1)");
+        return AREC_EINVAL;
+    }
+    return match_run(geom->g, ctx->stream, views_dev, x_dev, lam, outer, maxit, tol, cc_passes, method, flags, sx,
+                     sy, loop_hist);
+}
+
 extern "C" int arec_match(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *x_dev, float lam,
                           int32_t outer, int32_t maxit, float tol, int32_t cc_passes, int32_t method,
                           float *sx, float *sy, arec_hist *loop_hist) {
-    if (!ctx || !geom || !views_dev || !x_dev || outer < 0 || maxit < 1 || cc_passes < 0 || method < 1 ||
-        method > 3) {
-        set_error("arec_match: bad argument (method 1 = SIRT, 2 = CGLS, 3 = SIRT continued)");
+    return arec_match_ex(ctx, geom, views_dev, x_dev, lam, outer, maxit, tol, cc_passes, method, 0, sx, sy,
+                         loop_hist);
+}
+
+extern "C" int arec_align_centroid(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *sx) {
+    if (!ctx || !geom || !views_dev || !sx) {
+        set_error("arec_align_centroid: NULL argument");
         return AREC_EINVAL;
     }
-    return match_run(geom->g, ctx->stream, views_dev, x_dev, lam, outer, maxit, tol, cc_passes, method, sx, sy,
-                     loop_hist);
+    return centroid_x_shifts(geom->g, ctx->stream, views_dev, sx);
 }

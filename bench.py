@@ -147,7 +147,7 @@ def align_workload(args):
     ctx.synchronize()
     t0 = time.perf_counter()
     sx, sy, hl = A.match(ctx, g, b, x, lam, outer=args.outer, maxit=args.sirt_iters, cc_passes=args.cc_passes,
-                         method=args.recon)
+                         method=args.recon, x_from=args.x_from)
     ctx.synchronize()
     dt = time.perf_counter() - t0
     ex, ey = (sx - dx).astype(np.float64), (sy - dy).astype(np.float64)
@@ -165,10 +165,12 @@ def align_workload(args):
         "data": "synthetic (seeded ellipsoid phantom, seeded shifts U[-5,5] px, 1% Gaussian noise)",
         "config": {
             "workload": f"{args.outer} outer loops x ({args.sirt_iters} {args.recon} iterations + {args.cc_passes} "
-                        f"CC/shift passes), "
+                        f"CC/shift passes), x shifts from {'centroids' if args.x_from == 'centroid' else 'CC'}, "
                         f"{nx}x{nx}x{ny} volume, {na} angles, detector {nx}x{ny}",
             "ellipsoids": args.ellipsoids,
             "reconstruction": args.recon,
+            "x_shifts_from": {"cc": "cross-correlation passes (the reference's loop)",
+                              "centroid": "view centroids (y from cross-correlation passes)"}[args.x_from],
             "max_abs_error_x_px": float(np.abs(gx).max()),
             "max_abs_error_y_px": float(np.abs(gy).max()),
             "rms_error_x_px": float(np.sqrt(np.mean(gx ** 2))),
@@ -377,6 +379,8 @@ def main():
     ap.add_argument("--cc-passes", type=int, default=3, help="align workload: CC/shift passes per outer loop")
     ap.add_argument("--recon", choices=["sirt", "cgls", "sirt_warm"], default="sirt",
                     help="align workload: reconstruction of each outer loop")
+    ap.add_argument("--x-from", choices=["cc", "centroid"], default="centroid",
+                    help="align workload: x shifts from the view centroids (default) or the CC passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); several GPUs would each "

@@ -216,6 +216,21 @@ int arec_align_shift(arec_ctx *ctx, float *data_dev, int32_t nx, int32_t ny, int
 int arec_match(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *x_dev, float lam,
                int32_t outer, int32_t maxit, float tol, int32_t cc_passes, int32_t method, float *sx,
                float *sy, arec_hist *loop_hist);
+/* arec_match with flags (no counterpart in the reference): flags bit 0
+ * (AREC_MATCH_X_CENTROID) takes the x shifts from the views' centroids
+ * (arec_align_centroid, once, before the initial reconstruction) and only
+ * the y shifts from the cross-correlation passes.  flags 0 is arec_match. */
+#define AREC_MATCH_X_CENTROID 1
+int arec_match_ex(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *x_dev, float lam,
+                  int32_t outer, int32_t maxit, float tol, int32_t cc_passes, int32_t method, int32_t flags,
+                  float *sx, float *sy, arec_hist *loop_hist);
+/* x shift of every view from its centroid (no counterpart in the reference;
+ * parallel beam, object inside the field of view): the view's centroid
+ * along the detector minus the rotation axis (the centroid of the projected
+ * uniform cylinder), with the unobservable a cos(t) + b sin(t) (a
+ * translation of the object in the plane) removed by least squares.  sx:
+ * host, na floats.  AREC_EINVAL if a view has no positive mass. */
+int arec_align_centroid(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *sx);
 
 /* ---- synthetic inputs (no counterpart in the reference) --------------- */
 /* Sum of ellipsoids at the voxel centres of the compact cylinder of the

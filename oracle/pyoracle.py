@@ -219,6 +219,27 @@ def shift_images(stack, sx, sy):
     return out
 
 
+def centroid_x_shifts(method, views, angles, nx, ny):
+    """x shift of every view from its centroid, restated in FP64 numpy (no
+    counterpart in the reference; the device entry is arec_align_centroid):
+    d = view centroid along the detector - rotation axis (the centroid of
+    the projected uniform cylinder), minus its least-squares fit by
+    a cos(t) + b sin(t).  views: [na, ny, nx]."""
+    angles = np.asarray(angles, np.float32)
+    cord = cyl_geometry(0.5 * (nx - 1))
+    axis = project(method, np.ones((len(cord), ny), np.float64), angles, nx, ny, dtype=np.float64)
+    u = np.arange(nx, dtype=np.float64)
+
+    def centroid(v):
+        v = np.asarray(v, np.float64)
+        return (v.sum(axis=1) * u).sum(axis=1) / v.sum(axis=(1, 2))
+
+    d = centroid(views) - centroid(axis)
+    t = angles.astype(np.float64)
+    M = np.stack([np.cos(t), np.sin(t)], 1)
+    return d - M @ np.linalg.lstsq(M, d, rcond=None)[0]
+
+
 def soft_noneg(x):
     out = np.array(x, np.float32, copy=True)
     lib().orc_soft_noneg(out.reshape(-1), out.size)

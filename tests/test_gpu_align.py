@@ -115,3 +115,27 @@ def test_host_dropin_cc_shift(ctx):
     assert np.abs(i2.data - proj).max() < 0.05
     bad = A.ArecImage(meas[:2].copy(), nx, ny, 2)
     assert A.arecCCImages(None, i1, bad, sx, sy) == -1
+
+
+@pytest.mark.parametrize("method,nx,ny,na", [("sq", 64, 9, 45), ("kb", 33, 5, 30), ("sq", 200, 3, 7)])
+def test_centroid_x_shifts_vs_oracle(ctx, method, nx, ny, na):
+    """arec_align_centroid (FP64 device moments, host fit) against the FP64
+    numpy restatement on the same seeded, Fourier-shifted views."""
+    import arec3d_amd as A
+
+    print(f"\n[case] centroid x shifts {method} {nx}x{ny}x{na}", flush=True)
+    rng = np.random.default_rng(nx + ny + na)
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    g = A.Geometry(ctx, method, nx, ny, ang)
+    r = 0.5 * (nx - 1)
+    cord = O.cyl_geometry(r)
+    inner = ((cord[:, 0] - r) ** 2 + (cord[:, 1] - r) ** 2) < (0.6 * r) ** 2
+    x = (rng.random(g.vol_shape) * inner[:, None]).astype(np.float32)
+    p = ctx.empty(g.proj_shape)
+    g.project(ctx.to_device(x), p)
+    dx = rng.uniform(-2, 2, na).astype(np.float32)
+    A.align_shift(ctx, p, ctx.to_device(dx), ctx.to_device(np.zeros(na, np.float32)), sign=1.0)
+    views = p.download()
+    s = A.align_centroid(ctx, g, p)
+    so = O.centroid_x_shifts(method, views, ang, nx, ny)
+    np.testing.assert_allclose(s, so, atol=1e-4)  # FP32 device projection of the axis cylinder

@@ -358,3 +358,49 @@ def test_cc_images_recovers_shift():
 def test_soft_noneg():
     x = np.array([-2.0, 0.0, 3.0, -0.5], np.float32)
     np.testing.assert_array_equal(O.soft_noneg(x), [-1.0, 0.0, 3.0, -0.25])
+
+
+def _inner_object(method, nx, ny, ang, seed):
+    """Views of a random non-negative object confined to 0.6 of the radius
+    (zero margins on the detector), in FP64."""
+    rng = np.random.default_rng(seed)
+    r = 0.5 * (nx - 1)
+    cord = O.cyl_geometry(r)
+    inner = ((cord[:, 0] - r) ** 2 + (cord[:, 1] - r) ** 2) < (0.6 * r) ** 2
+    x = rng.random((len(cord), ny)) * inner[:, None]
+    return O.project(method, x, ang, nx, ny, dtype=np.float64)
+
+
+@pytest.mark.parametrize("method", ["sq", "kb"])
+def test_centroid_unshifted_views_give_zero(method):
+    """A parallel-beam view's first moment is the object's projected
+    centroid, a cos(t) + b sin(t) about the axis, so unshifted views give
+    zero shifts -- up to the footprints' binning: a footprint whose width is
+    not a whole number of pixels, sampled on the pixel grid, keeps its mass
+    but not exactly its first moment (~1e-3 px here).  Pins the axis
+    position (half a pixel off leaves ~0.3 px) and the removed gauge."""
+    nx, ny, na = 41, 3, 23
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    views = _inner_object(method, nx, ny, ang, 1)
+    s = O.centroid_x_shifts(method, views, ang, nx, ny)
+    assert np.abs(s).max() < 5e-3
+
+
+def test_centroid_recovers_integer_shifts():
+    """Integer shifts (np.roll over zero margins moves the centroid exactly)
+    come back up to the unobservable a cos(t) + b sin(t) -- and with the
+    right sign and axis (x, not y)."""
+    nx, ny, na = 41, 4, 31
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    views = _inner_object("sq", nx, ny, ang, 2)
+    d = np.random.default_rng(3).integers(-4, 5, na)
+    shifted = np.stack([np.roll(views[a], d[a], axis=1) for a in range(na)])
+    s0 = O.centroid_x_shifts("sq", views, ang, nx, ny)
+    s = O.centroid_x_shifts("sq", shifted, ang, nx, ny)
+    t = ang.astype(np.float64)
+    M = np.stack([np.cos(t), np.sin(t)], 1)
+    want = d - M @ np.linalg.lstsq(M, d.astype(np.float64), rcond=None)[0]
+    np.testing.assert_allclose(s - s0, want, atol=1e-9)
+    # a y shift does not move the x centroid
+    rolled_y = np.stack([np.roll(views[a], 1, axis=0) for a in range(na)])
+    np.testing.assert_allclose(O.centroid_x_shifts("sq", rolled_y, ang, nx, ny), s0, atol=1e-9)
