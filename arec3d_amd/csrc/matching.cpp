@@ -128,12 +128,15 @@ int match_run(const Geometry &g, hipStream_t st, const float *views0, float *x, 
     const int64_t nproj = g.nproj, nnz = g.nnz;
     const int nx = g.nx, ny = g.ny, na = g.na;
     const bool xc = (flags & kMatchXCentroid) != 0;
-    Dev aligned, prj, s, t;
+    std::vector<float> hs(2 * (size_t)na), ht(2 * (size_t)na), cx(na, 0.0f);
     int rc;
+    // centroid x shifts first: their scratch (a volume and a view stack) is
+    // freed before the loop's buffers are allocated
+    if (xc && (rc = centroid_x_shifts(g, st, views0, cx.data()))) return rc;
+    Dev aligned, prj, s, t;
     if ((rc = dalloc(aligned, nproj)) || (rc = dalloc(prj, nproj)) || (rc = dalloc(s, 2 * (size_t)na)) ||
         (rc = dalloc(t, 2 * (size_t)na)))
         return rc;
-    std::vector<float> hs(2 * (size_t)na), ht(2 * (size_t)na), cx(na, 0.0f);
     const Progress progress;
     // views0 shifted by the host shifts h (x from h[0..na), y from h[na..2na))
     auto shifted_views = [&](const std::vector<float> &h) -> int {
@@ -145,7 +148,6 @@ int match_run(const Geometry &g, hipStream_t st, const float *views0, float *x, 
     // initial reconstruction from the unshifted views (or, with centroid x
     // shifts, from the views shifted in x)
     if (xc) {
-        if ((rc = centroid_x_shifts(g, st, views0, cx.data()))) return rc;
         std::copy(cx.begin(), cx.end(), hs.begin());
         if ((rc = shifted_views(hs))) return rc;
         progress("centroid x shifts done", -1);
