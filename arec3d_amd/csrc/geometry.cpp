@@ -94,8 +94,11 @@ int geometry_build_host(Geometry &g, int method, int nx, int ny, double radius,
         g.coord = g.cord;
     }
 
-    // Back-projection tiles: columns bucketed by 8x8 boxes of their
-    // coordinates, buckets split into tiles of at most 64 columns.
+    // Back-projection tiles: columns bucketed by kTileBox x kTileBox boxes of
+    // their coordinates (in column order), buckets split into tiles of at
+    // most kTileCols columns.  Coordinates need not be distinct (the linear
+    // pair's shifted coordinates repeat), so slots are filled in order, not
+    // by position.
     std::map<std::pair<int, int>, std::vector<int>> boxes;
     for (int j = 0; j < g.nvalid; j++)
         boxes[{g.coord[2 * j + 1] / kTileBox, g.coord[2 * j] / kTileBox}].push_back(j);

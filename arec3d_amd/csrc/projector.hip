@@ -665,7 +665,8 @@ __global__ __launch_bounds__(64 * kFwdAngles) void forward_batch_kernel(const Ba
 // ---------------------------------------------------------------------------
 constexpr int kBpThreads = kTileCols;  // one voxel column per thread
 // Rows per workgroup: 16 keeps the kernel at 94 VGPRs (5 waves per SIMD);
-// measured at 512^3/720: 64 rows 27.5 ms, 32 rows 22.8 ms, 16 rows 20.8 ms.
+// measured at 512^3/720: 64 rows 27.5 ms, 32 rows 22.8 ms, 16 rows 20.8 ms,
+// 8 rows 22.0 ms; 2 or 8 angles per LDS pass were slower than 4.
 constexpr int kBpRows = 16;
 constexpr int kBpRowPad = kBpRows + 4; // LDS row stride (16-byte aligned)
 constexpr int kBpAngBlk = 4;           // angles staged per LDS pass
