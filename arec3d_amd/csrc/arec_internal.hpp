@@ -149,13 +149,8 @@ struct BpArgs {
     double radius;
     int xcent, nx, ny, na, nrowblk;
 };
-// One problem of a persistent batched launch: its arguments, a skip flag
-// (nullptr: never skipped) and the index of its first work item.
-struct BatchFwd {
-    FwdArgs a;
-    const int *skip;
-    int item0, pad;
-};
+// One problem of a persistent batched back projection: its arguments, a
+// skip flag (nullptr: never skipped) and the index of its first work item.
 struct BatchBp {
     BpArgs a;
     const int *skip;
@@ -184,11 +179,9 @@ int64_t forward_items(const Geometry &g);
 int64_t backward_items(const Geometry &g);
 FwdArgs fwd_args(const Geometry &g, const float *x, float *p, double *partials);
 BpArgs bp_args(const Geometry &g, const float *p, float *x, const float *bvec, double *partials);
-// Persistent launches over nprob problems (device arrays of descriptors,
-// item0 ascending, `total` items); d_counter is an int of scratch.  All
-// forward problems of one launch share rpl, all back problems the method.
-int launch_forward_batch(int rpl, const BatchFwd *d_probs, int nprob, int64_t total, int *d_counter,
-                         hipStream_t st);
+// Persistent back projection over nprob problems (device array of
+// descriptors, item0 ascending, `total` items, all with the same method);
+// d_counter is an int of scratch.
 int launch_backward_batch(int method, int mode, const BatchBp *d_probs, int nprob, int64_t total,
                           int *d_counter, hipStream_t st);
 

@@ -240,16 +240,21 @@ __device__ __forceinline__ void forward_body(const float *__restrict__ x, float 
     const int i0 = __builtin_amdgcn_readfirstlane(sd.x), i1 = __builtin_amdgcn_readfirstlane(sd.y);
     float *strip = s_out[wave];
 
-    float acc0[RPL], acc1[RPL], acc2[RPL];
+    // the window's three pixels (values, never addressed: a pointer to them
+    // would place them in scratch memory)
+    struct Rows {
+        float v[RPL];
+    };
+    Rows acc0, acc1, acc2;
 #pragma unroll
-    for (int r = 0; r < RPL; r++) acc0[r] = acc1[r] = acc2[r] = 0.0f;
+    for (int r = 0; r < RPL; r++) acc0.v[r] = acc1.v[r] = acc2.v[r] = 0.0f;
     double sq = 0.0;
     int base = __builtin_amdgcn_readfirstlane(sd.z);  // window: pixels base..base+2
     int strip_lo = u_lo;                                 // strip holds [strip_lo, +kFwdOut)
     const bool full_rows = (ybase + ROWS <= ny);
 
     // write the strip [strip_lo, strip_lo + kFwdOut) of the wave's rows
-    auto flush_strip = [&]() {
+    auto flush_strip = [&]() __attribute__((always_inline)) {
         __threadfence_block();
         for (int e = lane; e < ROWS * kFwdOut; e += 64) {
             int r = e / kFwdOut, c = e % kFwdOut;
@@ -260,49 +265,41 @@ __device__ __forceinline__ void forward_body(const float *__restrict__ x, float 
         strip_lo += kFwdOut;
     };
     // pixel q of the wave's rows is complete with values v
-    auto complete = [&](int q, const float *v) {
+    auto complete = [&](int q, const Rows v) __attribute__((always_inline)) {
         if (q < u_lo || q >= u_hi) return;
         while (q >= strip_lo + kFwdOut) flush_strip();
 #pragma unroll
         for (int r = 0; r < RPL; r++) {
-            strip[(lane * RPL + r) * LDW + (q - strip_lo)] = v[r];
+            strip[(lane * RPL + r) * LDW + (q - strip_lo)] = v.v[r];
             if (y0 + r < ny) {
-                float t = v[r] * v[r];
+                float t = v.v[r] * v.v[r];
                 sq += (double)t;
             }
         }
     };
-    float zero[RPL];
+    Rows zero;
 #pragma unroll
-    for (int r = 0; r < RPL; r++) zero[r] = 0.0f;
+    for (int r = 0; r < RPL; r++) zero.v[r] = 0.0f;
     // advance the window so that it starts at pixel nb (nb >= base)
-    auto advance = [&](int nb) {
-        int step = nb - base;
+    auto advance = [&](int nb) __attribute__((always_inline)) {
+        const int step = nb - base;
         if (step == 0) return;
+        complete(base, acc0);
+        if (step >= 2) complete(base + 1, acc1);
         if (step >= 3) {
-            complete(base, acc0);
-            complete(base + 1, acc1);
             complete(base + 2, acc2);
             for (int q = max(base + 3, u_lo); q < min(nb, u_hi); q++) complete(q, zero);
+        }
+        // the window's new pixels, every assignment unconditional: stores
+        // made in some branches only are merged by the compiler into one
+        // store through a selected pointer, which puts the window in scratch
 #pragma unroll
-            for (int r = 0; r < RPL; r++) acc0[r] = acc1[r] = acc2[r] = 0.0f;
-        } else if (step == 1) {
-            complete(base, acc0);
-#pragma unroll
-            for (int r = 0; r < RPL; r++) {
-                acc0[r] = acc1[r];
-                acc1[r] = acc2[r];
-                acc2[r] = 0.0f;
-            }
-        } else {
-            complete(base, acc0);
-            complete(base + 1, acc1);
-#pragma unroll
-            for (int r = 0; r < RPL; r++) {
-                acc0[r] = acc2[r];
-                acc1[r] = 0.0f;
-                acc2[r] = 0.0f;
-            }
+        for (int r = 0; r < RPL; r++) {
+            const float n0 = step == 1 ? acc1.v[r] : step == 2 ? acc2.v[r] : 0.0f;
+            const float n1 = step == 1 ? acc2.v[r] : 0.0f;
+            acc0.v[r] = n0;
+            acc1.v[r] = n1;
+            acc2.v[r] = 0.0f;
         }
         base = nb;
     };
@@ -356,9 +353,9 @@ __device__ __forceinline__ void forward_body(const float *__restrict__ x, float 
                         w2 = __uint_as_float(e[k].w);
 #pragma unroll
             for (int r = 0; r < RPL; r++) {
-                acc0[r] = fmaf(w0, v[k][r], acc0[r]);
-                acc1[r] = fmaf(w1, v[k][r], acc1[r]);
-                acc2[r] = fmaf(w2, v[k][r], acc2[r]);
+                acc0.v[r] = fmaf(w0, v[k][r], acc0.v[r]);
+                acc1.v[r] = fmaf(w1, v[k][r], acc1.v[r]);
+                acc2.v[r] = fmaf(w2, v[k][r], acc2.v[r]);
             }
         }
         }
@@ -374,12 +371,6 @@ __device__ __forceinline__ void forward_body(const float *__restrict__ x, float 
         for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
         if (lane == 0) partials[((size_t)a * nchunks + chunk) * nseg + sg] = sq;
     }
-}
-
-template <int RPL>
-__device__ __forceinline__ void forward_item(const FwdArgs &A, int b) {
-    forward_body<RPL>(A.x, A.p, A.table, A.seg, A.pixstart, A.partials, A.npix, A.nvalid, A.nseg, A.segw,
-                      A.nx, A.ny, A.na, A.nchunks, b);
 }
 
 // Plain kernel arguments (not a struct): the compiler then keeps the sizes in
@@ -625,39 +616,28 @@ __global__ __launch_bounds__(64 * kStripAngles) void forward_strip_kernel(
 }
 
 // Index of the problem that owns work item `it`: the last k with
-// item0(k) <= it (item0 ascending).  Uniform across the workgroup.
+// item0(k) <= it (item0 ascending).  Uniform across the workgroup.  The
+// item0 values of the first kBatchLdsProbs problems are staged in LDS once
+// per workgroup (stage_item0), so the search costs no global round trips
+// for batches of that size.
+constexpr int kBatchLdsProbs = 256;
+
 template <typename P>
-__device__ __forceinline__ int owner_of(const P *__restrict__ probs, int nprob, int it) {
+__device__ __forceinline__ void stage_item0(const P *__restrict__ probs, int nprob, int *s_item0) {
+    for (int i = threadIdx.x; i < min(nprob, kBatchLdsProbs); i += blockDim.x) s_item0[i] = probs[i].item0;
+    __syncthreads();
+}
+
+template <typename P>
+__device__ __forceinline__ int owner_of(const P *__restrict__ probs, const int *s_item0, int nprob, int it) {
     int lo = 0, hi = nprob - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (probs[mid].item0 <= it) lo = mid;
+        const int i0 = mid < kBatchLdsProbs ? s_item0[mid] : probs[mid].item0;
+        if (i0 <= it) lo = mid;
         else hi = mid - 1;
     }
     return lo;
-}
-
-// Persistent forward projector over many problems: workgroups take work
-// items (one forward workgroup's job of one problem) from a queue until it
-// is empty.  *counter must be 0 at launch.
-template <int RPL>
-__global__ __launch_bounds__(64 * kFwdAngles) void forward_batch_kernel(const BatchFwd *__restrict__ probs,
-                                                                       int nprob, int total,
-                                                                       int *__restrict__ counter) {
-    __shared__ int s_item;
-    for (;;) {
-        __syncthreads();  // every wave has read the previous item
-        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
-        __syncthreads();
-        // uniform: lets the compiler read the problem's arguments into
-        // scalar registers
-        const int it = __builtin_amdgcn_readfirstlane(s_item);
-        if (it >= total) break;
-        const int k = __builtin_amdgcn_readfirstlane(owner_of(probs, nprob, it));
-        const int *skip = probs[k].skip;
-        if (skip && *skip) continue;
-        forward_item<RPL>(probs[k].a, it - probs[k].item0);
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -846,18 +826,22 @@ __global__ __launch_bounds__(kBpThreads) void backward_kernel(BpArgs B, const in
     backward_item<M, MODE>(B, blockIdx.x);
 }
 
-// Persistent back projector over many problems (see forward_batch_kernel).
+// Persistent back projector over many problems: workgroups take work items
+// (one back-projection workgroup's job of one problem) from a queue until it
+// is empty.  *counter must be 0 at launch.
 template <int M, int MODE>
 __global__ __launch_bounds__(kBpThreads) void backward_batch_kernel(const BatchBp *__restrict__ probs, int nprob,
                                                                    int total, int *__restrict__ counter) {
     __shared__ int s_item;
+    __shared__ int s_item0[kBatchLdsProbs];
+    stage_item0(probs, nprob, s_item0);
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
         const int it = __builtin_amdgcn_readfirstlane(s_item);
         if (it >= total) break;
-        const int k = __builtin_amdgcn_readfirstlane(owner_of(probs, nprob, it));
+        const int k = __builtin_amdgcn_readfirstlane(owner_of(probs, s_item0, nprob, it));
         const int *skip = probs[k].skip;
         if (skip && *skip) continue;
         backward_item<M, MODE>(probs[k].a, it - probs[k].item0);
@@ -1206,6 +1190,8 @@ int launch_backward(const Geometry &g, const float *p, float *x, int mode, const
     return 0;
 }
 
+// The queue counter to zero by a kernel (a memset node would not be safe in
+// a captured graph).
 __global__ void zero_counter_kernel(int *c) { *c = 0; }
 
 // Persistent grid: as many workgroups as can be resident on the device.
@@ -1216,29 +1202,6 @@ static int resident_grid(K kernel, int threads, int *out) {
     AREC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     AREC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
     *out = std::max(1, ncu * std::max(1, per_cu));
-    return 0;
-}
-
-int launch_forward_batch(int rpl, const BatchFwd *d_probs, int nprob, int64_t total, int *d_counter,
-                         hipStream_t st) {
-    if (nprob <= 0 || total <= 0) return 0;
-    if (total > INT32_MAX) {
-        set_error("batched forward: too many work items");
-        return AREC_ESIZE;
-    }
-    zero_counter_kernel<<<1, 1, 0, st>>>(d_counter);
-    int grid = 0, rc;
-#define AREC_FWDB(R)                                                                                         \
-    {                                                                                                        \
-        if ((rc = resident_grid(forward_batch_kernel<R>, 64 * kFwdAngles, &grid))) return rc;               \
-        grid = (int)std::min<int64_t>(grid, total);                                                          \
-        forward_batch_kernel<R><<<grid, 64 * kFwdAngles, 0, st>>>(d_probs, nprob, (int)total, d_counter); \
-    }
-    if (rpl == 4) AREC_FWDB(4)
-    else if (rpl == 2) AREC_FWDB(2)
-    else AREC_FWDB(1)
-#undef AREC_FWDB
-    AREC_HIP(hipGetLastError());
     return 0;
 }
 

@@ -419,7 +419,7 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
         o_bp[k + 1] = o_bp[k] + std::max<int64_t>(1, backward_partial_count(*P[k].g));
     }
     const int cap = std::max(1, hist_cap);
-    DevBuf bdata, grad, proj, bp_part, fp_part, dstate, dhist, dvec, dall, dfwd, dbp, dbp0, dcnt, dseg1;
+    DevBuf bdata, grad, proj, bp_part, fp_part, dstate, dhist, dvec, dall, dbp, dbp0, dcnt;
     int rc;
     if ((rc = alloc(bdata, sizeof(float) * o_nnz[nprob])) || (rc = alloc(grad, sizeof(float) * o_nnz[nprob])) ||
         (rc = alloc(proj, sizeof(float) * o_proj[nprob])) || (rc = alloc(bp_part, sizeof(double) * o_bp[nprob])) ||
@@ -459,70 +459,20 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
     AREC_HIP(hipMemcpyAsync(S, s0.data(), sizeof(SirtState) * nprob, hipMemcpyHostToDevice, st));
     AREC_HIP(hipMemcpyAsync(dvec.p, vec.data(), sizeof(SirtVec) * nprob, hipMemcpyHostToDevice, st));
 
-    // Projector launches of a pass.  Batched: descriptors per forward rpl
-    // class and one back descriptor table, problems in decreasing order of
-    // per-item work so that the long items start first.
-    struct FwdClass {
-        int rpl;
-        int n;
-        int64_t total;
-        BatchFwd *d;
-    };
-    std::vector<FwdClass> fcls;
+    // Projector launches of a pass.  Batched: each problem's forward
+    // projection is its own launch with its own autotuned plan (the grid
+    // remap gives every XCD a contiguous range of neighbouring angle
+    // groups); the back projections of all problems share one persistent
+    // launch, problems in decreasing order of angle count so that the long
+    // items start first.  A persistent forward over all problems measured
+    // 0.66-0.85x of the problems solved one after the other -- its loop kept
+    // the inlined body's invariants in registers (86 instead of 44 VGPRs at
+    // 2 rows per lane), halving occupancy -- where separate forward
+    // launches make the batch 1.03x.
     int64_t bp_total = 0;
     std::vector<BatchBp> hb0;  // A^T b (store)
     std::vector<BatchBp> hb;   // gradient pass
-    std::vector<BatchFwd> hf_all;
     if (nprob > 1) {
-        // One detector segment per angle group: a batch has parallelism
-        // across problems, and fewer, longer items cost less queue overhead
-        // than the many short items a single problem's plan asks for.
-        // Results do not depend on the plan; the geometries keep theirs.
-        std::vector<int> segw1(nprob);
-        std::vector<int64_t> o_seg(nprob + 1, 0);
-        std::vector<int> hseg;
-        for (int k = 0; k < nprob; k++) {
-            const Geometry &g = *P[k].g;
-            segw1[k] = (g.nx + kFwdOut - 1) / kFwdOut * kFwdOut;
-            std::vector<int> t;
-            segment_table(g, segw1[k], 1, t);
-            hseg.insert(hseg.end(), t.begin(), t.end());
-            o_seg[k + 1] = (int64_t)hseg.size();
-        }
-        if ((rc = alloc(dseg1, sizeof(int) * hseg.size()))) return rc;
-        AREC_HIP(hipMemcpyAsync(dseg1.p, hseg.data(), sizeof(int) * hseg.size(), hipMemcpyHostToDevice, st));
-        std::vector<int> order(nprob);
-        for (int k = 0; k < nprob; k++) order[k] = k;
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-            const Geometry &ga = *P[a].g, &gb = *P[b].g;
-            return (int64_t)ga.nvalid * ga.fwd_rpl > (int64_t)gb.nvalid * gb.fwd_rpl;
-        });
-        for (int rpl : {4, 2, 1}) {
-            FwdClass c{rpl, 0, 0, nullptr};
-            for (int k : order) {
-                const Geometry &g = *P[k].g;
-                if (g.fwd_rpl != rpl) continue;
-                BatchFwd f;
-                f.a = fwd_args(g, P[k].x, proj.as<float>() + o_proj[k], nullptr);
-                f.a.seg = reinterpret_cast<const int4 *>(dseg1.as<int>() + o_seg[k]);
-                f.a.nseg = 1;
-                f.a.segw = segw1[k];
-                f.skip = &S[k].skip_proj;
-                f.item0 = (int)c.total;
-                f.pad = 0;
-                hf_all.push_back(f);
-                c.total += (int64_t)((g.na + kFwdAngles - 1) / kFwdAngles) * g.fwd_chunks;
-                c.n++;
-            }
-            if (c.n) fcls.push_back(c);
-        }
-        if ((rc = alloc(dfwd, sizeof(BatchFwd) * hf_all.size()))) return rc;
-        AREC_HIP(hipMemcpyAsync(dfwd.p, hf_all.data(), sizeof(BatchFwd) * hf_all.size(), hipMemcpyHostToDevice, st));
-        size_t off = 0;
-        for (FwdClass &c : fcls) {
-            c.d = dfwd.as<BatchFwd>() + off;
-            off += c.n;
-        }
         std::vector<int> border(nprob);
         for (int k = 0; k < nprob; k++) border[k] = k;
         std::stable_sort(border.begin(), border.end(), [&](int a, int b) { return P[a].g->na > P[b].g->na; });
@@ -578,8 +528,9 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
                                      bp_part.as<double>(), st, &S->skip_proj)))
                 return r;
         } else {
-            for (const FwdClass &c : fcls)
-                if ((r = launch_forward_batch(c.rpl, c.d, c.n, c.total, dcnt.as<int>(), st))) return r;
+            for (int k = 0; k < nprob; k++)
+                if ((r = launch_forward(*P[k].g, P[k].x, proj.as<float>() + o_proj[k], nullptr, st, &S[k].skip_proj)))
+                    return r;
             if ((r = launch_backward_batch(method, kBpSirtGrad, dbp.as<BatchBp>(), nprob, bp_total, dcnt.as<int>(), st)))
                 return r;
         }

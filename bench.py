@@ -212,7 +212,7 @@ def batch_workload(args):
         ne = int(rng.integers(8, 31))
         ang = (np.arange(na) * np.pi / na).astype(np.float32)
         g = A.Geometry(ctx, "sq", nx, ny, ang)
-        autotune(ctx, g, designs=(1,))  # the persistent kernels use the sweep's tables
+        autotune(ctx, g, designs=(1,))  # tuned over the detector sweep (design 1) only
         sup = 0.47 * nx
         f = nx / 128.0
         prm = P.ellipsoid_params(3000 + k, ne, sup, ny, axes=(4.0 * f, 30.0 * f), dens=(0.1, 1.0))
