@@ -212,7 +212,7 @@ def batch_workload(args):
         ne = int(rng.integers(8, 31))
         ang = (np.arange(na) * np.pi / na).astype(np.float32)
         g = A.Geometry(ctx, "sq", nx, ny, ang)
-        autotune(ctx, g, designs=(1,))  # tuned over the detector sweep (design 1) only
+        autotune(ctx, g)
         sup = 0.47 * nx
         f = nx / 128.0
         prm = P.ellipsoid_params(3000 + k, ne, sup, ny, axes=(4.0 * f, 30.0 * f), dens=(0.1, 1.0))
@@ -286,6 +286,13 @@ def cgls_workload(args):
     ctx = A.Context(0)
     ang = (np.arange(na) * np.pi / na).astype(np.float32)
     g = A.Geometry(ctx, "sq", nx, ny, ang)
+    if args.plan:
+        g.set_forward(*args.plan)
+    elif not args.no_autotune:
+        from arec3d_amd.autotune import autotune
+
+        autotune(ctx, g, log=lambda m: log("autotune " + m))
+    log(f"forward design {g.forward_design}, {g.get_plan()[0]} rows/lane")
     prm = P.ellipsoid_params(2, 40, 250.0, ny, axes=(8.0, 120.0), dens=(0.05, 1.0))
     xt = ctx.empty(g.vol_shape)
     P.phantom_device(ctx, xt, g.radius, ny, 250.0, prm)
@@ -331,6 +338,8 @@ def cgls_workload(args):
         "data": "synthetic (seeded ellipsoid phantom, Beer-Lambert counts at 1e4 photons)",
         "config": {
             "workload": f"CGLS (SQ) {iters} iterations, {nx}x{nx}x{ny} volume, {na} angles",
+            "forward_plan": {"design": {1: "detector sweep", 2: "depth strips"}[g.forward_design],
+                             "rows_per_lane": g.get_plan()[0]},
             "voxel_updates_per_s": round(vu / dt, 1),
             "ms_per_iteration": round(dt * 1e3 / iters, 3),
             "final_relnrm": hist[-1][2],
