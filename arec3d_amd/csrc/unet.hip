@@ -71,30 +71,55 @@ __global__ __launch_bounds__(kCvThreads) void conv3_mfma_kernel(
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) acc[nb] = f4v{0.f, 0.f, 0.f, 0.f};
     const int H2 = d.H >> 1, W2 = d.W >> 1;
-    for (int tap = 0; tap < 27; tap++) {
-        const int dz = tap / 9 - 1, dh = (tap / 3) % 3 - 1, dw = tap % 3 - 1;
+    // Per (dz, dh) row of taps, the operands of its three dw taps for KB
+    // channel blocks are loaded first and multiplied after, so a batch of
+    // loads is in flight at a time instead of one (tap, block) pair.  The
+    // MFMAs run per row in the order (channel batch, dw tap, block); when
+    // one batch holds all the input channels (Ct <= 16 KB) that is the
+    // one-tap order (tap, block), otherwise the FP32 sums are reassociated.
+    constexpr int KB = NB == 1 ? 3 : NB == 2 ? 2 : 1;
+    for (int row = 0; row < 9; row++) {
+        const int dz = row / 3 - 1, dh = row % 3 - 1;
         const int zz = z + dz, hh = h + dh;
         if (zz < 0 || zz >= d.D || hh < 0 || hh >= d.H) continue;  // uniform
-        const int pp = pa + dw;
-        const bool pv = pp >= 0 && pp < d.W;
-        const bf16_t *wt_tap = w + (size_t)tap * COUT * Ct;
-        for (int cb = 0; cb < Ct; cb += 16) {
-            const int ci = cb + kq;
-            s4v a = s4v{0, 0, 0, 0};
-            if (pv) {
-                const bf16_t *src;
-                if (ci < C1)
-                    src = in1 + (((size_t)zz * d.H + hh) * d.W + pp) * C1 + ci;
-                else
-                    src = in2 + (((size_t)(zz >> 1) * H2 + (hh >> 1)) * W2 + (pp >> 1)) * C2 + (ci - C1);
-                a = *reinterpret_cast<const s4v *>(src);
+        for (int cb0 = 0; cb0 < Ct; cb0 += 16 * KB) {
+            s4v a[3][KB], b[3][KB][NB];
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                const int pp = pa + t - 1;
+                const bool pv = pp >= 0 && pp < d.W;
+                const bf16_t *wt_tap = w + (size_t)(row * 3 + t) * COUT * Ct;
+#pragma unroll
+                for (int kb = 0; kb < KB; kb++) {
+                    const int ci = cb0 + kb * 16 + kq;
+                    a[t][kb] = s4v{0, 0, 0, 0};
+#pragma unroll
+                    for (int nb = 0; nb < NB; nb++) b[t][kb][nb] = s4v{0, 0, 0, 0};
+                    if (cb0 + kb * 16 < Ct) {  // uniform
+                        if (pv) {
+                            const bf16_t *src;
+                            if (ci < C1)
+                                src = in1 + (((size_t)zz * d.H + hh) * d.W + pp) * C1 + ci;
+                            else
+                                src = in2 + (((size_t)(zz >> 1) * H2 + (hh >> 1)) * W2 + (pp >> 1)) * C2 + (ci - C1);
+                            a[t][kb] = *reinterpret_cast<const s4v *>(src);
+                        }
+#pragma unroll
+                        for (int nb = 0; nb < NB; nb++) {
+                            const int co = nb * 16 + (lane & 15);
+                            b[t][kb][nb] = *reinterpret_cast<const s4v *>(wt_tap + (size_t)co * Ct + ci);
+                        }
+                    }
+                }
             }
 #pragma unroll
-            for (int nb = 0; nb < NB; nb++) {
-                const int co = nb * 16 + (lane & 15);
-                s4v b = *reinterpret_cast<const s4v *>(wt_tap + (size_t)co * Ct + ci);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
-            }
+            for (int t = 0; t < 3; t++)
+#pragma unroll
+                for (int kb = 0; kb < KB; kb++)
+                    if (cb0 + kb * 16 < Ct)
+#pragma unroll
+                        for (int nb = 0; nb < NB; nb++)
+                            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t][kb], b[t][kb][nb], acc[nb], 0, 0, 0);
         }
     }
     // lane holds rows 4*(lane/16)+i (positions) of column lane%16 (channel)
