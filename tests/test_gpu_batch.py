@@ -7,7 +7,7 @@ of the forward projector's launch plans.
 * Batched SIRT over problems of uneven width, height and angle count gives
   each problem exactly (bit for bit) the history and volume of its own
   single-problem SIRT, including problems in different rows-per-lane
-  classes and one that restarts.
+  classes, on either forward design, and one that restarts.
 * Each batched problem's history agrees with the FP32 oracle under the
   solver tests' tolerance (1e-5 relative, or 4x the FP32/FP64 oracle spread).
 """
@@ -81,6 +81,29 @@ def test_batched_sirt_equals_single(ctx):
         hb, lb = res[k]
         assert hb == h1, k
         assert lb == l1
+        assert np.array_equal(xs[k].download(), xk.download()), k
+
+
+def test_batched_sirt_with_depth_strips_equals_single(ctx):
+    """Problems on forward design 2 (depth strips) mixed with design-1
+    problems: each batched problem still equals its single SIRT bit for
+    bit (the batch launches every forward with the problem's own design)."""
+    import arec3d_amd as A
+
+    print("\n[case] batched sirt (depth strips) vs single", flush=True)
+    probs = _problems(ctx)
+    for k in (0, 2, 4):
+        probs[k][0].set_forward(2, 0)
+        assert probs[k][0].forward_design == 2
+    maxit, tol = 10, 1e-6
+    bs = [ctx.to_device(b) for _, b, _, _ in probs]
+    xs = [ctx.empty(g.vol_shape) for g, _, _, _ in probs]
+    res = A.sirt_batch(ctx, [p[0] for p in probs], bs, xs, [p[3] for p in probs], maxit, tol)
+    for k, (g, b, ang, lam) in enumerate(probs):
+        xk = ctx.empty(g.vol_shape)
+        h1, l1 = g.sirt(bs[k], xk, lam, maxit, tol)
+        assert res[k][0] == h1, k
+        assert res[k][1] == l1
         assert np.array_equal(xs[k].download(), xk.download()), k
 
 
