@@ -86,6 +86,10 @@ def _load():
         "arec_match": ([vp, vp, vp, vp, flt, i32, i32, flt, i32, i32, vp, vp, P(Hist)], C.c_int),
         "arec_match_ex": ([vp, vp, vp, vp, flt, i32, i32, flt, i32, i32, i32, vp, vp, P(Hist)], C.c_int),
         "arec_align_centroid": ([vp, vp, vp, vp], C.c_int),
+        "arec_take_log": ([vp, vp, i64], C.c_int),
+        "arec_periodic_decomposition": ([vp, vp, i32, i32, i32], C.c_int),
+        "arec_h_take_log": ([vp, i64], C.c_int),
+        "arec_h_periodic_decomposition": ([i32, i32, i32, vp], C.c_int),
         "arec_geom_get_plan": ([vp, P(i32), P(i32), P(i32)], C.c_int),
         "arec_unet_create": ([vp, P(vp), i32, i32, i32, i32, C.c_uint64], C.c_int),
         "arec_unet_destroy": ([vp], C.c_int),
@@ -124,7 +128,8 @@ EXPORTED = [
     "arec_unet_destroy", "arec_unet_nparams", "arec_unet_get_params", "arec_unet_set_params",
     "arec_unet_set_scale", "arec_unet_forward", "arec_unet_train_step", "arec_unet_gradient",
     "arec_conv3_check", "arec_conv3_wgrad_check", "arec_cyl_to_cube", "arec_cube_to_cyl", "arec_h_project", "arec_h_backproject",
-    "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan", "arec_match", "arec_match_ex", "arec_align_centroid", "arec_scale", "arec_geom_set_forward",
+    "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan", "arec_match", "arec_match_ex", "arec_align_centroid", "arec_take_log", "arec_periodic_decomposition",
+    "arec_h_take_log", "arec_h_periodic_decomposition", "arec_scale", "arec_geom_set_forward",
     "arec_geom_forward_design",
 ]
 
@@ -375,6 +380,17 @@ def match(ctx, g, views, x, lam, outer=5, maxit=20, tol=0.0, cc_passes=3, method
     _check(lib.arec_match_ex(ctx.h, g.h, views.ptr, x.ptr, lam, outer, maxit, tol, cc_passes, m, flags, _ptr(sx),
                              _ptr(sy), H))
     return sx, sy, _hist(H, outer)
+
+
+def take_log(ctx, data):
+    """arecImageTakeLog on a DeviceArray, in place: -log(x) for x > 0, else 0."""
+    _check(lib.arec_take_log(ctx.h, data.ptr, int(np.prod(data.shape))))
+
+
+def periodic_decomposition(ctx, views):
+    """periodicDecomposition of a view stack ([nv, ny, nx] DeviceArray), in place."""
+    nv, ny, nx = views.shape
+    _check(lib.arec_periodic_decomposition(ctx.h, views.ptr, nx, ny, nv))
 
 
 def align_centroid(ctx, g, views):

@@ -416,6 +416,75 @@ int shift_views(float *data, int nx, int ny, int nv, const float *sx, const floa
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// Preprocessing of the measured views (the driver's steps before the first
+// reconstruction, AREC3D.cpp:81,105).
+// ---------------------------------------------------------------------------
+
+// arecImageTakeLog (arecImage.cpp:358-381): x > 0 -> -log(x), else 0.
+__global__ void take_log_kernel(float *__restrict__ x, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = x[i];
+        x[i] = v > 0.0f ? -logf(v) : 0.0f;
+    }
+}
+
+// Moisan's boundary image of every view of a batch (makePeriodicSlice,
+// imagetools.cpp:26-35), as complex values with zero imaginary part: the
+// rows' jumps assigned (row ny-1 last), then the columns' jumps added
+// (column nx-1 last), in FP32 -- the reference's order, which matters only
+// for ny == 1 or nx == 1.
+__global__ void boundary_image_kernel(const float *__restrict__ u, float2 *__restrict__ c, int nx, int ny,
+                                      int64_t n) {
+    const int64_t per = (int64_t)nx * ny;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = e / per;
+        const int k = (int)(e - v * per), iy = k / nx, ix = k - iy * nx;
+        const float *__restrict__ uv = u + v * per;
+        const int64_t top = ix, bot = (int64_t)(ny - 1) * nx + ix;
+        const int64_t left = (int64_t)iy * nx, right = left + nx - 1;
+        float val = 0.0f;
+        if (iy == ny - 1)
+            val = uv[top] - uv[bot];
+        else if (iy == 0)
+            val = uv[bot] - uv[top];
+        if (ix == 0) val += uv[right] - uv[left];
+        if (ix == nx - 1) val += uv[left] - uv[right];
+        c[e] = make_float2(val, 0.0f);
+    }
+}
+
+// S = V / (2 cos(2 pi i/nx) + 2 cos(2 pi j/ny) - 4) with S(0,0) = 0
+// (imagetools.cpp:53-62: the quotient in double, stored as float).
+__global__ void periodic_divide_kernel(float2 *__restrict__ f, int nx, int ny, int64_t n) {
+    const double pi = 3.14159265358979323846;
+    const int64_t per = (int64_t)nx * ny;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(e % per), j = k / nx, i = k - j * nx;
+        if (k == 0) {
+            f[e] = make_float2(0.0f, 0.0f);
+            continue;
+        }
+        const double den = 2 * cos(2.0 * pi * i / nx) + 2 * cos(2.0 * pi * j / ny) - 4;
+        f[e] = make_float2((float)(f[e].x / den), (float)(f[e].y / den));
+    }
+}
+
+// u -= W(d) s with s = real part / (float)(nx ny) (the unnormalised inverse
+// transform, imagetools.cpp:68-75), W = exp(-d^2 / (2 * 5^2)), d the distance
+// to the nearest border (periodicDecomposition, imagetools.cpp:95-109).
+__global__ void periodic_apply_kernel(const float2 *__restrict__ c, float *__restrict__ u, int nx, int ny,
+                                      int64_t n, float div) {
+    const int64_t per = (int64_t)nx * ny;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(e % per), iy = k / nx, ix = k - iy * nx;
+        const int d = min(min(ix, nx - 1 - ix), min(iy, ny - 1 - iy));
+        const float s = c[e].x / div;
+        const double w = exp(-1.0 * d * d / (2 * 5.0 * 5.0));
+        u[e] = (float)((double)u[e] - w * (double)s);
+    }
+}
+
 // Zeroth and first moments of every view of a stack [nv][ny][nx], in FP64:
 // mom[3v..3v+2] = {sum p, sum u*p, sum y*p} over the view's pixels (u, y).
 // One workgroup per view; each thread sums a fixed set of columns of every
@@ -448,6 +517,38 @@ __global__ __launch_bounds__(256) void view_moments_kernel(const float *__restri
         __syncthreads();
     }
     if (tid < 3) mom[3 * (int64_t)v + tid] = red[tid][0];
+}
+
+int take_log(float *x, int64_t n, hipStream_t st) {
+    if (n <= 0) return 0;
+    take_log_kernel<<<ew_grid(n), 256, 0, st>>>(x, n);
+    AREC_HIP(hipGetLastError());
+    return 0;
+}
+
+// periodicDecomposition (imagetools.cpp:86-114) of every view, in batches.
+int periodic_views(float *data, int nx, int ny, int nv, hipStream_t st) {
+    AlignWork w;
+    const int vb = views_per_batch(nx, ny, nv);
+    const int64_t per = (int64_t)nx * ny;
+    int rc;
+    if ((rc = w.ensure((size_t)vb * per))) return rc;
+    for (int v0 = 0; v0 < nv; v0 += vb) {
+        const int n = std::min(vb, nv - v0);
+        const int64_t cnt = (int64_t)n * per;
+        boundary_image_kernel<<<ew_grid(cnt), 256, 0, st>>>(data + v0 * per, w.a, nx, ny, cnt);
+        if ((rc = fft2(w.a, w.t, nx, ny, n, -1, st))) break;
+        periodic_divide_kernel<<<ew_grid(cnt), 256, 0, st>>>(w.a, nx, ny, cnt);
+        if ((rc = fft2(w.a, w.t, nx, ny, n, +1, st))) break;
+        periodic_apply_kernel<<<ew_grid(cnt), 256, 0, st>>>(w.a, data + v0 * per, nx, ny, cnt, (float)(nx * ny));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "periodic_views");
+            break;
+        }
+    }
+    hipStreamSynchronize(st);
+    return rc;
 }
 
 int view_moments(const float *views, int nx, int ny, int nv, double *mom_dev, hipStream_t st) {
@@ -489,6 +590,22 @@ int arec_align_peak(arec_ctx *ctx, const float *c_dev, int32_t nx, int32_t ny, i
     speak_kernel<<<nviews, 256, 0, ctx->stream>>>(c_dev, nx, ny, sx_dev, sy_dev);
     AREC_HIP(hipGetLastError());
     return 0;
+}
+
+int arec_take_log(arec_ctx *ctx, float *data_dev, int64_t n) {
+    if (!ctx || (!data_dev && n > 0) || n < 0) {
+        set_error("arec_take_log: bad argument");
+        return AREC_EINVAL;
+    }
+    return take_log(data_dev, n, ctx->stream);
+}
+
+int arec_periodic_decomposition(arec_ctx *ctx, float *views_dev, int32_t nx, int32_t ny, int32_t nviews) {
+    if (!ctx || !views_dev || nx < 1 || ny < 1 || nviews < 1) {
+        set_error("arec_periodic_decomposition: bad argument");
+        return AREC_EINVAL;
+    }
+    return periodic_views(views_dev, nx, ny, nviews, ctx->stream);
 }
 
 int arec_align_shift(arec_ctx *ctx, float *data_dev, int32_t nx, int32_t ny, int32_t nviews,
@@ -566,6 +683,56 @@ int arec_h_shift_images(int32_t nx, int32_t ny, int32_t nz, float *data, const f
     hipFree(s);
     hipStreamDestroy(st);
     return rc;
+}
+
+// Host-pointer versions of the preprocessing (the drop-in's
+// arecImageTakeLog and periodicDecomposition): upload, compute, download.
+static int host_views_op(float *data, int64_t n, const char *what, int (*op)(float *, hipStream_t, const void *),
+                         const void *arg) {
+    hipStream_t st;
+    AREC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    float *a = nullptr;
+    int rc = 0;
+    if (hipMalloc((void **)&a, sizeof(float) * std::max<int64_t>(1, n)) != hipSuccess) {
+        rc = hip_fail(hipErrorOutOfMemory, what);
+    } else {
+        hipMemcpyAsync(a, data, sizeof(float) * n, hipMemcpyHostToDevice, st);
+        rc = op(a, st, arg);
+        if (!rc) {
+            hipMemcpyAsync(data, a, sizeof(float) * n, hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) rc = hip_fail(hipGetLastError(), what);
+        }
+    }
+    hipFree(a);
+    hipStreamDestroy(st);
+    return rc;
+}
+
+int arec_h_take_log(float *data, int64_t n) {
+    if ((!data && n > 0) || n < 0) {
+        set_error("arec_h_take_log: bad argument");
+        return AREC_EINVAL;
+    }
+    if (n == 0) return 0;
+    return host_views_op(data, n, "arec_h_take_log",
+                         [](float *a, hipStream_t st, const void *arg) {
+                             return take_log(a, *static_cast<const int64_t *>(arg), st);
+                         },
+                         &n);
+}
+
+int arec_h_periodic_decomposition(int32_t nx, int32_t ny, int32_t nz, float *data) {
+    if (!data || nx < 1 || ny < 1 || nz < 1) {
+        set_error("arec_h_periodic_decomposition: bad argument");
+        return AREC_EINVAL;
+    }
+    const int32_t dims[3] = {nx, ny, nz};
+    return host_views_op(data, (int64_t)nx * ny * nz, "arec_h_periodic_decomposition",
+                         [](float *a, hipStream_t st, const void *arg) {
+                             const int32_t *d = static_cast<const int32_t *>(arg);
+                             return periodic_views(a, d[0], d[1], d[2], st);
+                         },
+                         dims);
 }
 
 }  // extern "C"

@@ -216,6 +216,10 @@ int cc_views(const float *proj, const float *meas, int nx, int ny, int nv, float
              float *sy, hipStream_t st);
 int shift_views(float *data, int nx, int ny, int nv, const float *sx, const float *sy, float sgn,
                 hipStream_t st);
+// Preprocessing: -log of positive values (else 0); Moisan's periodic
+// decomposition of every view in place.
+int take_log(float *x, int64_t n, hipStream_t st);
+int periodic_views(float *data, int nx, int ny, int nv, hipStream_t st);
 // Per-view moments {sum p, sum u p, sum y p} (FP64, deterministic) into
 // mom_dev[3 * nv].
 int view_moments(const float *views, int nx, int ny, int nv, double *mom_dev, hipStream_t st);

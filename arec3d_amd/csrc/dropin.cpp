@@ -259,3 +259,16 @@ int arecCCImages(MPI_Comm comm, arecImage images1, arecImage images2, float *sx,
 void arecShiftImages(arecImage images, float *sx, float *sy) {
     arec_h_shift_images(images.nx, images.ny, images.nz, images.data, sx, sy);
 }
+
+// ---- preprocessing -------------------------------------------------------------
+
+// arecImage.cpp:358-381: a cylinder image holds nnz values, a stack nx*ny*nz.
+void arecImageTakeLog(arecImage image) {
+    const int64_t n = image.is_cyl == 1 ? (int64_t)image.nnz : (int64_t)image.nx * image.ny * image.nz;
+    arec_h_take_log(image.data, n);
+}
+
+// imagetools.cpp:86-114: every view of the stack, in place.
+void periodicDecomposition(arecImage img) {
+    arec_h_periodic_decomposition(img.nx, img.ny, img.nz, img.data);
+}

@@ -201,6 +201,17 @@ int arec_align_peak(arec_ctx *ctx, const float *c_dev, int32_t nx, int32_t ny, i
 int arec_align_shift(arec_ctx *ctx, float *data_dev, int32_t nx, int32_t ny, int32_t nviews,
                      const float *sx_dev, const float *sy_dev, float sign);
 
+/* ---- preprocessing of the measured views (AREC3D.cpp:81,105) ---------- */
+/* -log of every value > 0, 0 elsewhere, in place (arecImageTakeLog,
+ * arecImage.cpp:358-381; n values, e.g. nx*ny*nz of a view stack). */
+int arec_take_log(arec_ctx *ctx, float *data_dev, int64_t n);
+/* Moisan's periodic plus smooth decomposition of every view [nviews][ny][nx]
+ * in place (periodicDecomposition, imagetools.cpp:17-114): the smooth part s
+ * solves the periodic discrete Poisson equation with the boundary jumps as
+ * source (2-D FFT), and each view loses W(d) s, W = exp(-d^2/50), d the
+ * distance to the nearest border. */
+int arec_periodic_decomposition(arec_ctx *ctx, float *views_dev, int32_t nx, int32_t ny, int32_t nviews);
+
 /* Projection matching: the reference driver's alignment loop (AREC3D.cpp:
  * 140-165 initial reconstruction, :185-325 outer loops, translational
  * alignment only) with every buffer on the device.  views_dev [na][ny][nx]
@@ -301,6 +312,10 @@ int arec_h_cc_images(int32_t nx1, int32_t ny1, int32_t nz1, const float *images1
 /* arecShiftImages (arecImage.cpp:653-671): each view shifted by (-sx, -sy). */
 int arec_h_shift_images(int32_t nx, int32_t ny, int32_t nz, float *data, const float *sx,
                         const float *sy);
+/* arecImageTakeLog / periodicDecomposition on host data (upload, compute,
+ * download; the drop-in's versions). */
+int arec_h_take_log(float *data, int64_t n);
+int arec_h_periodic_decomposition(int32_t nx, int32_t ny, int32_t nz, float *data);
 
 #ifdef __cplusplus
 }

@@ -74,4 +74,8 @@ int cyl_cgls_SQ_stat(MPI_Comm comm, arecImage images, float *angles, arecImage *
 int arecCCImages(MPI_Comm comm, arecImage images1, arecImage images2, float *sx, float *sy);
 void arecShiftImages(arecImage images, float *sx, float *sy);
 
+/* arecImage.h:34, imagetools.h:5 (preprocessing of the measured views) */
+void arecImageTakeLog(arecImage image);
+void periodicDecomposition(arecImage img);
+
 #endif

@@ -449,6 +449,73 @@ void orc_cc_images(int nx, int ny, int nz, const float *proj, const float *meas,
     free(cf);
 }
 
+/* arecImageTakeLog, arecImage.cpp:358-381: x > 0 -> -log(x), else 0.  Under
+ * C++, <math.h> resolves log(float) to the float overload, so the restatement
+ * uses logf. */
+void orc_take_log(float *x, long n) {
+    for (long i = 0; i < n; i++) x[i] = x[i] > 0.0f ? -logf(x[i]) : 0.0f;
+}
+
+/* makePeriodicSlice, imagetools.cpp:17-80 (Moisan's periodic plus smooth
+ * decomposition): the boundary image v of u (rows first, assigned; then
+ * columns, added -- so for ny == 1 or nx == 1 the reference's order of
+ * assignments holds), S = DFT(v) / (2 cos(2 pi i/nx) + 2 cos(2 pi j/ny) - 4)
+ * (the quotient in double, stored as float) with S(0,0) = 0, and the smooth
+ * part s = real(IDFT(S)) / (float)(nx ny).  The reference's FP32 FFTW is
+ * replaced by an FP64 DFT. */
+void orc_periodic_smooth(int nx, int ny, const float *u, float *s) {
+    const double pi = 4.0 * atan(1.0);
+    size_t n = (size_t)nx * ny;
+    float *v = (float *)calloc(n, sizeof(float));
+    double *re = (double *)calloc(n, sizeof(double)), *im = (double *)calloc(n, sizeof(double));
+    for (int i = 0; i < nx; i++) {
+        v[i] = u[(size_t)nx * (ny - 1) + i] - u[i];
+        v[(size_t)(ny - 1) * nx + i] = u[i] - u[(size_t)nx * (ny - 1) + i];
+    }
+    for (int j = 0; j < ny; j++) {
+        v[(size_t)j * nx] += u[(size_t)nx * j + (nx - 1)] - u[(size_t)nx * j];
+        v[(size_t)j * nx + (nx - 1)] += u[(size_t)nx * j] - u[(size_t)nx * j + (nx - 1)];
+    }
+    for (size_t i = 0; i < n; i++) re[i] = v[i];
+    orc_dft2(re, im, nx, ny, -1);
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            size_t k = (size_t)nx * j + i;
+            if (k == 0) {
+                re[k] = im[k] = 0.0;
+                continue;
+            }
+            double den = 2 * cos(2.0 * pi * i / nx) + 2 * cos(2.0 * pi * j / ny) - 4;
+            re[k] = (float)(re[k] / den);
+            im[k] = (float)(im[k] / den);
+        }
+    orc_dft2(re, im, nx, ny, +1);
+    for (size_t i = 0; i < n; i++) s[i] = (float)re[i] / (float)(nx * ny);
+    free(v);
+    free(re);
+    free(im);
+}
+
+/* periodicDecomposition, imagetools.cpp:86-114: per view, u -= W(d) s with s
+ * the smooth part, W = exp(-d^2 / (2 * 5^2)) and d the distance to the
+ * nearest border (the product in double, the result stored as float). */
+void orc_periodic_decomposition(int nx, int ny, int nz, float *data) {
+    size_t n = (size_t)nx * ny;
+    float *s = (float *)malloc(sizeof(float) * n);
+    for (int z = 0; z < nz; z++) {
+        float *u = data + (size_t)z * n;
+        orc_periodic_smooth(nx, ny, u, s);
+        for (size_t i = 0; i < n; i++) {
+            int iy = (int)(i / nx), ix = (int)(i % nx);
+            int dx = ix < nx - 1 - ix ? ix : nx - 1 - ix, dy = iy < ny - 1 - iy ? iy : ny - 1 - iy;
+            double d = dx < dy ? dx : dy;
+            double w = exp(-1.0 * d * d / (2 * 5.0 * 5.0));
+            u[i] = (float)(u[i] - w * s[i]);
+        }
+    }
+    free(s);
+}
+
 /* arecShiftImages, arecImage.cpp:653-671: each view shifted by (-sx, -sy). */
 void orc_shift_images(int nx, int ny, int nz, float *data, const float *sx, const float *sy) {
     size_t n = (size_t)nx * ny;

@@ -100,6 +100,9 @@ void orc_cc_images(int nx, int ny, int nz, const float *proj, const float *meas,
 void orc_shift_images(int nx, int ny, int nz, float *data, const float *sx, const float *sy);
 /* AREC3D/AREC3D/arecImage.cpp:812-826 */
 void orc_soft_noneg(float *x, long n);
+void orc_take_log(float *x, long n);
+void orc_periodic_smooth(int nx, int ny, const float *u, float *s);
+void orc_periodic_decomposition(int nx, int ny, int nz, float *data);
 
 #ifdef __cplusplus
 }

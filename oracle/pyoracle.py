@@ -66,6 +66,9 @@ def lib():
         L.orc_cc_images.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, _F, _F]
         L.orc_shift_images.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, _F]
         L.orc_soft_noneg.argtypes = [_F, C.c_long]
+        L.orc_take_log.argtypes = [_F, C.c_long]
+        L.orc_periodic_smooth.argtypes = [C.c_int, C.c_int, _F, _F]
+        L.orc_periodic_decomposition.argtypes = [C.c_int, C.c_int, C.c_int, _F]
         _lib = L
     return _lib
 
@@ -238,6 +241,29 @@ def centroid_x_shifts(method, views, angles, nx, ny):
     t = angles.astype(np.float64)
     M = np.stack([np.cos(t), np.sin(t)], 1)
     return d - M @ np.linalg.lstsq(M, d, rcond=None)[0]
+
+
+def take_log(x):
+    """arecImageTakeLog: x > 0 -> -log(x), else 0 (FP32)."""
+    out = np.array(x, np.float32, copy=True)
+    lib().orc_take_log(out.reshape(-1), out.size)
+    return out
+
+
+def periodic_smooth(u):
+    """The smooth part of Moisan's decomposition of one view [ny, nx]."""
+    ny, nx = u.shape
+    s = np.zeros((ny, nx), np.float32)
+    lib().orc_periodic_smooth(nx, ny, np.ascontiguousarray(u, np.float32), s)
+    return s
+
+
+def periodic_decomposition(stack):
+    """periodicDecomposition of a view stack [nz, ny, nx] (a copy)."""
+    nz, ny, nx = stack.shape
+    out = np.array(stack, np.float32, copy=True)
+    lib().orc_periodic_decomposition(nx, ny, nz, out.reshape(-1))
+    return out
 
 
 def soft_noneg(x):

@@ -139,3 +139,42 @@ def test_centroid_x_shifts_vs_oracle(ctx, method, nx, ny, na):
     s = A.align_centroid(ctx, g, p)
     so = O.centroid_x_shifts(method, views, ang, nx, ny)
     np.testing.assert_allclose(s, so, atol=1e-4)  # FP32 device projection of the axis cylinder
+
+
+def test_take_log_vs_oracle(ctx):
+    """arec_take_log against the oracle's logf restatement: within 2 ulp
+    (the device and host logf differ by at most that), zeros and negatives
+    exact."""
+    import arec3d_amd as A
+
+    print("\n[case] take log", flush=True)
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.uniform(1e-6, 10.0, 4000), [0.0, -1.0, 1.0, -0.0]]).astype(np.float32)
+    d = ctx.to_device(x)
+    A.take_log(ctx, d)
+    g, o = d.download(), O.take_log(x)
+    ulp = np.spacing(np.abs(o).astype(np.float32))
+    assert np.all(np.abs(g - o) <= 2 * ulp + 1e-12)
+    assert np.array_equal(g[-4:], o[-4:])
+    # host-pointer version (the drop-in's arecImageTakeLog)
+    h = x.copy()
+    assert A.lib.arec_h_take_log(h.ctypes.data, h.size) == 0
+    np.testing.assert_array_equal(h, g)
+
+
+@pytest.mark.parametrize("nz,ny,nx", [(3, 32, 64), (2, 17, 30), (1, 1, 9), (2, 8, 1)])
+def test_periodic_decomposition_vs_oracle(ctx, nz, ny, nx):
+    """arec_periodic_decomposition (FP32 FFTs) against the oracle (FP64
+    DFTs) on seeded views: within 2e-5 of the views' scale."""
+    import arec3d_amd as A
+
+    print(f"\n[case] periodic decomposition {nz}x{ny}x{nx}", flush=True)
+    rng = np.random.default_rng(nx + 7 * ny + nz)
+    u = (rng.random((nz, ny, nx)) * 3.0 + np.linspace(0, 2, nx)[None, None, :]).astype(np.float32)
+    d = ctx.to_device(u)
+    A.periodic_decomposition(ctx, d)
+    g, o = d.download(), O.periodic_decomposition(u)
+    assert np.abs(g - o).max() <= 2e-5 * np.abs(u).max()
+    h = u.copy()
+    assert A.lib.arec_h_periodic_decomposition(nx, ny, nz, h.ctypes.data) == 0
+    np.testing.assert_array_equal(h, g)

@@ -404,3 +404,52 @@ def test_centroid_recovers_integer_shifts():
     # a y shift does not move the x centroid
     rolled_y = np.stack([np.roll(views[a], 1, axis=0) for a in range(na)])
     np.testing.assert_allclose(O.centroid_x_shifts("sq", rolled_y, ang, nx, ny), s0, atol=1e-9)
+
+
+def test_take_log_closed_form():
+    x = np.array([1.0, np.exp(-2.0), np.exp(3.0), 0.0, -1.5, 0.5], np.float32)
+    out = O.take_log(x)
+    np.testing.assert_allclose(out[:3], [0.0, 2.0, -3.0], atol=2e-7)
+    assert out[3] == 0.0 and out[4] == 0.0
+    assert out[5] == np.float32(-np.log(np.float32(0.5)))
+
+
+def _boundary_image(u):
+    """Moisan's boundary image of u, FP64, written independently."""
+    u = u.astype(np.float64)
+    v = np.zeros_like(u)
+    v[0, :] += u[-1, :] - u[0, :]
+    v[-1, :] += u[0, :] - u[-1, :]
+    v[:, 0] += u[:, -1] - u[:, 0]
+    v[:, -1] += u[:, 0] - u[:, -1]
+    return v
+
+
+@pytest.mark.parametrize("ny,nx", [(10, 12), (7, 9), (16, 16)])
+def test_periodic_smooth_solves_the_periodic_poisson_equation(ny, nx):
+    """The smooth part s solves Lap_per s = v (the boundary image), the
+    equation Moisan's DFT quotient inverts: a wrong sign, a dropped term or
+    a transposed frequency in the denominator breaks it.  s has zero mean."""
+    rng = np.random.default_rng(nx * ny)
+    u = rng.random((ny, nx)).astype(np.float32)
+    s = O.periodic_smooth(u).astype(np.float64)
+    lap = (np.roll(s, 1, 0) + np.roll(s, -1, 0) + np.roll(s, 1, 1) + np.roll(s, -1, 1) - 4 * s)
+    v = _boundary_image(u)
+    np.testing.assert_allclose(lap, v, atol=2e-6 * np.abs(v).max())
+    assert abs(s.mean()) < 1e-6
+
+
+def test_periodic_decomposition_window_and_periodic_input():
+    """An image with no boundary jumps (a constant) is unchanged; otherwise
+    u - out = W(d) s with W = exp(-d^2/50), d the distance to the border."""
+    const = np.full((1, 8, 11), 2.5, np.float32)
+    np.testing.assert_array_equal(O.periodic_decomposition(const), const)
+    rng = np.random.default_rng(4)
+    u = rng.random((2, 9, 13)).astype(np.float32)
+    out = O.periodic_decomposition(u)
+    iy, ix = np.mgrid[0:9, 0:13]
+    d = np.minimum(np.minimum(ix, 12 - ix), np.minimum(iy, 8 - iy)).astype(np.float64)
+    w = np.exp(-d * d / 50.0)
+    for z in range(2):
+        s = O.periodic_smooth(u[z]).astype(np.float64)
+        np.testing.assert_allclose(u[z].astype(np.float64) - out[z], w * s, atol=3e-7)
