@@ -90,6 +90,9 @@ def _load():
         "arec_periodic_decomposition": ([vp, vp, i32, i32, i32], C.c_int),
         "arec_h_take_log": ([vp, i64], C.c_int),
         "arec_h_periodic_decomposition": ([i32, i32, i32, vp], C.c_int),
+        "arec_mrc_read_header": ([C.c_char_p, P(i32), P(i32), P(i32), P(i32)], C.c_int),
+        "arec_mrc_read": ([C.c_char_p, vp, i64], C.c_int),
+        "arec_mrc_write": ([C.c_char_p, i32, i32, i32, vp], C.c_int),
         "arec_geom_get_plan": ([vp, P(i32), P(i32), P(i32)], C.c_int),
         "arec_unet_create": ([vp, P(vp), i32, i32, i32, i32, C.c_uint64], C.c_int),
         "arec_unet_destroy": ([vp], C.c_int),
@@ -130,7 +133,7 @@ EXPORTED = [
     "arec_conv3_check", "arec_conv3_wgrad_check", "arec_cyl_to_cube", "arec_cube_to_cyl", "arec_h_project", "arec_h_backproject",
     "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan", "arec_match", "arec_match_ex", "arec_align_centroid", "arec_take_log", "arec_periodic_decomposition",
     "arec_h_take_log", "arec_h_periodic_decomposition", "arec_scale", "arec_geom_set_forward",
-    "arec_geom_forward_design",
+    "arec_geom_forward_design", "arec_mrc_read_header", "arec_mrc_read", "arec_mrc_write",
 ]
 
 
@@ -391,6 +394,32 @@ def periodic_decomposition(ctx, views):
     """periodicDecomposition of a view stack ([nv, ny, nx] DeviceArray), in place."""
     nv, ny, nx = views.shape
     _check(lib.arec_periodic_decomposition(ctx.h, views.ptr, nx, ny, nv))
+
+
+def read_mrc_header(path):
+    """(nx, ny, nz, mode) of an MRC file (arecReadImageHeader)."""
+    v = [C.c_int32() for _ in range(4)]
+    _check(lib.arec_mrc_read_header(os.fsencode(path), *[C.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+def read_mrc(path):
+    """An MRC file as a float32 array [nz, ny, nx] (arecReadImage: modes 1, 2, 6)."""
+    nx, ny, nz, _ = read_mrc_header(path)
+    out = np.empty((max(nz, 0), max(ny, 0), max(nx, 0)), np.float32)
+    _check(lib.arec_mrc_read(os.fsencode(path), _ptr(out), out.size))
+    return out
+
+
+def write_mrc(path, data):
+    """A float32 array [nz, ny, nx] (or [ny, nx]) as a mode-2 MRC file (arecWriteImage)."""
+    a = np.ascontiguousarray(data, np.float32)
+    if a.ndim == 2:
+        a = a[None]
+    if a.ndim != 3:
+        raise ValueError("write_mrc takes a 2-D or 3-D array")
+    nz, ny, nx = a.shape
+    _check(lib.arec_mrc_write(os.fsencode(path), nx, ny, nz, _ptr(a)))
 
 
 def align_centroid(ctx, g, views):

@@ -272,3 +272,37 @@ void arecImageTakeLog(arecImage image) {
 void periodicDecomposition(arecImage img) {
     arec_h_periodic_decomposition(img.nx, img.ny, img.nz, img.data);
 }
+
+// arecImageIO.cpp:10-31
+int arecReadImageHeader(char *fname, int *nx, int *ny, int *nz) {
+    const int rc = arec_mrc_read_header(fname, nx, ny, nz, nullptr);
+    if (rc) fprintf(stderr, "%s\n", arec_last_error());
+    return rc;
+}
+
+// arecImageIO.cpp:33-82: the buffer is malloc'ed here, as the reference does.
+int arecReadImage(char *fname, arecImage *image) {
+    int nx = 0, ny = 0, nz = 0;
+    int rc = arec_mrc_read_header(fname, &nx, &ny, &nz, nullptr);
+    if (rc == 0) {
+        image->nx = nx;
+        image->ny = ny;
+        image->nz = nz;
+        const int64_t n = (int64_t)nx * ny * nz;
+        image->data = (float *)malloc((size_t)(n > 0 ? n : 1) * sizeof(float));
+        if (!image->data) {
+            fprintf(stderr, "arecReadImage: out of memory for %s\n", fname);
+            return -1;
+        }
+        rc = arec_mrc_read(fname, image->data, n);
+    }
+    if (rc) fprintf(stderr, "%s\n", arec_last_error());
+    return rc;
+}
+
+// arecImageIO.cpp:84-173
+int arecWriteImage(char *fname, arecImage image) {
+    const int rc = arec_mrc_write(fname, image.nx, image.ny, image.nz, image.data);
+    if (rc) fprintf(stderr, "%s\n", arec_last_error());
+    return rc;
+}

@@ -317,6 +317,21 @@ int arec_h_shift_images(int32_t nx, int32_t ny, int32_t nz, float *data, const f
 int arec_h_take_log(float *data, int64_t n);
 int arec_h_periodic_decomposition(int32_t nx, int32_t ny, int32_t nz, float *data);
 
+/* MRC files (arecImageIO.cpp:10-173, header arecImageIO.h:5-63), host only:
+ * a 1024-byte header, then nx*ny*nz values, x fastest, host byte order.
+ * arec_mrc_read_header (arecReadImageHeader) reads the sizes and the mode;
+ * any pointer may be NULL.  arec_mrc_read (arecReadImage) reads mode 2
+ * (float32) and modes 1 and 6 (16-bit, converted as signed) into data, which
+ * holds capacity floats (AREC_ESIZE when the image is larger); -1 for a file
+ * that fails to open or read or has another mode, as the reference.
+ * arec_mrc_write (arecWriteImage) writes mode 2 with the reference's header:
+ * cell 1x1x1 at 90 degrees, axes 1/2/3, amin/amax/amean of the data, "MAP",
+ * rms 1, no labels; the fields the reference leaves unset are zero except
+ * machinestamp, which is the little-endian stamp 0x4444. */
+int arec_mrc_read_header(const char *path, int32_t *nx, int32_t *ny, int32_t *nz, int32_t *mode);
+int arec_mrc_read(const char *path, float *data, int64_t capacity);
+int arec_mrc_write(const char *path, int32_t nx, int32_t ny, int32_t nz, const float *data);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,4 +78,10 @@ void arecShiftImages(arecImage images, float *sx, float *sy);
 void arecImageTakeLog(arecImage image);
 void periodicDecomposition(arecImage img);
 
+/* arecImageIO.h:64-66 (arecImageIO.cpp:10-173): MRC files.  arecReadImage
+ * mallocs image->data (free with arecImageFree) and sets nx, ny, nz. */
+int arecReadImageHeader(char *fname, int *nx, int *ny, int *nz);
+int arecReadImage(char *fname, arecImage *image);
+int arecWriteImage(char *fname, arecImage image);
+
 #endif

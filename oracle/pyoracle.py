@@ -266,6 +266,67 @@ def periodic_decomposition(stack):
     return out
 
 
+# MRC header: the reference's MRCheader (AREC3D/AREC3D/arecImageIO.h:5-63) as
+# (name, struct code) in file order; 1024 bytes, host (little-endian) order.
+MRC_FIELDS = [("nx", "i"), ("ny", "i"), ("nz", "i"), ("mode", "i"), ("nxstart", "i"), ("nystart", "i"),
+              ("nzstart", "i"), ("mx", "i"), ("my", "i"), ("mz", "i"), ("xlen", "f"), ("ylen", "f"),
+              ("zlen", "f"), ("alpha", "f"), ("beta", "f"), ("gamma", "f"), ("mapc", "i"), ("mapr", "i"),
+              ("maps", "i"), ("amin", "f"), ("amax", "f"), ("amean", "f"), ("ispg", "i"), ("nsymbt", "i"),
+              ("user", "25i"), ("xorigin", "f"), ("yorigin", "f"), ("zorigin", "f"), ("map", "4s"),
+              ("machinestamp", "i"), ("rms", "f"), ("nlabels", "i"), ("labels", "800s")]
+MRC_FORMAT = "<" + "".join(c for _, c in MRC_FIELDS)
+
+
+def mrc_header(nx, ny, nz, data):
+    """The header arecWriteImage writes (arecImageIO.cpp:84-147): mode 2, cell
+    1x1x1 at 90 degrees, axes 1/2/3, min/max/mean of the data accumulated in
+    double with the mean divided by the float count, user[] spaces with a zero
+    last word, "MAP", rms 1, no labels.  The reference leaves machinestamp and
+    labels unset; here they are zero."""
+    import struct
+
+    d = np.asarray(data, np.float32).reshape(-1)
+    amin, amax, mean = 1.0e20, -1.0e20, 0.0
+    for v in d.astype(np.float64):
+        amax = v if v > amax else amax
+        amin = v if v < amin else amin
+        mean += v
+    mean = mean / float(np.float32(nx * ny * nz))
+    user = [ord(" ")] * 24 + [0]
+    return struct.pack(MRC_FORMAT, nx, ny, nz, 2, 0, 0, 0, nx, ny, nz, 1.0, 1.0, 1.0, 90.0, 90.0, 90.0, 1, 2, 3,
+                       amin, amax, mean, 0, 0, *user, 0.0, 0.0, 0.0, b"MAP\0", 0, 1.0, 0, b"")
+
+
+def mrc_parse_header(raw):
+    """The 1024-byte header as a dict of the fields above."""
+    import struct
+
+    vals = list(struct.unpack(MRC_FORMAT, raw[:1024]))
+    out = {}
+    for name, code in MRC_FIELDS:
+        if code == "25i":
+            out[name], vals = vals[:25], vals[25:]
+        else:
+            out[name], vals = vals[0], vals[1:]
+    return out
+
+
+def mrc_read(path):
+    """arecReadImage (arecImageIO.cpp:33-82): modes 1 and 6 as signed 16-bit,
+    mode 2 as float32, any other mode an error; [nz, ny, nx] float32."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    h = mrc_parse_header(raw)
+    n = h["nx"] * h["ny"] * h["nz"]
+    if h["mode"] in (1, 6):
+        v = np.frombuffer(raw, "<i2", count=n, offset=1024).astype(np.float32)
+    elif h["mode"] == 2:
+        v = np.frombuffer(raw, "<f4", count=n, offset=1024).copy()
+    else:
+        raise ValueError(f"invalid mrc mode {h['mode']}")
+    return v.reshape(h["nz"], h["ny"], h["nx"])
+
+
 def soft_noneg(x):
     out = np.array(x, np.float32, copy=True)
     lib().orc_soft_noneg(out.reshape(-1), out.size)
@@ -313,3 +374,23 @@ def ref_solve(solver, method, b, angles, lam, maxit, tol, x0=None):
                          np.ascontiguousarray(angles, np.float32), lam, maxit, tol, 0 if x0 is None else 1, x,
                          C.byref(nnz))
     return st, x.reshape(-1)[: nnz.value].reshape(-1, ny)
+
+
+def ref_mrc_write(path, data):
+    """arecWriteImage of a [nz, ny, nx] float32 array through the reference build."""
+    R = ref()
+    a = np.ascontiguousarray(data, np.float32)
+    nz, ny, nx = a.shape
+    R.ref_mrc_write.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _F]
+    return R.ref_mrc_write(os.fsencode(path), nx, ny, nz, a)
+
+
+def ref_mrc_read(path, capacity):
+    """arecReadImage through the reference build: (status, [nz, ny, nx] array)."""
+    R = ref()
+    R.ref_mrc_read.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _F,
+                               C.c_long]
+    nx, ny, nz = C.c_int(0), C.c_int(0), C.c_int(0)
+    buf = np.zeros(capacity, np.float32)
+    st = R.ref_mrc_read(os.fsencode(path), C.byref(nx), C.byref(ny), C.byref(nz), buf, capacity)
+    return st, buf[: nx.value * ny.value * nz.value].reshape(nz.value, ny.value, nx.value)

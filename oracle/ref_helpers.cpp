@@ -7,8 +7,10 @@
 //    AREC3D/AREC3D/arecImage.cpp:15-27, 29-86, 234-241 and 796-810; the
 //    cylinder mask is the same as orc_cyl_geometry in arec_oracle.c.
 // 2. extern "C" wrappers give ctypes a flat interface to the reference's
-//    arecProject2D*/arecBackProject2D*, cyl_sirt* and cyl_cgls*.
+//    arecProject2D*/arecBackProject2D*, cyl_sirt*, cyl_cgls* and the MRC
+//    reader and writer (arecReadImage / arecWriteImage).
 #include "arecImage.h"
+#include "arecImageIO.h"
 #include "arecproject.h"
 #include "areccgls.h"
 #include "areccgls_KB.h"
@@ -198,6 +200,31 @@ int ref_solve(int solver, int method, const float *b, int nx, int ny, int na,
     if (xv.data) arecImageFree(&xv);
     arecImageFree(&img);
     free(ang);
+    return st;
+}
+
+// arecWriteImage of a [nz][ny][nx] float array.
+int ref_mrc_write(const char *path, int nx, int ny, int nz, const float *data) {
+    arecImage img;
+    memset(&img, 0, sizeof(img));
+    img.nx = nx;
+    img.ny = ny;
+    img.nz = nz;
+    img.data = (float *)data;
+    return arecWriteImage((char *)path, img);
+}
+
+// arecReadImage into data (capacity floats); sizes into nx, ny, nz.
+int ref_mrc_read(const char *path, int *nx, int *ny, int *nz, float *data, long capacity) {
+    arecImage img;
+    memset(&img, 0, sizeof(img));
+    int st = arecReadImage((char *)path, &img);
+    *nx = img.nx;
+    *ny = img.ny;
+    *nz = img.nz;
+    const long n = (long)img.nx * img.ny * img.nz;
+    if (st == 0 && n <= capacity && img.data) memcpy(data, img.data, sizeof(float) * n);
+    free(img.data);
     return st;
 }
 
