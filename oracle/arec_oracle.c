@@ -532,3 +532,264 @@ void orc_soft_noneg(float *x, long n) {
     for (long i = 0; i < n; i++)
         if (x[i] < 0.0f) x[i] *= 0.5f;
 }
+
+/* ------------------------------------------------------------------ */
+/* Rotational alignment                                                */
+/* ------------------------------------------------------------------ */
+
+/* getRadialProfile with getCoord_blerp_raw, arecImage.cpp:516-553: for each
+ * of `length` directions alpha_i = 2 pi i / length about the view centre
+ * ((nx-1)/2, (ny-1)/2), out[i] = sum over r = 0..R-1 of r times the bilinear
+ * sample at radius r.  Coordinates are rounded to float, fractions and the
+ * blend are double, the sample is rounded to float and the sum is float, as
+ * in the reference. */
+void orc_radial_profile(int nx, int ny, const float *img, int length, int R, float *out) {
+    const double pi = 4.0 * atan(1.0), xc = 0.5 * (nx - 1), yc = 0.5 * (ny - 1);
+    for (int i = 0; i < length; i++) {
+        const double alpha = 2 * pi * i / length, ca = cos(alpha), sa = sin(alpha);
+        float acc = 0.0f;
+        for (int r = 0; r < R; r++) {
+            const float x = (float)(xc + ca * r), y = (float)(yc + sa * r);
+            const int ix = (int)x, iy = (int)y;
+            double fx = (double)(x - (float)ix), fy = (double)(y - (float)iy);
+            if (fx < 0.0) fx = 0.0;
+            if (fy < 0.0) fy = 0.0;
+            const float *row = img + (size_t)iy * nx;
+            const double v00 = row[ix], v10 = row[ix + 1], v01 = row[nx + ix], v11 = row[nx + ix + 1];
+            const double top = v01 + fx * (v11 - v01), bot = v00 + fx * (v10 - v00);
+            const float v = (float)(bot + fy * (top - bot));
+            acc += v * (float)r;
+        }
+        out[i] = acc;
+    }
+}
+
+/* ccorr1d, align2dstack.cpp:437-475: the unnormalised inverse transform of
+ * conj(DFT x) * DFT y, i.e. c[k] = n * sum_m x[m] y[(m + k) mod n], summed
+ * here directly in double (m ascending). */
+void orc_ccorr1d(int n, const float *x, const float *y, double *c) {
+    for (int k = 0; k < n; k++) {
+        double acc = 0.0;
+        for (int m = 0; m < n; m++) acc += (double)x[m] * (double)y[(m + k) % n];
+        c[k] = (double)n * acc;
+    }
+}
+
+/* findPeak1d with PRB1D, arecImage.cpp:489-514 and align2dstack.cpp:1028-1054:
+ * the first strict maximum, then the vertex of the least-squares parabola
+ * through the 7 circular neighbours (offset C2 / (2 C3) - 4 from the first
+ * of them), returned as float. */
+float orc_findpeak1d(int n, const float *c) {
+    int best = 0;
+    for (int i = 1; i < n; i++)
+        if (c[i] > c[best]) best = i;
+    double b[7];
+    for (int k = -3; k <= 3; k++) b[k + 3] = (double)c[(best + k + n) % n];
+    const double c2 = 49. * b[0] + 6. * b[1] - 21. * b[2] - 32. * b[3] - 27. * b[4] - 6. * b[5] + 31. * b[6];
+    const double c3 = 5. * b[0] - 3. * b[2] - 4. * b[3] - 3. * b[4] + 5. * b[6];
+    const double pos = c3 != 0.0 ? c2 / (2. * c3) - 4 : 0.0;
+    return (float)(pos + best);
+}
+
+/* arecRotCCImages, arecImage.cpp:555-599: per view, the radial profiles of
+ * both stacks (R = r2, length (int)(2 pi r2)), their 1-D correlation stored
+ * as float, its peak as an angle in degrees wrapped to (-180, 180]; angles
+ * beyond 45 degrees are taken as a bad correlation and set to 0. */
+void orc_rot_cc_images(int nx, int ny, int nz, const float *img1, const float *img2, int r2, float *angles) {
+    const double pi = 4.0 * atan(1.0);
+    const int n = (int)(r2 * 2 * pi);
+    float *p1 = (float *)malloc(sizeof(float) * n), *p2 = (float *)malloc(sizeof(float) * n);
+    float *cf = (float *)malloc(sizeof(float) * n);
+    double *cd = (double *)malloc(sizeof(double) * n);
+    for (int v = 0; v < nz; v++) {
+        orc_radial_profile(nx, ny, img1 + (size_t)v * nx * ny, n, r2, p1);
+        orc_radial_profile(nx, ny, img2 + (size_t)v * nx * ny, n, r2, p2);
+        orc_ccorr1d(n, p1, p2, cd);
+        for (int k = 0; k < n; k++) cf[k] = (float)cd[k];
+        float a = orc_findpeak1d(n, cf);
+        a = (float)(a * (2 * pi / n));
+        if (a > pi) a = (float)(a - 2.0 * pi);
+        a = (float)(a * (180.0 / pi));
+        if (fabs(a) > 45) a = 0.0f;
+        angles[v] = a;
+    }
+    free(p1);
+    free(p2);
+    free(cf);
+    free(cd);
+}
+
+/* Fractional pass of shearX_circ / shearY_circ (imagetools.cpp:244-291) on
+ * one line of n samples at stride s, after its circular shift by k:
+ * r[i] = line[(i - k) mod n], out[i] = (r[i] - r[i] f) + r[i-1 mod n] f. */
+static void orc_shear_line(float *line, int n, long s, int k, float f, float *tmp) {
+    for (int i = 0; i < n; i++) {
+        int j = (int)(((long)i - k) % n);
+        if (j < 0) j += n;
+        tmp[i] = line[(long)j * s];
+    }
+    for (int i = 0; i < n; i++) {
+        const float r = tmp[i], l = tmp[(i + n - 1) % n];
+        line[(long)i * s] = (r - r * f) + l * f;
+    }
+}
+
+/* shearX_circ: row y moves by shear * (y - (ny-1)/2) along x, circularly. */
+void orc_shear_x(float *img, float shear, int nx, int ny) {
+    const double center = 0.5 * (ny - 1);
+    float *tmp = (float *)malloc(sizeof(float) * nx);
+    for (int y = 0; y < ny; y++) {
+        const float skew = (float)(shear * (y - center));
+        const int k = (int)floorf(skew);
+        const float f = (float)(skew - 1.0 * k);
+        orc_shear_line(img + (size_t)y * nx, nx, 1, k, f, tmp);
+    }
+    free(tmp);
+}
+
+/* shearY_circ: column x moves by shear * (x - (nx-1)/2) along y, circularly. */
+void orc_shear_y(float *img, float shear, int nx, int ny) {
+    const double center = 0.5 * (nx - 1);
+    float *tmp = (float *)malloc(sizeof(float) * ny);
+    for (int x = 0; x < nx; x++) {
+        const float skew = (float)(shear * (x - center));
+        const int k = (int)floorf(skew);
+        const float f = (float)(skew - 1.0 * k);
+        orc_shear_line(img + x, ny, nx, k, f, tmp);
+    }
+    free(tmp);
+}
+
+/* The three-shear rotation of arecRotateImages_skew(_safe), arecImage.cpp:
+ * 733-736 and 766-770: shears tan(theta/2), -sin(theta), tan(theta/2), with
+ * theta in radians held as a float, the tangent in double and the sine the
+ * float function's. */
+static void orc_rotate_shears(float *img, int nx, int ny, float deg) {
+    const double pi = 4.0 * atan(1.0);
+    const float rad = (float)(deg * pi / 180.0);
+    const float a = (float)(-tan(-0.5 * rad)), b = sinf(-rad);
+    orc_shear_x(img, a, nx, ny);
+    orc_shear_y(img, b, nx, ny);
+    orc_shear_x(img, a, nx, ny);
+}
+
+/* rotate2d, align2dstack.cpp:774-822: bilinear rotation by deg about the
+ * 1-based centre (nx/2 + 1, ny/2 + 1); samples whose source cell leaves the
+ * image are 0.  The angle is float and its cosine and sine are the float
+ * functions' (the reference's argument is a float). */
+void orc_rotate2d(int nx, int ny, const float *in, float *out, float deg) {
+    const double pi = 4.0 * atan(1.0);
+    float th = (float)(deg * pi / 180.0);
+    if (th > pi) th = (float)(-2.0 * pi + th);
+    if (th < -pi) th = (float)(2. * pi + th);
+    const double cod = cosf(th), sid = sinf(th);
+    const int kc = nx / 2 + 1, ic = ny / 2 + 1;
+    for (int i = 1; i <= ny; i++) {
+        const double y = i - ic, ycod = y * cod + ic, ysid = -y * sid + kc;
+        for (int k = 1; k <= nx; k++) {
+            const double x = k - kc, xold = x * cod + ysid, yold = x * sid + ycod;
+            const int iy = (int)yold, ix = (int)xold;
+            float v = 0.0f;
+            if (iy >= 1 && iy <= ny - 1 && ix >= 1 && ix <= nx - 1) {
+                const double yd = yold - iy, yr = 1.0 - yd, xd = xold - ix, xr = 1.0 - xd;
+                const long a = (long)(iy - 1) * nx + ix - 1; /* 0-based (ix, iy) */
+                v = (float)(yd * (in[a + nx] * xr + in[a + nx + 1] * xd) + yr * (in[a] * xr + in[a + 1] * xd));
+            }
+            out[(size_t)(i - 1) * nx + (k - 1)] = v;
+        }
+    }
+}
+
+/* arecRotateImages_skew, arecImage.cpp:693-741: each view rotated by its
+ * angle, which is first wrapped by 360 into [-180, 180]; beyond 45 degrees
+ * it is set to 0 (no rotation), beyond 10 the bilinear rotate2d is used,
+ * else the three shears.  The angles array is updated as the reference
+ * does. */
+void orc_rotate_skew(int nx, int ny, int nz, float *data, float *angles) {
+    const size_t n = (size_t)nx * ny;
+    float *tmp = (float *)malloc(sizeof(float) * n);
+    for (int v = 0; v < nz; v++) {
+        float *img = data + v * n;
+        if (angles[v] < -180) angles[v] += 360;
+        if (angles[v] > 180) angles[v] -= 360;
+        if (fabs(angles[v]) > 45) {
+            angles[v] = 0.0f;
+        } else if (fabs(angles[v]) > 10) {
+            orc_rotate2d(nx, ny, img, tmp, angles[v]);
+            memcpy(img, tmp, sizeof(float) * n);
+        } else {
+            orc_rotate_shears(img, nx, ny, angles[v]);
+        }
+    }
+    free(tmp);
+}
+
+/* tukey_filter on a unit image, imagetools.cpp:332-367: the float product
+ * of the two 1-D windows, each 0.5 (1 + cos(2 pi / alpha (t - alpha/2)))
+ * on the rising edge t < alpha/2, the mirrored form on the falling edge,
+ * 1 between (t = x / (n-1)). */
+static double orc_tukey_1d(int i, int n, double alpha) {
+    const double pi = 4.0 * atan(1.0), A = 2.0 * pi / alpha;
+    const double ta = 0.5 * alpha * (n - 1), tb = (1.0 - 0.5 * alpha) * (n - 1);
+    if (i < ta) return 0.5 * (1.0 + cos(A * (1.0 * i / (n - 1) - alpha / 2)));
+    if (i > tb) return 0.5 * (1.0 + cos(A * (1.0 * i / (n - 1) - 1.0 + alpha / 2)));
+    return 1.0;
+}
+
+/* getNormalizedCrossCorrelationWithFilter, imagetools.cpp:415-447: the
+ * correlation coefficient of the two views times the Tukey window
+ * (alpha 0.3), means and sums in double. */
+double orc_tukey_ncc(const float *a, const float *b, int nx, int ny) {
+    const size_t n = (size_t)nx * ny;
+    float *w = (float *)malloc(sizeof(float) * n);
+    for (int y = 0; y < ny; y++) {
+        const double hy = orc_tukey_1d(y, ny, 0.3);
+        for (int x = 0; x < nx; x++) w[(size_t)y * nx + x] = (float)(1.0 * orc_tukey_1d(x, nx, 0.3) * hy);
+    }
+    double ma = 0.0, mb = 0.0, cov = 0.0, va = 0.0, vb = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        ma += a[i] * w[i];
+        mb += b[i] * w[i];
+    }
+    ma /= 1.0 * nx * ny;
+    mb /= 1.0 * nx * ny;
+    for (size_t i = 0; i < n; i++) {
+        const double da = a[i] * w[i] - ma, db = b[i] * w[i] - mb;
+        cov += da * db;
+        va += da * da;
+        vb += db * db;
+    }
+    free(w);
+    return cov / sqrt(va * vb);
+}
+
+/* arecRotateImages_skew_safe, arecImage.cpp:743-794: each view rotated by
+ * its angle with the three shears; kept when its windowed correlation with
+ * the reference view rises by more than 1e-5, and then its angle is halved
+ * (the reference's bookkeeping); else restored, with angle 0.  Returns the
+ * number of views kept; *mean_cc gets the mean of the chosen correlations. */
+int orc_rotate_skew_safe(int nx, int ny, int nz, float *data, const float *ref, float *angles, double *mean_cc) {
+    const size_t n = (size_t)nx * ny;
+    float *backup = (float *)malloc(sizeof(float) * n);
+    double sum = 0.0;
+    int kept = 0;
+    for (int v = 0; v < nz; v++) {
+        float *img = data + v * n;
+        memcpy(backup, img, sizeof(float) * n);
+        orc_rotate_shears(img, nx, ny, angles[v]);
+        const double c1 = orc_tukey_ncc(img, ref + v * n, nx, ny);
+        const double c2 = orc_tukey_ncc(backup, ref + v * n, nx, ny);
+        if (c2 + 0.00001 < c1) {
+            sum += c1;
+            kept++;
+            angles[v] *= 0.5;
+        } else {
+            memcpy(img, backup, sizeof(float) * n);
+            angles[v] = 0.0f;
+            sum += c2;
+        }
+    }
+    free(backup);
+    if (mean_cc) *mean_cc = nz > 0 ? sum / nz : 0.0;
+    return kept;
+}

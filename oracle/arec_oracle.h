@@ -104,6 +104,21 @@ void orc_take_log(float *x, long n);
 void orc_periodic_smooth(int nx, int ny, const float *u, float *s);
 void orc_periodic_decomposition(int nx, int ny, int nz, float *data);
 
+/* Rotational alignment: AREC3D/AREC3D/arecImage.cpp:489-599 (radial profiles,
+ * 1-D correlation peak, angles), :693-794 (rotations), imagetools.cpp:
+ * 244-291 (shears), :332-367, 415-447 (windowed correlation coefficient),
+ * align2dstack.cpp:437-475 (ccorr1d), :774-822 (rotate2d). */
+void orc_radial_profile(int nx, int ny, const float *img, int length, int R, float *out);
+void orc_ccorr1d(int n, const float *x, const float *y, double *c);
+float orc_findpeak1d(int n, const float *c);
+void orc_rot_cc_images(int nx, int ny, int nz, const float *img1, const float *img2, int r2, float *angles);
+void orc_shear_x(float *img, float shear, int nx, int ny);
+void orc_shear_y(float *img, float shear, int nx, int ny);
+void orc_rotate2d(int nx, int ny, const float *in, float *out, float deg);
+void orc_rotate_skew(int nx, int ny, int nz, float *data, float *angles);
+double orc_tukey_ncc(const float *a, const float *b, int nx, int ny);
+int orc_rotate_skew_safe(int nx, int ny, int nz, float *data, const float *ref, float *angles, double *mean_cc);
+
 #ifdef __cplusplus
 }
 #endif

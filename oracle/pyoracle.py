@@ -69,6 +69,19 @@ def lib():
         L.orc_take_log.argtypes = [_F, C.c_long]
         L.orc_periodic_smooth.argtypes = [C.c_int, C.c_int, _F, _F]
         L.orc_periodic_decomposition.argtypes = [C.c_int, C.c_int, C.c_int, _F]
+        L.orc_radial_profile.argtypes = [C.c_int, C.c_int, _F, C.c_int, C.c_int, _F]
+        L.orc_ccorr1d.argtypes = [C.c_int, _F, _F, _D]
+        L.orc_findpeak1d.argtypes = [C.c_int, _F]
+        L.orc_findpeak1d.restype = C.c_float
+        L.orc_rot_cc_images.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, C.c_int, _F]
+        L.orc_shear_x.argtypes = [_F, C.c_float, C.c_int, C.c_int]
+        L.orc_shear_y.argtypes = [_F, C.c_float, C.c_int, C.c_int]
+        L.orc_rotate2d.argtypes = [C.c_int, C.c_int, _F, _F, C.c_float]
+        L.orc_rotate_skew.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F]
+        L.orc_tukey_ncc.argtypes = [_F, _F, C.c_int, C.c_int]
+        L.orc_tukey_ncc.restype = C.c_double
+        L.orc_rotate_skew_safe.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, _F, C.POINTER(C.c_double)]
+        L.orc_rotate_skew_safe.restype = C.c_int
         _lib = L
     return _lib
 
@@ -264,6 +277,88 @@ def periodic_decomposition(stack):
     out = np.array(stack, np.float32, copy=True)
     lib().orc_periodic_decomposition(nx, ny, nz, out.reshape(-1))
     return out
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def rot_r2(nx, ny):
+    """The profile radius of the driver's rotational step (AREC3D.cpp:237-238
+    with radius (nx-1)/2 and height ny): min(nx, ny - 10) // 2."""
+    return min(nx, ny - 10) // 2
+
+
+def radial_profile(img, length, R):
+    """getRadialProfile of one view [ny, nx]."""
+    ny, nx = img.shape
+    out = np.zeros(length, np.float32)
+    lib().orc_radial_profile(nx, ny, _f32(img), length, R, out)
+    return out
+
+
+def ccorr1d(x, y):
+    """ccorr1d: c[k] = n sum_m x[m] y[(m+k) mod n] (FP64)."""
+    c = np.zeros(len(x), np.float64)
+    lib().orc_ccorr1d(len(x), _f32(x), _f32(y), c)
+    return c
+
+
+def findpeak1d(c):
+    return float(lib().orc_findpeak1d(len(c), _f32(c)))
+
+
+def rot_cc_images(img1, img2, r2):
+    """arecRotCCImages: per-view rotation angles (degrees) of img2 against img1."""
+    nz, ny, nx = img1.shape
+    ang = np.zeros(nz, np.float32)
+    lib().orc_rot_cc_images(nx, ny, nz, _f32(img1), _f32(img2), r2, ang)
+    return ang
+
+
+def shear_x(img, shear):
+    out = np.array(img, np.float32, copy=True)
+    ny, nx = out.shape
+    lib().orc_shear_x(out.reshape(-1), shear, nx, ny)
+    return out
+
+
+def shear_y(img, shear):
+    out = np.array(img, np.float32, copy=True)
+    ny, nx = out.shape
+    lib().orc_shear_y(out.reshape(-1), shear, nx, ny)
+    return out
+
+
+def rotate2d(img, deg):
+    ny, nx = img.shape
+    out = np.zeros((ny, nx), np.float32)
+    lib().orc_rotate2d(nx, ny, _f32(img), out, deg)
+    return out
+
+
+def rotate_skew(stack, angles):
+    """arecRotateImages_skew on a copy: (rotated stack, updated angles)."""
+    out = np.array(stack, np.float32, copy=True)
+    ang = np.array(angles, np.float32, copy=True)
+    nz, ny, nx = out.shape
+    lib().orc_rotate_skew(nx, ny, nz, out.reshape(-1), ang)
+    return out, ang
+
+
+def tukey_ncc(a, b):
+    ny, nx = a.shape
+    return float(lib().orc_tukey_ncc(_f32(a), _f32(b), nx, ny))
+
+
+def rotate_skew_safe(stack, ref, angles):
+    """arecRotateImages_skew_safe on a copy: (stack, angles, kept, mean cc)."""
+    out = np.array(stack, np.float32, copy=True)
+    ang = np.array(angles, np.float32, copy=True)
+    nz, ny, nx = out.shape
+    m = C.c_double(0.0)
+    kept = lib().orc_rotate_skew_safe(nx, ny, nz, out.reshape(-1), _f32(ref), ang, C.byref(m))
+    return out, ang, kept, m.value
 
 
 # MRC header: the reference's MRCheader (AREC3D/AREC3D/arecImageIO.h:5-63) as
