@@ -84,7 +84,13 @@ def _load():
         "arec_geom_set_forward": ([vp, vp, i32, i32, i32], C.c_int),
         "arec_geom_forward_design": ([vp], i32),
         "arec_match": ([vp, vp, vp, vp, flt, i32, i32, flt, i32, i32, vp, vp, P(Hist)], C.c_int),
-        "arec_match_ex": ([vp, vp, vp, vp, flt, i32, i32, flt, i32, i32, i32, vp, vp, P(Hist)], C.c_int),
+        "arec_match_ex": ([vp, vp, vp, vp, flt, i32, i32, flt, i32, i32, i32, vp, vp, vp, P(Hist)], C.c_int),
+        "arec_rot_cc": ([vp, vp, vp, i32, i32, i32, i32, vp], C.c_int),
+        "arec_rotate_views": ([vp, vp, i32, i32, i32, vp, i32], C.c_int),
+        "arec_rotate_views_safe": ([vp, vp, vp, i32, i32, i32, vp, P(i32), P(dbl)], C.c_int),
+        "arec_h_rot_cc_images": ([i32, i32, i32, vp, i32, i32, i32, vp, i32, vp], C.c_int),
+        "arec_h_rotate_images": ([i32, i32, i32, vp, vp, i32], C.c_int),
+        "arec_h_rotate_images_safe": ([i32, i32, i32, vp, vp, vp, P(i32), P(dbl)], C.c_int),
         "arec_align_centroid": ([vp, vp, vp, vp], C.c_int),
         "arec_take_log": ([vp, vp, i64], C.c_int),
         "arec_periodic_decomposition": ([vp, vp, i32, i32, i32], C.c_int),
@@ -134,6 +140,8 @@ EXPORTED = [
     "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan", "arec_match", "arec_match_ex", "arec_align_centroid", "arec_take_log", "arec_periodic_decomposition",
     "arec_h_take_log", "arec_h_periodic_decomposition", "arec_scale", "arec_geom_set_forward",
     "arec_geom_forward_design", "arec_mrc_read_header", "arec_mrc_read", "arec_mrc_write",
+    "arec_rot_cc", "arec_rotate_views", "arec_rotate_views_safe", "arec_h_rot_cc_images", "arec_h_rotate_images",
+    "arec_h_rotate_images_safe",
 ]
 
 
@@ -370,19 +378,58 @@ def sirt_batch(ctx, geoms, bs, xs, lams, maxit, tol, hist_cap=None):
     return out
 
 
-def match(ctx, g, views, x, lam, outer=5, maxit=20, tol=0.0, cc_passes=3, method="sirt", x_from="cc"):
+def match(ctx, g, views, x, lam, outer=5, maxit=20, tol=0.0, cc_passes=3, method="sirt", x_from="cc",
+          rotate=False):
     """Projection matching (arec_match / arec_match_ex): returns (sx, sy,
-    last history line of each outer loop's reconstruction).  x_from="cc"
-    takes every shift from the cross-correlation passes (the reference's
-    loop); x_from="centroid" takes the x shifts from the views' centroids."""
+    last history line of each outer loop's reconstruction), and the summed
+    rotation angles (degrees) as a fourth item when rotate=True (the
+    driver's align = 2).  x_from="cc" takes every shift from the
+    cross-correlation passes (the reference's loop); x_from="centroid" takes
+    the x shifts from the views' centroids."""
     sx = np.zeros(g.na, np.float32)
     sy = np.zeros(g.na, np.float32)
+    rot = np.zeros(g.na, np.float32)
     H = (Hist * max(1, outer))()
     m = {"sirt": 1, "cgls": 2, "sirt_warm": 3}[method]
-    flags = {"cc": 0, "centroid": 1}[x_from]
+    flags = {"cc": 0, "centroid": 1}[x_from] | (2 if rotate else 0)
     _check(lib.arec_match_ex(ctx.h, g.h, views.ptr, x.ptr, lam, outer, maxit, tol, cc_passes, m, flags, _ptr(sx),
-                             _ptr(sy), H))
-    return sx, sy, _hist(H, outer)
+                             _ptr(sy), _ptr(rot), H))
+    return (sx, sy, _hist(H, outer), rot) if rotate else (sx, sy, _hist(H, outer))
+
+
+ROT_RULES = {"skew": 0, "bilinear": 1}
+
+
+def rot_r2(nx, ny):
+    """The driver's profile radius for the rotational step: min(nx, ny - 10) // 2."""
+    return min(nx, ny - 10) // 2
+
+
+def rot_cc(ctx, images1, images2, r2):
+    """arecRotCCImages on [nv, ny, nx] DeviceArrays: angles (degrees) of images2 against images1."""
+    nv, ny, nx = images1.shape
+    ang = np.zeros(nv, np.float32)
+    _check(lib.arec_rot_cc(ctx.h, images1.ptr, images2.ptr, nx, ny, nv, r2, _ptr(ang)))
+    return ang
+
+
+def rotate_views(ctx, views, angles, rule="skew"):
+    """arecRotateImages_skew (rule "skew") or arecRotateImages ("bilinear")
+    on a DeviceArray, in place; returns the angles as the reference leaves them."""
+    nv, ny, nx = views.shape
+    ang = np.array(angles, np.float32, copy=True)
+    _check(lib.arec_rotate_views(ctx.h, views.ptr, nx, ny, nv, _ptr(ang), ROT_RULES[rule]))
+    return ang
+
+
+def rotate_views_safe(ctx, views, ref, angles):
+    """arecRotateImages_skew_safe on DeviceArrays (views in place): (angles, kept, mean cc)."""
+    nv, ny, nx = views.shape
+    ang = np.array(angles, np.float32, copy=True)
+    kept, mcc = C.c_int32(0), C.c_double(0.0)
+    _check(lib.arec_rotate_views_safe(ctx.h, views.ptr, ref.ptr, nx, ny, nv, _ptr(ang), C.byref(kept),
+                                      C.byref(mcc)))
+    return ang, kept.value, mcc.value
 
 
 def take_log(ctx, data):

@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libarec3d_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["geometry.cpp", "capi.cpp", "projector.hip", "solver.hip", "align.hip", "phantom.hip", "unet.hip",
-           "matching.cpp", "mrc.cpp"]
+           "matching.cpp", "mrc.cpp", "rotate.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -223,13 +223,22 @@ int periodic_views(float *data, int nx, int ny, int nv, hipStream_t st);
 // Per-view moments {sum p, sum u p, sum y p} (FP64, deterministic) into
 // mom_dev[3 * nv].
 int view_moments(const float *views, int nx, int ny, int nv, double *mom_dev, hipStream_t st);
+// Rotational alignment (rotate.hip); angles are host arrays, in degrees.
+int rot_check(int nx, int ny, int nv, const char *what);
+int rot_cc_views(const float *img1, const float *img2, int nx, int ny, int nv, int r2, float *angles,
+                 hipStream_t st);
+int rotate_views(float *views, int nx, int ny, int nv, float *angles, int rule, hipStream_t st);
+int rotate_views_safe(float *views, const float *ref, int nx, int ny, int nv, float *angles, int *kept,
+                      double *mean_cc, hipStream_t st);
 // Projection matching (matching.cpp).  flags bit 0: x shifts from the views'
-// centroids (centroid_x_shifts) instead of cross-correlation.
-constexpr int kMatchXCentroid = 1;
+// centroids (centroid_x_shifts) instead of cross-correlation; bit 1: the
+// rotational step every third outer loop (the driver's align = 2).
+constexpr int kMatchXCentroid = AREC_MATCH_X_CENTROID;
+constexpr int kMatchRotate = AREC_MATCH_ROTATE;
 int centroid_x_shifts(const Geometry &g, hipStream_t st, const float *views, float *sx_host);
 int match_run(const Geometry &g, hipStream_t st, const float *views0, float *x, float lam, int outer,
               int maxit, float tol, int cc_passes, int method, int flags, float *sx_host, float *sy_host,
-              arec_hist *loop_hist);
+              float *rot_host, arec_hist *loop_hist);
 
 // Fill with a constant by a kernel (graph-capture safe); skip as above.
 int fill(float *x, float v, int64_t n, hipStream_t st, const int *skip);

@@ -260,6 +260,60 @@ void arecShiftImages(arecImage images, float *sx, float *sy) {
     arec_h_shift_images(images.nx, images.ny, images.nz, images.data, sx, sy);
 }
 
+// ---- rotational alignment -------------------------------------------------------
+
+// arecImage.cpp:555-599: per-view angles (degrees) of images2 against images1.
+int arecRotCCImages(MPI_Comm comm, arecImage images1, arecImage images2, int r2, float *angles) {
+    (void)comm;
+    int rc = arec_h_rot_cc_images(images1.nx, images1.ny, images1.nz, images1.data, images2.nx, images2.ny,
+                                  images2.nz, images2.data, r2, angles);
+    if (rc == -1) {
+        fprintf(stderr, "arecRotCCImages2: mismatch in dimension:\n");
+        fprintf(stderr, "(%d,%d,%d) (%d,%d,%d)\n", images1.nx, images1.ny, images1.nz, images2.nx, images2.ny,
+                images2.nz);
+    } else if (rc) {
+        fprintf(stderr, "%s\n", arec_last_error());
+    }
+    return rc;
+}
+
+// arecImage.cpp:673-691: rotate2d for every view.
+void arecRotateImages(arecImage *images, float *angles) {
+    if (arec_h_rotate_images(images->nx, images->ny, images->nz, images->data, angles, AREC_ROT_BILINEAR))
+        fprintf(stderr, "%s\n", arec_last_error());
+}
+
+// arecImage.cpp:693-741, with the reference's messages for the angles it
+// wraps, drops or hands to the bilinear rotation.
+void arecRotateImages_skew(arecImage *images, float *angles) {
+    for (int j = 0; j < images->nz; j++) {
+        float a = angles[j];
+        if (a < -180 || a > 180) {
+            const float b = a < -180 ? a + 360 : a - 360;
+            printf("Angle changed from %4.2f to %4.2f \n", a, b);
+            a = b;
+        }
+        if (std::fabs(a) > 45)
+            printf("Angle %4.2f too large, probably bad CC, do nothing \n", a);
+        else if (std::fabs(a) > 10)
+            printf("Angle %4.2f too large, using bilinear instead\n", a);
+    }
+    if (arec_h_rotate_images(images->nx, images->ny, images->nz, images->data, angles, AREC_ROT_SKEW))
+        fprintf(stderr, "%s\n", arec_last_error());
+}
+
+// arecImage.cpp:743-794
+void arecRotateImages_skew_safe(arecImage *images, arecImage *images_ref, float *angles) {
+    int32_t kept = 0;
+    double mean_cc = 0.0;
+    if (arec_h_rotate_images_safe(images->nx, images->ny, images->nz, images->data, images_ref->data, angles, &kept,
+                                  &mean_cc)) {
+        fprintf(stderr, "%s\n", arec_last_error());
+        return;
+    }
+    printf("arecRotateImages_skew_safe: Mean CC=%4.4f n_rot=%d/%d\n", mean_cc, kept, images->nz);
+}
+
 // ---- preprocessing -------------------------------------------------------------
 
 // arecImage.cpp:358-381: a cylinder image holds nnz values, a stack nx*ny*nz.

@@ -2,15 +2,20 @@
 // resident: the views, their aligned copy, the reprojection and the volume
 // stay in HBM; only the per-view shifts (8 bytes a view) cross to the host.
 //
-// One outer loop (align = 1, no rotational step):
+// One outer loop (it = 0, 1, ...):
 //   prj = A x                                  (:188-191)
-//   aligned = views0; s = 0
+//   aligned = R(views0); s = 0                 (:201-204; R below)
 //   cc_passes times: t = CC(prj, aligned)      (:215-223, arecCCImages)
 //                    aligned = shift(aligned, -t); s += t
-//   aligned = shift(views0, -s)                (:227-229, fresh views)
+//   aligned = shift(R(views0), -s)             (:227-230, fresh views)
+//   with the rotational step (align = 2), every third loop:
+//     a = RotCC(prj, aligned); aligned = RotSafe(aligned, prj, a);
+//     rot += a                                 (:235-243)
 //   x = SIRT from zero, or CGLS from x         (:294-315), or SIRT from x
 //   x = soft_noneg(x)                          (:318)
-// preceded by the initial reconstruction from views0 (:140-165).  The
+// where R rotates by the summed angles rot from the second loop on
+// (arecRotateImages_skew; identity without the rotational step), preceded
+// by the initial reconstruction from views0 (:140-165).  The
 // reference moves views and slabs between these steps through MRC files
 // (:197-202,249,293); here they never leave the device.
 //
@@ -124,12 +129,19 @@ int centroid_x_shifts(const Geometry &g, hipStream_t st, const float *views, flo
 
 int match_run(const Geometry &g, hipStream_t st, const float *views0, float *x, float lam, int outer,
               int maxit, float tol, int cc_passes, int method, int flags, float *sx_host, float *sy_host,
-              arec_hist *loop_hist) {
+              float *rot_host, arec_hist *loop_hist) {
     const int64_t nproj = g.nproj, nnz = g.nnz;
     const int nx = g.nx, ny = g.ny, na = g.na;
-    const bool xc = (flags & kMatchXCentroid) != 0;
-    std::vector<float> hs(2 * (size_t)na), ht(2 * (size_t)na), cx(na, 0.0f);
+    const bool xc = (flags & kMatchXCentroid) != 0, rotating = (flags & kMatchRotate) != 0;
+    std::vector<float> hs(2 * (size_t)na), ht(2 * (size_t)na), cx(na, 0.0f), rsum(na, 0.0f), rcur(na);
+    // the driver's profile radius: min(2 radius + 1, height - 10) / 2 with
+    // radius (nx-1)/2 and height ny (AREC3D.cpp:237-238)
+    const int r2 = std::min(nx, ny - 10) / 2;
     int rc;
+    if (rotating && (r2 < 1 || (rc = rot_check(nx, ny, na, "arec_match")))) {
+        if (r2 < 1) set_error("arec_match: the rotational step needs ny >= 12 and nx >= 2");
+        return AREC_EINVAL;
+    }
     // centroid x shifts first: their scratch (a volume and a view stack) is
     // freed before the loop's buffers are allocated
     if (xc && (rc = centroid_x_shifts(g, st, views0, cx.data()))) return rc;
@@ -138,29 +150,34 @@ int match_run(const Geometry &g, hipStream_t st, const float *views0, float *x, 
         (rc = dalloc(t, 2 * (size_t)na)))
         return rc;
     const Progress progress;
-    // views0 shifted by the host shifts h (x from h[0..na), y from h[na..2na))
-    auto shifted_views = [&](const std::vector<float> &h) -> int {
+    // views0, rotated by the summed angles when `rotate`, then shifted by the
+    // host shifts h (x from h[0..na), y from h[na..2na))
+    auto shifted_views = [&](const std::vector<float> &h, bool rotate) -> int {
         int r;
         AREC_HIP(hipMemcpyAsync(s.p, h.data(), sizeof(float) * 2 * na, hipMemcpyHostToDevice, st));
         if ((r = copy(aligned.p, views0, nproj, st))) return r;
+        if (rotate && (r = rotate_views(aligned.p, nx, ny, na, rsum.data(), AREC_ROT_SKEW, st))) return r;
         return shift_views(aligned.p, nx, ny, na, s.p, s.p + na, -1.0f, st);
     };
     // initial reconstruction from the unshifted views (or, with centroid x
     // shifts, from the views shifted in x)
     if (xc) {
         std::copy(cx.begin(), cx.end(), hs.begin());
-        if ((rc = shifted_views(hs))) return rc;
+        if ((rc = shifted_views(hs, false))) return rc;
         progress("centroid x shifts done", -1);
     }
     if ((rc = reconstruct(g, st, method, xc ? aligned.p : views0, x, lam, maxit, tol, true, nullptr))) return rc;
     progress("initial reconstruction done", -1);
     for (int it = 0; it < outer; it++) {
+        const bool rotate = rotating && it > 0;
         if ((rc = launch_forward(g, x, prj.p, nullptr, st, nullptr))) return rc;
         std::fill(hs.begin(), hs.end(), 0.0f);
         if (xc) {
             std::copy(cx.begin(), cx.end(), hs.begin());
-            if ((rc = shifted_views(hs))) return rc;
+            if ((rc = shifted_views(hs, rotate))) return rc;
         } else if ((rc = copy(aligned.p, views0, nproj, st))) {
+            return rc;
+        } else if (rotate && (rc = rotate_views(aligned.p, nx, ny, na, rsum.data(), AREC_ROT_SKEW, st))) {
             return rc;
         }
         for (int pass = 0; pass < cc_passes; pass++) {
@@ -175,7 +192,13 @@ int match_run(const Geometry &g, hipStream_t st, const float *views0, float *x, 
             for (size_t i = 0; i < hs.size(); i++) hs[i] += ht[i];  // FP32 sums (:220-223)
         }
         // the fresh views shifted by the loop's total
-        if ((rc = shifted_views(hs))) return rc;
+        if ((rc = shifted_views(hs, rotate))) return rc;
+        if (rotating && (it + 1) % 3 == 0) {  // the rotational step (AREC3D.cpp:235-243)
+            if ((rc = rot_cc_views(prj.p, aligned.p, nx, ny, na, r2, rcur.data(), st))) return rc;
+            if ((rc = rotate_views_safe(aligned.p, prj.p, nx, ny, na, rcur.data(), nullptr, nullptr, st))) return rc;
+            for (int a = 0; a < na; a++) rsum[a] = rsum[a] + rcur[a];
+            progress("rotational step done", it);
+        }
         progress("reprojection and alignment done", it);
         arec_hist last;
         if ((rc = reconstruct(g, st, method, aligned.p, x, lam, maxit, tol, false, &last))) return rc;
@@ -186,6 +209,7 @@ int match_run(const Geometry &g, hipStream_t st, const float *views0, float *x, 
     AREC_HIP(hipStreamSynchronize(st));
     if (sx_host) std::copy(hs.begin(), hs.begin() + na, sx_host);
     if (sy_host) std::copy(hs.begin() + na, hs.end(), sy_host);
+    if (rot_host) std::copy(rsum.begin(), rsum.end(), rot_host);
     return 0;
 }
 
@@ -195,21 +219,22 @@ using namespace arec;
 
 extern "C" int arec_match_ex(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *x_dev,
                              float lam, int32_t outer, int32_t maxit, float tol, int32_t cc_passes,
-                             int32_t method, int32_t flags, float *sx, float *sy, arec_hist *loop_hist) {
+                             int32_t method, int32_t flags, float *sx, float *sy, float *rot,
+                             arec_hist *loop_hist) {
     if (!ctx || !geom || !views_dev || !x_dev || outer < 0 || maxit < 1 || cc_passes < 0 || method < 1 ||
-        method > 3 || (flags & ~kMatchXCentroid)) {
-        set_error("arec_match: bad argument (method 1 = SIRT, 2 = CGLS, 3 = SIRT continued; flags 0 or 1)");
+        method > 3 || (flags & ~(kMatchXCentroid | kMatchRotate))) {
+        set_error("arec_match: bad argument (method 1 = SIRT, 2 = CGLS, 3 = SIRT continued; flags bits 0-1)");
         return AREC_EINVAL;
     }
     return match_run(geom->g, ctx->stream, views_dev, x_dev, lam, outer, maxit, tol, cc_passes, method, flags, sx,
-                     sy, loop_hist);
+                     sy, rot, loop_hist);
 }
 
 extern "C" int arec_match(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *x_dev, float lam,
                           int32_t outer, int32_t maxit, float tol, int32_t cc_passes, int32_t method,
                           float *sx, float *sy, arec_hist *loop_hist) {
     return arec_match_ex(ctx, geom, views_dev, x_dev, lam, outer, maxit, tol, cc_passes, method, 0, sx, sy,
-                         loop_hist);
+                         nullptr, loop_hist);
 }
 
 extern "C" int arec_align_centroid(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *sx) {

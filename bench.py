@@ -146,8 +146,9 @@ def align_workload(args):
     x = ctx.empty(g.vol_shape)
     ctx.synchronize()
     t0 = time.perf_counter()
-    sx, sy, hl = A.match(ctx, g, b, x, lam, outer=args.outer, maxit=args.sirt_iters, cc_passes=args.cc_passes,
-                         method=args.recon, x_from=args.x_from)
+    res = A.match(ctx, g, b, x, lam, outer=args.outer, maxit=args.sirt_iters, cc_passes=args.cc_passes,
+                  method=args.recon, x_from=args.x_from, rotate=args.rotate)
+    sx, sy, hl = res[:3]
     ctx.synchronize()
     dt = time.perf_counter() - t0
     ex, ey = (sx - dx).astype(np.float64), (sy - dy).astype(np.float64)
@@ -183,6 +184,81 @@ def align_workload(args):
             "geometry_setup_s": round(t_geom, 2),
             "geometry_device_bytes": g.device_bytes(),
             "last_relnrm_per_loop": [h[2] for h in hl],
+            "rotational_step": bool(args.rotate),
+        },
+    }
+    if args.rotate:
+        out["config"]["max_abs_rotation_deg"] = float(np.abs(res[3]).max())
+    print(json.dumps(out), flush=True)
+
+
+def rotate_workload(args):
+    """The rotational step of projection matching (the driver's align = 2,
+    AREC3D.cpp:235-243: arecRotCCImages, then arecRotateImages_skew_safe) at
+    the full slab's view shape: the 2048 x 1024 x 1800 sinogram of the
+    seeded 200-ellipsoid phantom as the reprojection, and the same views
+    turned by seeded angles U[-3, 3] degrees (bilinear rotate2d, seed 60000)
+    as the measured views.  One step is one rotational step over all views;
+    reports its time, the recovered angles' error and the views kept."""
+    import arec3d_amd as A
+    from arec3d_amd import phantom as P
+
+    nx, ny, na = args.n, args.rows, args.angles
+    f = nx / 2048.0
+    ctx = A.Context(0)
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    g = A.Geometry(ctx, "sq", nx, ny, ang)
+    support = 1000.0 * f
+    prm = P.ellipsoid_params(5, args.ellipsoids, support, ny, axes=(32.0 * f, 480.0 * f), dens=(0.05, 1.0))
+    xt = ctx.empty(g.vol_shape)
+    P.phantom_device(ctx, xt, g.radius, ny, support, prm)
+    ref = ctx.empty(g.proj_shape)
+    g.project(xt, ref)
+    ctx.synchronize()
+    xt.free()
+    turn = np.random.default_rng(60000).uniform(-3, 3, na).astype(np.float32)
+    r2 = A.rot_r2(nx, ny)
+    log(f"sinogram {nx}x{ny}x{na} done; r2 {r2}; {args.warmup} + {args.steps} rotational steps follow")
+    times, errs, kept_n, mccs = [], [], [], []
+    for step in range(args.warmup + args.steps):
+        mov = ctx.empty(g.proj_shape)
+        A.copy(ctx, mov, ref)
+        A.rotate_views(ctx, mov, turn, "bilinear")
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        a = A.rot_cc(ctx, ref, mov, r2)
+        t1 = time.perf_counter()
+        a2, kept, mcc = A.rotate_views_safe(ctx, mov, ref, a)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        mov.free()
+        if step >= args.warmup:
+            times.append((t1 - t0, t2 - t1))
+            errs.append(float(np.abs(a + turn).max()))
+            kept_n.append(kept)
+            mccs.append(mcc)
+        log(f"step {step}: rot_cc {t1 - t0:.3f} s, skew_safe {t2 - t1:.3f} s, max |angle + turn| "
+            f"{np.abs(a + turn).max():.4f} deg, kept {kept}/{na}, mean cc {mcc:.5f}")
+    tc = float(np.median([t[0] for t in times]))
+    ts = float(np.median([t[1] for t in times]))
+    out = {
+        "metric": "rotational alignment step (arecRotCCImages + arecRotateImages_skew_safe) wall time",
+        "value": round(tc + ts, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": False,
+        "dtype": "f32/f64",
+        "data": "synthetic (seeded ellipsoid phantom sinogram; views turned by seeded angles U[-3,3] deg)",
+        "config": {
+            "workload": f"rotational step over {na} views of {nx}x{ny}, profile radius {r2} "
+                        f"({int(r2 * 2 * np.pi)} directions)",
+            "rot_cc_s": round(tc, 4),
+            "skew_safe_s": round(ts, 4),
+            "max_abs_angle_error_deg": max(errs),
+            "views_kept": kept_n[-1],
+            "mean_cc": mccs[-1],
         },
     }
     print(json.dumps(out), flush=True)
@@ -376,10 +452,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: 3 forward + 3 back launches and a 1 GiB copy kernel, no solve")
-    ap.add_argument("--workload", choices=["warm", "align", "batch", "cgls"], default="warm",
+    ap.add_argument("--workload", choices=["warm", "align", "batch", "cgls", "rotate"], default="warm",
                     help="warm: the headline warm-start solve; align: projection matching with known shifts; "
                          "batch: 64 uneven SIRT problems in one batch; cgls: CGLS 512x512x256/720 with an "
-                         "FP64 check")
+                         "FP64 check; rotate: the rotational alignment step at the full slab's view shape")
     ap.add_argument("--samples", type=int, default=64, help="batch workload: problems")
     ap.add_argument("--cpu-rows-cgls", type=int, default=2, help="cgls workload: rows of the FP64 check")
     ap.add_argument("--outer", type=int, default=5)
@@ -390,6 +466,8 @@ def main():
                     help="align workload: reconstruction of each outer loop")
     ap.add_argument("--x-from", choices=["cc", "centroid"], default="centroid",
                     help="align workload: x shifts from the view centroids (default) or the CC passes")
+    ap.add_argument("--rotate", action="store_true",
+                    help="align workload: add the rotational step every third outer loop (align = 2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); several GPUs would each "
@@ -397,12 +475,14 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    full = (2048, 1024, 1800) if args.workload == "align" else (512, 512, 720)
+    full = (2048, 1024, 1800) if args.workload in ("align", "rotate") else (512, 512, 720)
     args.n = args.n or full[0]
     args.rows = args.rows or full[1]
     args.angles = args.angles or full[2]
     if args.workload == "align":
         return align_workload(args)
+    if args.workload == "rotate":
+        return rotate_workload(args)
     if args.workload == "batch":
         if args.sirt_iters == 20:
             args.sirt_iters = 100

@@ -227,14 +227,23 @@ int arec_periodic_decomposition(arec_ctx *ctx, float *views_dev, int32_t nx, int
 int arec_match(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *x_dev, float lam,
                int32_t outer, int32_t maxit, float tol, int32_t cc_passes, int32_t method, float *sx,
                float *sy, arec_hist *loop_hist);
-/* arec_match with flags (no counterpart in the reference): flags bit 0
- * (AREC_MATCH_X_CENTROID) takes the x shifts from the views' centroids
+/* arec_match with flags.  Bit 0 (AREC_MATCH_X_CENTROID, no counterpart in
+ * the reference) takes the x shifts from the views' centroids
  * (arec_align_centroid, once, before the initial reconstruction) and only
- * the y shifts from the cross-correlation passes.  flags 0 is arec_match. */
+ * the y shifts from the cross-correlation passes.  Bit 1 (AREC_MATCH_ROTATE)
+ * adds the driver's rotational step (its align = 2, AREC3D.cpp:203-204,
+ * 229-243): from the second loop on, the fresh views are first rotated by
+ * the summed angles (arecRotateImages_skew), and every third loop the
+ * shifted views are matched in rotation against the reprojection
+ * (arecRotCCImages with r2 = min(nx, ny - 10) / 2, then
+ * arecRotateImages_skew_safe) before the reconstruction; rot (host, na
+ * floats, may be NULL) receives the summed angles in degrees.  flags 0 is
+ * arec_match. */
 #define AREC_MATCH_X_CENTROID 1
+#define AREC_MATCH_ROTATE 2
 int arec_match_ex(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *x_dev, float lam,
                   int32_t outer, int32_t maxit, float tol, int32_t cc_passes, int32_t method, int32_t flags,
-                  float *sx, float *sy, arec_hist *loop_hist);
+                  float *sx, float *sy, float *rot, arec_hist *loop_hist);
 /* x shift of every view from its centroid (no counterpart in the reference;
  * parallel beam, object inside the field of view): the view's centroid
  * along the detector minus the rotation axis (the centroid of the projected
@@ -242,6 +251,32 @@ int arec_match_ex(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, 
  * translation of the object in the plane) removed by least squares.  sx:
  * host, na floats.  AREC_EINVAL if a view has no positive mass. */
 int arec_align_centroid(arec_ctx *ctx, const arec_geom *geom, const float *views_dev, float *sx);
+
+/* Rotational alignment of view stacks [nviews][ny][nx] (views of at least
+ * 2x2 and below 2^31 pixels, at most 65535 a call).  Angles are host arrays
+ * of nviews floats, in degrees; a positive angle rotates x towards -y.
+ * arec_rot_cc (arecRotCCImages, arecImage.cpp:555-599): the rotation of each
+ * view of images2 against images1 from the 1-D correlation of their radial
+ * profiles (L = (int)(2 pi r2) directions, radii below r2, 1 <= r2 <=
+ * min(nx, ny)/2), wrapped to (-180, 180], 0 beyond 45 degrees.
+ * arec_rotate_views: rule AREC_ROT_SKEW is arecRotateImages_skew
+ * (arecImage.cpp:693-741: angles wrapped by 360 and updated, 0 and no
+ * rotation beyond 45, bilinear rotate2d beyond 10, else three circular
+ * shears); AREC_ROT_BILINEAR is arecRotateImages (:673-691, rotate2d for
+ * every view).  arec_rotate_views_safe (arecRotateImages_skew_safe,
+ * :743-794): the three shears (angles within +-90), kept where the
+ * Tukey-windowed correlation coefficient with ref rises by more than 1e-5,
+ * the angle then halved; else the view restored and its angle 0; kept and
+ * mean_cc (may be NULL) get the count kept and the mean of the chosen
+ * coefficients. */
+#define AREC_ROT_SKEW 0
+#define AREC_ROT_BILINEAR 1
+int arec_rot_cc(arec_ctx *ctx, const float *images1_dev, const float *images2_dev, int32_t nx, int32_t ny,
+                int32_t nviews, int32_t r2, float *angles);
+int arec_rotate_views(arec_ctx *ctx, float *views_dev, int32_t nx, int32_t ny, int32_t nviews, float *angles,
+                      int32_t rule);
+int arec_rotate_views_safe(arec_ctx *ctx, float *views_dev, const float *ref_dev, int32_t nx, int32_t ny,
+                           int32_t nviews, float *angles, int32_t *kept, double *mean_cc);
 
 /* ---- synthetic inputs (no counterpart in the reference) --------------- */
 /* Sum of ellipsoids at the voxel centres of the compact cylinder of the
@@ -316,6 +351,14 @@ int arec_h_shift_images(int32_t nx, int32_t ny, int32_t nz, float *data, const f
  * download; the drop-in's versions). */
 int arec_h_take_log(float *data, int64_t n);
 int arec_h_periodic_decomposition(int32_t nx, int32_t ny, int32_t nz, float *data);
+/* arecRotCCImages / arecRotateImages(_skew) / arecRotateImages_skew_safe on
+ * host stacks (the drop-in's versions): -1 when the two stacks differ in
+ * size, as the reference. */
+int arec_h_rot_cc_images(int32_t nx1, int32_t ny1, int32_t nz1, const float *images1, int32_t nx2, int32_t ny2,
+                         int32_t nz2, const float *images2, int32_t r2, float *angles);
+int arec_h_rotate_images(int32_t nx, int32_t ny, int32_t nz, float *data, float *angles, int32_t rule);
+int arec_h_rotate_images_safe(int32_t nx, int32_t ny, int32_t nz, float *data, const float *ref, float *angles,
+                              int32_t *kept, double *mean_cc);
 
 /* MRC files (arecImageIO.cpp:10-173, header arecImageIO.h:5-63), host only:
  * a 1024-byte header, then nx*ny*nz values, x fastest, host byte order.

@@ -74,6 +74,12 @@ int cyl_cgls_SQ_stat(MPI_Comm comm, arecImage images, float *angles, arecImage *
 int arecCCImages(MPI_Comm comm, arecImage images1, arecImage images2, float *sx, float *sy);
 void arecShiftImages(arecImage images, float *sx, float *sy);
 
+/* arecImage.h:37,46-48 (rotational alignment) */
+int arecRotCCImages(MPI_Comm comm, arecImage images1, arecImage images2, int r2, float *angle);
+void arecRotateImages(arecImage *images, float *angles);
+void arecRotateImages_skew(arecImage *images, float *angles);
+void arecRotateImages_skew_safe(arecImage *images, arecImage *images_ref, float *angles);
+
 /* arecImage.h:34, imagetools.h:5 (preprocessing of the measured views) */
 void arecImageTakeLog(arecImage image);
 void periodicDecomposition(arecImage img);

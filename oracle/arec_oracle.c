@@ -18,6 +18,14 @@
  *   fshift2d        integer shifts equal circular shifts; fractional shifts
  *                   against a direct FP64 evaluation of the band-limited
  *                   interpolant with the reference's frequency tables
+ *   take_log,       closed forms; the smooth part solves the periodic
+ *   periodic_*      discrete Poisson equation with the boundary image
+ *   rotational      correlation theorem (numpy FFT), least-squares parabola
+ *   alignment       (numpy polyfit), closed-form profiles of linear images,
+ *                   circular shifts for integer shears, exact interpolation
+ *                   of linear rows, point rotation about the stated centre,
+ *                   Pearson coefficient with SciPy's Tukey window; the
+ *                   reference's sources need FFTW, so no reference vector
  */
 #include "arec_oracle.h"
 

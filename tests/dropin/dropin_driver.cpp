@@ -106,6 +106,32 @@ int main(int argc, char **argv) {
         arecImageFree(&xe);
         free(angles);
     }
+    {  // rotational alignment: a stack turned by +5 degrees (rotate2d) measures about -5
+        const int nx = 64, ny = 64, nz = 2;
+        arecImage a, b;
+        arecAllocateCBImage(&a, nx, ny, nz);
+        arecAllocateCBImage(&b, nx, ny, nz);
+        for (int z = 0; z < nz; z++)
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++) {
+                    const double dx = x - 44.0 - 3 * z, dy = y - 30.0, ex = x - 20.0, ey = y - 40.0 + 2 * z;
+                    a.data[(z * ny + y) * nx + x] = (float)(std::exp(-(dx * dx + dy * dy) / 18.0) +
+                                                            0.6 * std::exp(-(ex * ex + ey * ey) / 24.0));
+                }
+        for (int i = 0; i < nx * ny * nz; i++) b.data[i] = a.data[i];
+        float turn[2] = {5.0f, 5.0f}, ang[2] = {0.0f, 0.0f};
+        arecRotateImages(&b, turn);
+        CHECK(arecRotCCImages(MPI_COMM_WORLD, a, b, 27, ang) == 0);
+        CHECK(std::fabs(ang[0] + 5.0f) < 0.3f && std::fabs(ang[1] + 5.0f) < 0.3f);
+        arecRotateImages_skew_safe(&b, &a, ang);
+        CHECK(std::fabs(ang[0] + 2.5f) < 0.2f);  // kept and halved
+        arecImage c;
+        arecAllocateCBImage(&c, nx, ny, 1);
+        CHECK(arecRotCCImages(MPI_COMM_WORLD, a, c, 27, ang) == -1);  // size mismatch
+        arecImageFree(&a);
+        arecImageFree(&b);
+        arecImageFree(&c);
+    }
     MPI_Finalize();
     printf(fails ? "dropin driver: %d failures\n" : "dropin driver: all checks passed%.0d\n", fails);
     return fails ? 1 : 0;

@@ -28,7 +28,11 @@ def test_dropin_driver_links():
     assert os.path.exists(build_driver())
     out = subprocess.run(["nm", "-DC", DROPIN], capture_output=True, text=True).stdout
     for sym in ["arecProject2D_SQ(Image, float*, int, Image*)", "cyl_sirt_SQ(int, Image, float*, Image*, float, int, float)",
-                "cyl_cgls(int, Image, float*, Image*, float, int, float)", "arecCCImages(int, Image, Image, float*, float*)"]:
+                "cyl_cgls(int, Image, float*, Image*, float, int, float)", "arecCCImages(int, Image, Image, float*, float*)",
+                "arecRotCCImages(int, Image, Image, int, float*)", "arecRotateImages_skew_safe(Image*, Image*, float*)",
+                "arecRotateImages_skew(Image*, float*)", "arecRotateImages(Image*, float*)",
+                "arecReadImage(char*, Image*)", "arecWriteImage(char*, Image)",
+                "arecReadImageHeader(char*, int*, int*, int*)", "periodicDecomposition(Image)"]:
         assert sym in out, sym
 
 

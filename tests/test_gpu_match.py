@@ -21,16 +21,20 @@ import pyoracle as O
 pytestmark = pytest.mark.gpu
 
 
-def oracle_match(views, ang, nx, ny, lam, outer, maxit, cc_passes, x_centroid=False):
+def oracle_match(views, ang, nx, ny, lam, outer, maxit, cc_passes, x_centroid=False, rotate=False):
     na = len(ang)
     cx = O.centroid_x_shifts("sq", views, ang, nx, ny).astype(np.float32) if x_centroid else None
     start = O.shift_images(views, cx, np.zeros(na, np.float32)) if x_centroid else views
     _, x, _, _ = O.sirt("sq", start, ang, lam, maxit, 0.0)
-    for _ in range(outer):
+    rsum = np.zeros(na, np.float32)
+    for it in range(outer):
         prj = O.project("sq", x, ang, nx, ny)
         sx = cx.copy() if x_centroid else np.zeros(na, np.float32)
         sy = np.zeros(na, np.float32)
-        aligned = O.shift_images(views, sx, sy) if x_centroid else views.copy()
+        base = views
+        if rotate and it > 0:  # AREC3D.cpp:203-204
+            base, rsum = O.rotate_skew(views, rsum)
+        aligned = O.shift_images(base, sx, sy) if x_centroid else base.copy()
         for _ in range(cc_passes):
             tx, ty = O.cc_images(prj, aligned)
             if x_centroid:
@@ -38,10 +42,16 @@ def oracle_match(views, ang, nx, ny, lam, outer, maxit, cc_passes, x_centroid=Fa
             aligned = O.shift_images(aligned, tx, ty)
             sx += tx
             sy += ty
-        aligned = O.shift_images(views, sx, sy)
+        if rotate and it > 0:  # AREC3D.cpp:228-229
+            base, rsum = O.rotate_skew(views, rsum)
+        aligned = O.shift_images(base, sx, sy)
+        if rotate and (it + 1) % 3 == 0:  # AREC3D.cpp:235-243
+            a = O.rot_cc_images(prj, aligned, O.rot_r2(nx, ny))
+            aligned, a, _, _ = O.rotate_skew_safe(aligned, prj, a)
+            rsum = (rsum + a).astype(np.float32)
         _, x, _, _ = O.sirt("sq", aligned, ang, lam, maxit, 0.0)
         x = O.soft_noneg(x)
-    return sx, sy, x
+    return (sx, sy, x, rsum) if rotate else (sx, sy, x)
 
 
 def shifted_views(ctx, g, nx, ny, na, seed, amp, ell=12, noise=0.01):
@@ -149,3 +159,31 @@ def test_match_centroid_x_recovers_shifts(ctx):
     ex, ey = gauge_free_error(sx, sy, dx, dy, ang)
     print(f"max |error| x {np.abs(ex).max():.4f} px, y {np.abs(ey).max():.4f} px")
     assert np.abs(ex).max() <= 0.1 and np.abs(ey).max() <= 0.1
+
+
+def test_match_rotate_vs_oracle(ctx):
+    """The loop with the rotational step (the driver's align = 2) against the
+    oracle's restatement: views turned by seeded angles in [-4, 4] degrees
+    and shifted; 4 outer loops, so the summed angles from the step at the
+    third loop rotate the fresh views of the fourth."""
+    import arec3d_amd as A
+
+    print("\n[case] match with rotation vs oracle", flush=True)
+    nx, ny, na = 33, 30, 24
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    g = A.Geometry(ctx, "sq", nx, ny, ang)
+    b, dx, dy = shifted_views(ctx, g, nx, ny, na, 11, 1.0, ell=4)
+    turn = np.random.default_rng(12).uniform(-4, 4, na).astype(np.float32)
+    A.rotate_views(ctx, b, turn, "bilinear")
+    views = b.download()
+    lam = 1.0 / (na * nx)
+    x = ctx.empty(g.vol_shape)
+    sx, sy, hl, rot = A.match(ctx, g, b, x, lam, outer=4, maxit=6, cc_passes=2, rotate=True)
+    assert np.array_equal(b.download(), views)
+    osx, osy, ox, orot = oracle_match(views, ang, nx, ny, lam, 4, 6, 2, rotate=True)
+    print("rot", rot[:6], "oracle", orot[:6], flush=True)
+    assert np.count_nonzero(orot) > 0  # the step kept some rotations
+    assert np.abs(rot - orot).max() <= 1e-3
+    assert np.abs(sx - osx).max() <= 2e-3 and np.abs(sy - osy).max() <= 2e-3
+    xg = x.download()
+    assert np.linalg.norm(xg - ox) <= 1e-3 * np.linalg.norm(ox)
