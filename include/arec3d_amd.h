@@ -97,9 +97,9 @@ int64_t arec_geom_device_bytes(const arec_geom *geom);
 int arec_geom_plan(arec_ctx *ctx, arec_geom *geom, int32_t rows_per_lane, int32_t min_groups);
 int arec_geom_get_plan(const arec_geom *geom, int32_t *rows_per_lane, int32_t *min_groups, int32_t *nseg);
 /* Forward projector design (no counterpart in the reference): 1 = detector
- * sweep (the default; arec_geom_plan sets its plan), 2 = depth strips (8-pixel
+ * sweep (the default; arec_geom_plan sets its plan), 2 = depth strips (16-pixel
  * detector strips walked in depth order by 4 neighbouring angles in
- * lockstep; its table, ~1.25 x the sweep table, is built when design 2 is
+ * lockstep; its table, ~1.125 x the sweep table, is built when design 2 is
  * selected, refused above 32 GiB, and released when design 1 is selected).
  * Results agree to FP32 summation order.  arec_geom_get_plan reports the
  * selected design's rows per lane. */
