@@ -48,7 +48,7 @@ constexpr int kStripW = 16;  // 23.8 ms at 512^3/720 (8: 24.8, 12: 24.3, 20: 27.
 constexpr int kDepthBin = 16;
 constexpr int kStripGroups = kStripW + 2;
 constexpr int kStripAngles = 4;  // angles (wavefronts) per depth-strip workgroup
-constexpr int kStripBatch = 8;   // depth-strip table entries whose loads are in flight together
+constexpr int kStripBatch = 4;   // depth-strip table entries whose loads are in flight together (8: 3% slower)
 // The back projector's tap table is built when it takes at most this much.
 constexpr int64_t kMaxBackTableBytes = 16ll << 30;
 // The depth-strip forward table is built (on request) up to this size.

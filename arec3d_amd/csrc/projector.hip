@@ -414,7 +414,9 @@ __device__ __forceinline__ void strip_add(float (&acc)[kStripW + 4][RPL], float 
 // the batch waits for its rows, so it stays in flight; it is written one
 // batch later, and read about kStripAhead entries later still.  A write
 // replaces entries at least 128 - kStripAhead - 8 before the batch, which no
-// batch reads again.
+// batch reads again.  With batches of kStripBatch < 8 entries consecutive
+// fetches overlap, and an entry is written (with the same value) more than
+// once before it is read.
 constexpr int kStripAhead = 64;
 
 struct StripRing {

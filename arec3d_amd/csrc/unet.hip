@@ -51,89 +51,96 @@ struct Dims {
 // 3x3x3 convolution, forward form, on MFMA.
 // Input channels [0, C1) come from in1 (dims d), [C1, C1+C2) from in2 at half
 // resolution (nearest upsampling: position s reads in2[s/2]).
+// A workgroup computes 64 positions along W (16 per wave) in conv_rows(COUT)
+// consecutive H rows: each weight operand loaded for a tap serves every row,
+// which divides the weight traffic by the row count and keeps 3 x (rows + NB)
+// loads in flight per (tap row, channel block).
 // ---------------------------------------------------------------------------
+__host__ __device__ constexpr int conv_rows(int cout) { return cout >= 64 ? 2 : 4; }
+
 template <int COUT>
 __global__ __launch_bounds__(kCvThreads) void conv3_mfma_kernel(
     const bf16_t *__restrict__ in1, int C1, const bf16_t *__restrict__ in2, int C2,
     const bf16_t *__restrict__ w, const float *__restrict__ bias, bf16_t *__restrict__ out, Dims d,
     int relu) {
-    constexpr int NB = COUT / 16;
+    constexpr int NB = COUT / 16, R = conv_rows(COUT);
     const int Ct = C1 + C2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wtiles = (d.W + 63) / 64;
+    const int wtiles = (d.W + 63) / 64, hblocks = (d.H + R - 1) / R;
     const int wt = blockIdx.x % wtiles;
-    const int h = (blockIdx.x / wtiles) % d.H;
-    const int z = blockIdx.x / (wtiles * d.H);
+    const int h0 = ((blockIdx.x / wtiles) % hblocks) * R;
+    const int z = blockIdx.x / (wtiles * hblocks);
     const int pbase = wt * 64 + wave * 16;
     const int pa = pbase + (lane & 15);  // this lane's A-operand position
     const int kq = (lane >> 4) * 4;      // this lane's 4 input channels within a 16-block
-    f4v acc[NB];
+    f4v acc[R][NB];
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) acc[nb] = f4v{0.f, 0.f, 0.f, 0.f};
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) acc[r][nb] = f4v{0.f, 0.f, 0.f, 0.f};
     const int H2 = d.H >> 1, W2 = d.W >> 1;
-    // Per (dz, dh) row of taps, the operands of its three dw taps for KB
-    // channel blocks are loaded first and multiplied after, so a batch of
-    // loads is in flight at a time instead of one (tap, block) pair.  The
-    // MFMAs run per row in the order (channel batch, dw tap, block); when
-    // one batch holds all the input channels (Ct <= 16 KB) that is the
-    // one-tap order (tap, block), otherwise the FP32 sums are reassociated.
-    constexpr int KB = NB == 1 ? 3 : NB == 2 ? 2 : 1;
+    // Per (dz, dh) row of taps and 16-channel block, the operands of its three
+    // dw taps for all rows are loaded first and multiplied after.  The MFMAs
+    // of an output run in the order (tap row, channel block, dw tap): with
+    // one block (Ct = 16) that is the tap order, otherwise the FP32 sums are
+    // reassociated.
     for (int row = 0; row < 9; row++) {
         const int dz = row / 3 - 1, dh = row % 3 - 1;
-        const int zz = z + dz, hh = h + dh;
-        if (zz < 0 || zz >= d.D || hh < 0 || hh >= d.H) continue;  // uniform
-        for (int cb0 = 0; cb0 < Ct; cb0 += 16 * KB) {
-            s4v a[3][KB], b[3][KB][NB];
+        const int zz = z + dz;
+        if (zz < 0 || zz >= d.D) continue;  // uniform
+        for (int cb = 0; cb < Ct; cb += 16) {
+            const int ci = cb + kq;  // the block lies in in1 or in in2 (C1 is a multiple of 16)
+            s4v a[3][R], b[3][NB];
 #pragma unroll
             for (int t = 0; t < 3; t++) {
                 const int pp = pa + t - 1;
                 const bool pv = pp >= 0 && pp < d.W;
                 const bf16_t *wt_tap = w + (size_t)(row * 3 + t) * COUT * Ct;
 #pragma unroll
-                for (int kb = 0; kb < KB; kb++) {
-                    const int ci = cb0 + kb * 16 + kq;
-                    a[t][kb] = s4v{0, 0, 0, 0};
+                for (int nb = 0; nb < NB; nb++) {
+                    const int co = nb * 16 + (lane & 15);
+                    b[t][nb] = *reinterpret_cast<const s4v *>(wt_tap + (size_t)co * Ct + ci);
+                }
 #pragma unroll
-                    for (int nb = 0; nb < NB; nb++) b[t][kb][nb] = s4v{0, 0, 0, 0};
-                    if (cb0 + kb * 16 < Ct) {  // uniform
-                        if (pv) {
-                            const bf16_t *src;
-                            if (ci < C1)
-                                src = in1 + (((size_t)zz * d.H + hh) * d.W + pp) * C1 + ci;
-                            else
-                                src = in2 + (((size_t)(zz >> 1) * H2 + (hh >> 1)) * W2 + (pp >> 1)) * C2 + (ci - C1);
-                            a[t][kb] = *reinterpret_cast<const s4v *>(src);
-                        }
-#pragma unroll
-                        for (int nb = 0; nb < NB; nb++) {
-                            const int co = nb * 16 + (lane & 15);
-                            b[t][kb][nb] = *reinterpret_cast<const s4v *>(wt_tap + (size_t)co * Ct + ci);
-                        }
+                for (int r = 0; r < R; r++) {
+                    const int hh = h0 + r + dh;
+                    a[t][r] = s4v{0, 0, 0, 0};
+                    if (pv && hh >= 0 && hh < d.H) {
+                        const bf16_t *src;
+                        if (ci < C1)
+                            src = in1 + (((size_t)zz * d.H + hh) * d.W + pp) * C1 + ci;
+                        else
+                            src = in2 + (((size_t)(zz >> 1) * H2 + (hh >> 1)) * W2 + (pp >> 1)) * C2 + (ci - C1);
+                        a[t][r] = *reinterpret_cast<const s4v *>(src);
                     }
                 }
             }
 #pragma unroll
             for (int t = 0; t < 3; t++)
 #pragma unroll
-                for (int kb = 0; kb < KB; kb++)
-                    if (cb0 + kb * 16 < Ct)
+                for (int r = 0; r < R; r++)
 #pragma unroll
-                        for (int nb = 0; nb < NB; nb++)
-                            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t][kb], b[t][kb][nb], acc[nb], 0, 0, 0);
+                    for (int nb = 0; nb < NB; nb++)
+                        acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t][r], b[t][nb], acc[r][nb], 0, 0, 0);
         }
     }
     // lane holds rows 4*(lane/16)+i (positions) of column lane%16 (channel)
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) {
-        const int co = nb * 16 + (lane & 15);
-        const float bv = bias ? bias[co] : 0.f;
+    for (int r = 0; r < R; r++) {
+        const int h = h0 + r;
+        if (h >= d.H) continue;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int p = pbase + (lane >> 4) * 4 + i;
-            if (p < d.W) {
-                float v = acc[nb][i] + bv;
-                if (relu) v = fmaxf(v, 0.f);
-                out[(((size_t)z * d.H + h) * d.W + p) * COUT + co] = f2bf(v);
+        for (int nb = 0; nb < NB; nb++) {
+            const int co = nb * 16 + (lane & 15);
+            const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int p = pbase + (lane >> 4) * 4 + i;
+                if (p < d.W) {
+                    float v = acc[r][nb][i] + bv;
+                    if (relu) v = fmaxf(v, 0.f);
+                    out[(((size_t)z * d.H + h) * d.W + p) * COUT + co] = f2bf(v);
+                }
             }
         }
     }
@@ -491,7 +498,12 @@ static dim3 ew(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<
 // ---------------------------------------------------------------------------
 static int launch_conv3(const bf16_t *in1, int C1, const bf16_t *in2, int C2, const bf16_t *w,
                         const float *bias, bf16_t *out, int Cout, Dims d, int relu, hipStream_t st) {
-    dim3 grid((unsigned)(((d.W + 63) / 64) * d.H * d.D));
+    if (C1 % 16 || C2 % 16) {
+        set_error("conv3: channel counts must be multiples of 16");
+        return AREC_EINVAL;
+    }
+    const int R = conv_rows(Cout);
+    dim3 grid((unsigned)(((d.W + 63) / 64) * ((d.H + R - 1) / R) * d.D));
     switch (Cout) {
     case 16: conv3_mfma_kernel<16><<<grid, kCvThreads, 0, st>>>(in1, C1, in2, C2, w, bias, out, d, relu); break;
     case 32: conv3_mfma_kernel<32><<<grid, kCvThreads, 0, st>>>(in1, C1, in2, C2, w, bias, out, d, relu); break;
@@ -845,7 +857,10 @@ int arec_conv3_check(arec_ctx *ctx, const float *in1, int32_t C1, const float *i
     AREC_HIP(hipMalloc((void **)&b1, 2 * S * C1));
     if (C2) AREC_HIP(hipMalloc((void **)&b2, 2 * S2 * C2));
     AREC_HIP(hipMalloc((void **)&bw, 2 * 27 * Cout * Ct));
-    AREC_HIP(hipMalloc((void **)&bo, 2 * S * Cout));
+    // output plus a guard of 4 W-rows filled with 0x7f7f, checked afterwards
+    const int64_t guard = 4 * (int64_t)W * Cout;
+    AREC_HIP(hipMalloc((void **)&bo, 2 * (S * Cout + guard)));
+    AREC_HIP(hipMemsetAsync(bo + S * Cout, 0x7f, 2 * guard, st));
     to_bf16_kernel<<<ew(S * C1), 256, 0, st>>>(in1, b1, S * C1);
     if (C2) to_bf16_kernel<<<ew(S2 * C2), 256, 0, st>>>(in2, b2, S2 * C2);
     to_bf16_kernel<<<ew(27 * Cout * Ct), 256, 0, st>>>(w, bw, 27 * Cout * Ct);
@@ -861,6 +876,14 @@ int arec_conv3_check(arec_ctx *ctx, const float *in1, int32_t C1, const float *i
             memcpy(&f[i], &u32, 4);
         }
         hipMemcpy(out, f.data(), 4 * S * Cout, hipMemcpyHostToDevice);
+        std::vector<uint16_t> g(guard);
+        hipMemcpy(g.data(), bo + S * Cout, 2 * guard, hipMemcpyDeviceToHost);
+        for (int64_t i = 0; i < guard; i++)
+            if (g[i] != 0x7f7f) {
+                set_error("arec_conv3_check: the convolution wrote past its output");
+                rc = AREC_EINVAL;
+                break;
+            }
     }
     hipFree(b1);
     hipFree(b2);

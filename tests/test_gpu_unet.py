@@ -38,6 +38,29 @@ def test_conv3_forward_vs_oracle(ctx, C1, C2, Cout):
     assert np.abs(got - ref).max() <= 1e-2 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("C1,C2,Cout", [(16, 0, 16), (16, 32, 16), (64, 0, 64), (32, 0, 48)])
+@pytest.mark.parametrize("D,H,W", [(4, 10, 40), (2, 6, 70), (2, 2, 16)])
+def test_conv3_forward_row_blocks(ctx, C1, C2, Cout, D, H, W):
+    """Heights that are not a multiple of the kernel's rows per workgroup
+    (4, or 2 at 64+ output channels): the last block's extra rows are not
+    written (arec_conv3_check fails if a guard past the output changes)."""
+    import arec3d_amd as A
+
+    print(f"\n[case] conv3 row blocks {C1}+{C2}->{Cout} {D}x{H}x{W}", flush=True)
+    rng = np.random.default_rng(D * 1000 + H * 10 + Cout)
+    a = U.bf16_round(rng.standard_normal((D, H, W, C1)))
+    b = U.bf16_round(rng.standard_normal((D // 2, H // 2, W // 2, C2))) if C2 else None
+    w = U.bf16_round(rng.standard_normal((27, Cout, C1 + C2)) / np.sqrt(27 * (C1 + C2)))
+    bias = rng.standard_normal(Cout).astype(np.float32)
+    ref = U.conv3(a, w, bias, in2=b, relu=False)
+    da, dw, db = ctx.to_device(a), ctx.to_device(w), ctx.to_device(bias)
+    do = ctx.empty((D, H, W, Cout))
+    d2 = ctx.to_device(b) if C2 else None
+    A._check(A.lib.arec_conv3_check(ctx.h, da.ptr, C1, None if d2 is None else d2.ptr, C2, dw.ptr, db.ptr, do.ptr,
+                                    Cout, D, H, W, 0))
+    assert np.abs(do.download() - ref).max() <= 1e-2 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("C1,C2,Cout", [(16, 0, 16), (32, 0, 64), (32, 64, 32), (16, 32, 16)])
 def test_conv3_wgrad_vs_oracle(ctx, C1, C2, Cout):
     import arec3d_amd as A
