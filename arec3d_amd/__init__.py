@@ -315,7 +315,8 @@ class Geometry:
         _check(lib.arec_geom_plan(self.ctx.h, self.h, int(rows_per_lane), int(min_groups)))
 
     def set_forward(self, design, rows_per_lane=0, min_groups=0):
-        """Forward design 1 (detector sweep) or 2 (depth strips) and its plan."""
+        """Forward design 1 (detector sweep), 2 (depth strips) or 3 (depth
+        strips walked by angle pairs) and its plan."""
         _check(lib.arec_geom_set_forward(self.ctx.h, self.h, int(design), int(rows_per_lane), int(min_groups)))
 
     @property

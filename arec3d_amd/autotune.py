@@ -6,9 +6,9 @@ designs to FP32 summation order; its speed depends on both.  Design 1 (the
 detector sweep) has plans (detector rows per lane, minimum workgroup count):
 rows per lane sets how many rows one wavefront carries (1, 2 or 4: one, two
 or four FP32 per lane per column read), the workgroup count how finely each
-angle's detector is split into segments.  Design 2 (depth strips) has only
-rows per lane; its table is built while it is timed and released again when
-design 1 wins.  :func:`autotune` times every candidate on the geometry with
+angle's detector is split into segments.  Designs 2 (depth strips) and 3
+(depth strips walked by angle pairs) have only rows per lane; their table is
+built while timed and released again when design 1 wins.  :func:`autotune` times every candidate on the geometry with
 HIP events and keeps the fastest; results are cached in a JSON file keyed by
 (method, nx, ny, na) so a shape is timed once.
 """
@@ -22,6 +22,7 @@ import arec3d_amd as A
 ROWS_PER_LANE = (4, 2, 1)
 MIN_GROUPS = (512, 1024, 2048, 4096)
 STRIP_ROWS_PER_LANE = (4, 2)
+PAIR_ROWS_PER_LANE = (2, 1)
 
 
 def default_cache():
@@ -41,13 +42,15 @@ def _load(path):
         return {}
 
 
-def candidates(g, designs=(1, 2)):
+def candidates(g, designs=(1, 2, 3)):
     """(design, rows_per_lane, min_groups) plans allowed for geometry g."""
     out = []
     if 1 in designs:
         out += [(1, r, m) for r in ROWS_PER_LANE if g.ny % r == 0 for m in MIN_GROUPS]
     if 2 in designs:
         out += [(2, r, 0) for r in STRIP_ROWS_PER_LANE if g.ny % r == 0]
+    if 3 in designs:
+        out += [(3, r, 0) for r in PAIR_ROWS_PER_LANE if g.ny % r == 0]
     return out
 
 
@@ -56,7 +59,7 @@ def apply(g, plan):
     g.set_forward(design, r, m)
 
 
-def autotune(ctx, g, cache=None, reps=3, log=None, designs=(1, 2)):
+def autotune(ctx, g, cache=None, reps=3, log=None, designs=(1, 2, 3)):
     """Pick and set the fastest forward design and plan for geometry g;
     returns (design, rows_per_lane, min_groups, ms)."""
     path = cache or default_cache()

@@ -98,8 +98,12 @@ struct Geometry {
     // tap - (s*kStripW - 2) in [0, kStripGroups); entry = {column | d << 22,
     // w0, w1, w2}.  strip_off[a][s*nbins + b] is the first entry of (s, b)
     // relative to strip_base[a].
+    // Design 3 (depth strips, angle pairs) uses the same fields per pair of
+    // angles (strip_unit = 2): an entry is two uint4, {column | d0 << 22 |
+    // d1 << 27, w0[3]} and {w1[3], 0}, d = kStripGroups for an angle absent
+    // from the entry, and entries are ordered by (bin, d0, d1).
     int fwd_design = 1;
-    int nstrips = 0, nbins = 0, strip_rpl = 1;
+    int nstrips = 0, nbins = 0, strip_rpl = 1, strip_unit = 1;
     std::vector<int64_t> strip_base;  // na + 1
     std::vector<int> strip_off;       // na * (nstrips * nbins + 1)
     uint4 *d_strip = nullptr;
@@ -124,6 +128,11 @@ void geometry_segments(Geometry &g);
 // with (column | strip << 22) per entry, angle after angle.
 void geometry_strip_count(Geometry &g);
 void geometry_strip_angles(Geometry &g, int a0, int a1, std::vector<uint32_t> &order);
+// Angle-pair ordering (design 3): the same fields per pair of angles
+// (strip_base has (na + 1) / 2 + 1 entries); order gets (column | strip << 22
+// | present flags << 30) per entry, pair after pair.
+void geometry_pair_count(Geometry &g);
+void geometry_pair_units(Geometry &g, int u0, int u1, std::vector<uint32_t> &order);
 // Segment table of segments of width segw (nseg of them) for g's forward table.
 void segment_table(const Geometry &g, int segw, int nseg, std::vector<int> &seg);
 

@@ -154,7 +154,7 @@ int64_t arec_geom_device_bytes(const arec_geom *geom) {
     if (!geom) return 0;
     const Geometry &g = geom->g;
     const int64_t btab = (g.d_btab ? (int64_t)sizeof(uint4) * g.ntiles * g.na * kTileCols : 0) +
-                         (g.d_strip ? (int64_t)sizeof(uint4) * g.strip_base[g.na] : 0);
+                         (g.d_strip ? (int64_t)sizeof(uint4) * g.strip_unit * g.strip_base.back() : 0);
     return (int64_t)sizeof(uint4) * g.na * g.nvalid + btab + (int64_t)sizeof(AngleConst) * g.na +
            (int64_t)sizeof(int) * (g.coord.size() + g.tile_cols.size() + g.tile_org.size() + g.seg.size() +
                                    g.pixstart.size());
@@ -219,7 +219,7 @@ int32_t arec_geom_forward_design(const arec_geom *geom) { return geom ? geom->g.
 
 int arec_geom_get_plan(const arec_geom *geom, int32_t *rows_per_lane, int32_t *min_groups, int32_t *nseg) {
     CHECK_ARG(geom, "NULL argument");
-    if (rows_per_lane) *rows_per_lane = geom->g.fwd_design == 2 ? geom->g.strip_rpl : geom->g.fwd_rpl;
+    if (rows_per_lane) *rows_per_lane = geom->g.fwd_design >= 2 ? geom->g.strip_rpl : geom->g.fwd_rpl;
     if (min_groups) *min_groups = geom->g.fwd_min_groups;
     if (nseg) *nseg = geom->g.nseg;
     return 0;

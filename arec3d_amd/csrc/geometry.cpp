@@ -280,6 +280,104 @@ void geometry_strip_angles(Geometry &g, int a0, int a1, std::vector<uint32_t> &o
     for (auto &th : pool) th.join();
 }
 
+// Angle pairs (design 3): unit u holds angles 2u and 2u + 1 (the second
+// absent when 2u + 1 == na).  A column is an entry of (u, strip s) when s
+// is one of its strips at either angle; the entry carries a present flag
+// per angle, so every (angle, strip, column) contribution of design 2 is in
+// exactly one entry.  Entries are ordered by the depth bin at angle 2u and,
+// inside a bin, by (d0, d1), the first-tap offsets in the strip (absent =
+// kStripGroups).
+static int pair_strips(const Geometry &g, int a, int j, int *s, unsigned *mask) {
+    int sa[2], sb[2];
+    const int n0 = strips_of(host_first_tap(g, a, j), g.nstrips, sa);
+    const int n1 = a + 1 < g.na ? strips_of(host_first_tap(g, a + 1, j), g.nstrips, sb) : 0;
+    int n = 0;
+    auto add = [&](int st, unsigned bit) {
+        for (int k = 0; k < n; k++)
+            if (s[k] == st) {
+                mask[k] |= bit;
+                return;
+            }
+        s[n] = st;
+        mask[n++] = bit;
+    };
+    for (int k = 0; k < n0; k++) add(sa[k], 1u);
+    for (int k = 0; k < n1; k++) add(sb[k], 2u);
+    return n;
+}
+
+void geometry_pair_count(Geometry &g) {
+    g.nstrips = (g.nx + kStripW - 1) / kStripW;
+    double c = g.method == kLinear ? (double)g.xcent : g.radius;
+    g.nbins = (int)std::ceil((2 * c + 4) / kDepthBin) + 1;
+    const int nu = (g.na + 1) / 2;
+    g.strip_base.assign((size_t)nu + 1, 0);
+    std::vector<int64_t> cnt(nu, 0);
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    nth = std::min<unsigned>(nth, (unsigned)nu);
+    auto work = [&](unsigned t) {
+        int s[4];
+        unsigned m[4];
+        for (int u = (int)t; u < nu; u += (int)nth) {
+            int64_t n = 0;
+            for (int j = 0; j < g.nvalid; j++) n += pair_strips(g, 2 * u, j, s, m);
+            cnt[u] = n;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; t++) pool.emplace_back(work, t);
+    for (auto &th : pool) th.join();
+    for (int u = 0; u < nu; u++) g.strip_base[u + 1] = g.strip_base[u] + cnt[u];
+    g.strip_off.assign((size_t)nu * ((size_t)g.nstrips * g.nbins + 1), 0);
+}
+
+void geometry_pair_units(Geometry &g, int u0, int u1, std::vector<uint32_t> &order) {
+    constexpr int G = kStripGroups + 1, G2 = G * G;
+    const int nb = g.nstrips * g.nbins, nk = nb * G2;
+    const int64_t e0 = g.strip_base[u0];
+    order.resize((size_t)(g.strip_base[u1] - e0));
+    double c = g.method == kLinear ? (double)g.xcent : g.radius;
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    nth = std::min<unsigned>(nth, (unsigned)(u1 - u0));
+    auto work = [&](unsigned t) {
+        std::vector<int> cnt(nk + 1);
+        std::vector<int> nstr(g.nvalid);
+        std::vector<int> str(4 * (size_t)g.nvalid), key(4 * (size_t)g.nvalid);
+        std::vector<unsigned> msk(4 * (size_t)g.nvalid);
+        for (int u = u0 + (int)t; u < u1; u += (int)nth) {
+            const int a = 2 * u;
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (int j = 0; j < g.nvalid; j++) {
+                int b = (int)std::floor((host_depth(g, a, j) + c + 1.0) / kDepthBin);
+                b = std::min(std::max(b, 0), g.nbins - 1);
+                const int p0 = host_first_tap(g, a, j);
+                const int p1 = a + 1 < g.na ? host_first_tap(g, a + 1, j) : 0;
+                nstr[j] = pair_strips(g, a, j, &str[4 * (size_t)j], &msk[4 * (size_t)j]);
+                for (int k = 0; k < nstr[j]; k++) {
+                    const size_t q = 4 * (size_t)j + k;
+                    const int sj = str[q];
+                    const int d0 = (msk[q] & 1u) ? p0 - (sj * kStripW - 2) : kStripGroups;
+                    const int d1 = (msk[q] & 2u) ? p1 - (sj * kStripW - 2) : kStripGroups;
+                    key[q] = (sj * g.nbins + b) * G2 + d0 * G + d1;
+                    cnt[key[q] + 1]++;
+                }
+            }
+            for (int i = 1; i <= nk; i++) cnt[i] += cnt[i - 1];
+            int *off = &g.strip_off[(size_t)u * (nb + 1)];
+            for (int i = 0; i <= nb; i++) off[i] = cnt[(size_t)i * G2];
+            uint32_t *out = order.data() + (g.strip_base[u] - e0);
+            for (int j = 0; j < g.nvalid; j++)
+                for (int k = 0; k < nstr[j]; k++) {
+                    const size_t q = 4 * (size_t)j + k;
+                    out[cnt[key[q]]++] = (uint32_t)j | ((uint32_t)str[q] << kColBits) | (msk[q] << 30);
+                }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; t++) pool.emplace_back(work, t);
+    for (auto &th : pool) th.join();
+}
+
 // Stable counting sort of the contributing columns of angles [a0, a1) by
 // their first tap; order receives (a1-a0)*nvalid column indices.  Also
 // fills pixstart: for each q, the first entry whose first tap is >=

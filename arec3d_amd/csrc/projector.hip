@@ -195,6 +195,51 @@ __global__ void build_strip_table(const uint32_t *__restrict__ order, const int6
     }
 }
 
+// Angle-pair table (design 3) for pairs [u0, u0 + nub): entry e of pair u
+// (order[e] = column | strip << 22 | present flags << 30, bit 30 angle 2u,
+// bit 31 angle 2u + 1) -> {column | d0 << 22 | d1 << 27, w0[3]}, {w1[3], 0},
+// d = first tap - (strip * kStripW - 2) in [0, kStripW + 1] for a present
+// angle, kStripGroups (and zero weights) for an absent one.
+template <int M>
+__global__ void build_pair_table(const uint32_t *__restrict__ order, const int64_t *__restrict__ base, int u0,
+                                 int nub, int na, const int *__restrict__ coord, const AngleConst *__restrict__ ang,
+                                 double radius, int xcent, uint4 *__restrict__ tab, int *__restrict__ err) {
+    const int ub = blockIdx.y;
+    if (ub >= nub) return;
+    const int u = u0 + ub;
+    const int64_t lo = base[u] - base[u0], n = base[u + 1] - base[u];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = order[lo + i];
+        const int j = (int)(o & (kMaxCols - 1)), sidx = (int)((o >> kColBits) & 255);
+        unsigned d[2];
+        Taps t[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            t[h] = Taps{0, 0.f, 0.f, 0.f};
+            d[h] = kStripGroups;
+            const int a = 2 * u + h;
+            if ((o >> (30 + h)) & 1u) {
+                if (a >= na) {
+                    atomicOr(err, 8);
+                    continue;
+                }
+                t[h] = tap_weights<M>(coord[2 * j], coord[2 * j + 1], ang[a], radius, xcent);
+                const int dd = t[h].p0 - (sidx * kStripW - 2);
+                if (dd < 0 || dd > kStripW + 1) {
+                    atomicOr(err, 8);
+                    t[h] = Taps{0, 0.f, 0.f, 0.f};
+                } else {
+                    d[h] = (unsigned)dd;
+                }
+            }
+        }
+        uint4 *e = tab + 2 * (base[u] + i);
+        e[0] = make_uint4((unsigned)j | (d[0] << kColBits) | (d[1] << (kColBits + 5)), __float_as_uint(t[0].w0),
+                          __float_as_uint(t[0].w1), __float_as_uint(t[0].w2));
+        e[1] = make_uint4(__float_as_uint(t[1].w0), __float_as_uint(t[1].w1), __float_as_uint(t[1].w2), 0u);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Forward projector
 // ---------------------------------------------------------------------------
@@ -617,6 +662,190 @@ __global__ __launch_bounds__(64 * kStripAngles) void forward_strip_kernel(
     forward_strip_body<RPL>(x, p, tab, base, off, partials, nstrips, nbins, nx, ny, na, nchunks, b);
 }
 
+// Forward projection, depth strips with angle pairs (design 3): as design 2,
+// but each wave walks one pair of angles through the strip with one table
+// whose entries carry both angles' offsets and weights, so each column's
+// rows are loaded once for the two angles (the union of the two angles'
+// columns is about 1.1 times either).  The ring holds 128 two-uint4
+// entries; each batch's fetch of 8 entries uses lanes 0-15, one uint4 each.
+struct PairRing {
+    uint4 *ring;  // this wave's 128 entries (256 uint4) in LDS
+    const uint4 *__restrict__ t;
+    int s0, last;
+    int fetched;  // first entry of the fetch in flight
+    uint4 pend;   // lane l < 16: half l & 1 of entry fetched + l / 2
+};
+
+__device__ __forceinline__ uint4 pair_half_load(const PairRing &R, int e, int h) {
+    return R.t[2 * (int64_t)min(e, R.last) + h];
+}
+
+// Word q (0-7) of entry idx (s0 <= idx <= last, in the ring), wave-uniform.
+__device__ __forceinline__ unsigned pair_ring_word(const PairRing &R, const int idx, const int q) {
+    const unsigned *r = reinterpret_cast<const unsigned *>(R.ring);
+    return __builtin_amdgcn_readfirstlane(r[8 * (idx & 127) + q]);
+}
+
+template <int RPL, bool FULL>
+__device__ __forceinline__ void pair_batch(PairRing &R, const int i, const float *__restrict__ x, const int ny,
+                                           const int y0, const int lane, float (&acc0)[kStripW + 4][RPL],
+                                           float (&acc1)[kStripW + 4][RPL]) {
+    if (lane < 16) R.ring[2 * ((R.fetched + (lane >> 1)) & 127) + (lane & 1)] = R.pend;
+    float v[kStripBatch][RPL];
+#pragma unroll
+    for (int k = 0; k < kStripBatch; k++) {
+        const int col = (int)(pair_ring_word(R, min(i + k, R.last), 0) & (kMaxCols - 1));
+        const float *__restrict__ xj = x + (size_t)col * ny + y0;
+        if constexpr (FULL && RPL == 2) {
+            const float2 q = *reinterpret_cast<const float2 *>(xj);
+            v[k][0] = q.x;
+            v[k][RPL > 1 ? 1 : 0] = q.y;
+        } else if constexpr (FULL && RPL == 1) {
+            v[k][0] = xj[0];
+        } else {
+#pragma unroll
+            for (int r = 0; r < RPL; r++) v[k][r] = (y0 + r < ny) ? xj[r] : 0.0f;
+        }
+    }
+    R.fetched = i + kStripAhead;
+    R.pend = pair_half_load(R, R.fetched + ((lane & 15) >> 1), lane & 1);
+    unsigned d0[kStripBatch], d1[kStripBatch], p0 = 0, p1 = 0;
+    float w0[kStripBatch][3], w1[kStripBatch][3];
+#pragma unroll
+    for (int k = 0; k < kStripBatch; k++) {
+        const int e = min(i + k, R.last);
+        const unsigned word = pair_ring_word(R, e, 0);
+        const bool in = i + k <= R.last;
+        d0[k] = in ? (word >> kColBits) & 31u : (unsigned)kStripGroups;
+        d1[k] = in ? word >> (kColBits + 5) : (unsigned)kStripGroups;
+        p0 |= 1u << d0[k];
+        p1 |= 1u << d1[k];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            w0[k][q] = __uint_as_float(pair_ring_word(R, e, 1 + q));
+            w1[k][q] = __uint_as_float(pair_ring_word(R, e, 4 + q));
+        }
+    }
+    strip_add_present<0, RPL>(acc0, p0, d0, w0, v);
+    strip_add_present<0, RPL>(acc1, p1, d1, w1, v);
+}
+
+template <int RPL, bool FULL>
+__device__ __forceinline__ void pair_walk(uint4 *ring, const float *__restrict__ x, const uint4 *__restrict__ t,
+                                          const int *__restrict__ o, const int nbins, const bool active,
+                                          const int ny, const int y0, const int lane,
+                                          float (&acc0)[kStripW + 4][RPL], float (&acc1)[kStripW + 4][RPL]) {
+    PairRing R;
+    R.ring = ring;
+    R.t = t;
+    R.s0 = __builtin_amdgcn_readfirstlane(o[0]);
+    R.last = __builtin_amdgcn_readfirstlane(o[nbins]) - 1;
+    const bool any = active && R.last >= R.s0;
+    if (any) {
+        static_assert(kStripAhead <= 64, "one entry per lane at the start");
+        if (lane < kStripAhead) {
+            const int slot = 2 * ((R.s0 + lane) & 127);
+            R.ring[slot] = pair_half_load(R, R.s0 + lane, 0);
+            R.ring[slot + 1] = pair_half_load(R, R.s0 + lane, 1);
+        }
+        R.fetched = R.s0 + kStripAhead;
+        R.pend = pair_half_load(R, R.fetched + ((lane & 15) >> 1), lane & 1);
+    }
+    int i = R.s0;
+    for (int bin = 0; bin < nbins; bin++) {
+        if (any) {
+            const int bend = __builtin_amdgcn_readfirstlane(o[bin + 1]);
+            for (; i < bend; i += kStripBatch) pair_batch<RPL, FULL>(R, i, x, ny, y0, lane, acc0, acc1);
+        }
+        __builtin_amdgcn_s_barrier();
+    }
+}
+
+// Pixels st*kStripW + q of angle a (rows y0 .. y0 + RPL - 1) from acc[q + 2];
+// the FP64 sum of their FP32 squares into partials (optional).
+template <int RPL>
+__device__ __forceinline__ void strip_store(const float (&acc)[kStripW + 4][RPL], float *__restrict__ p,
+                                            double *__restrict__ partials, const int a, const int st,
+                                            const int chunk, const int nchunks, const int nstrips, const int nx,
+                                            const int ny, const int y0, const int lane) {
+    double sq = 0.0;
+    const int u0 = st * kStripW;
+    float *__restrict__ pa = p + (size_t)a * nx * ny;
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+        const int y = y0 + r;
+        if (y >= ny) continue;
+#pragma unroll
+        for (int q = 0; q < kStripW; q++) {
+            if (u0 + q < nx) {
+                const float val = acc[q + 2][r];
+                pa[(size_t)y * nx + u0 + q] = val;
+                const float s2 = val * val;
+                sq += (double)s2;
+            }
+        }
+    }
+    if (partials) {
+#pragma unroll
+        for (int off2 = 32; off2 > 0; off2 >>= 1) sq += __shfl_xor(sq, off2, 64);
+        if (lane == 0) partials[((size_t)a * nchunks + chunk) * nstrips + st] = sq;
+    }
+}
+
+// Workgroup b: group of kStripAngles pairs b % ngroups (one per wave), then
+// detector strip, then row chunk.
+template <int RPL>
+__device__ __forceinline__ void forward_pair_body(const float *__restrict__ x, float *__restrict__ p,
+                                                  const uint4 *__restrict__ tab, const int64_t *__restrict__ base,
+                                                  const int *__restrict__ off, double *__restrict__ partials,
+                                                  const int nstrips, const int nbins, const int nx, const int ny,
+                                                  const int na, const int nchunks, int b) {
+    constexpr int ROWS = 64 * RPL;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int nu = (na + 1) / 2;
+    const int ngroups = (nu + kStripAngles - 1) / kStripAngles;
+    const int grp = b % ngroups;
+    const int rest = b / ngroups;
+    const int st = rest % nstrips;
+    const int chunk = rest / nstrips;
+    const int u_raw = grp * kStripAngles + wave;
+    const bool active = u_raw < nu;
+    const int u = active ? u_raw : nu - 1;
+    const int ybase = chunk * ROWS;
+    const int y0 = ybase + lane * RPL;
+    const bool full_rows = (ybase + ROWS <= ny);
+    const uint4 *__restrict__ t = tab + 2 * base[u];
+    const int *__restrict__ o = off + (size_t)u * ((size_t)nstrips * nbins + 1) + (size_t)st * nbins;
+    float acc0[kStripW + 4][RPL], acc1[kStripW + 4][RPL];
+#pragma unroll
+    for (int i = 0; i < kStripW + 4; i++)
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            acc0[i][r] = 0.0f;
+            acc1[i][r] = 0.0f;
+        }
+    __shared__ uint4 s_ring[kStripAngles][4 * 64];
+    if (full_rows)
+        pair_walk<RPL, true>(s_ring[wave], x, t, o, nbins, active, ny, y0, lane, acc0, acc1);
+    else
+        pair_walk<RPL, false>(s_ring[wave], x, t, o, nbins, active, ny, y0, lane, acc0, acc1);
+    if (!active) return;
+    strip_store<RPL>(acc0, p, partials, 2 * u, st, chunk, nchunks, nstrips, nx, ny, y0, lane);
+    if (2 * u + 1 < na) strip_store<RPL>(acc1, p, partials, 2 * u + 1, st, chunk, nchunks, nstrips, nx, ny, y0, lane);
+}
+
+template <int RPL>
+__global__ __launch_bounds__(64 * kStripAngles) void forward_pair_kernel(
+    const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ tab,
+    const int64_t *__restrict__ base, const int *__restrict__ off, double *__restrict__ partials, int nstrips,
+    int nbins, int nx, int ny, int na, int nchunks, const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    const int nblk = gridDim.x;
+    int b = blockIdx.x;
+    if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+    forward_pair_body<RPL>(x, p, tab, base, off, partials, nstrips, nbins, nx, ny, na, nchunks, b);
+}
+
 // Index of the problem that owns work item `it`: the last k with
 // item0(k) <= it (item0 ascending).  Uniform across the workgroup.  The
 // item0 values of the first kBatchLdsProbs problems are staged in LDS once
@@ -854,7 +1083,7 @@ __global__ __launch_bounds__(kBpThreads) void backward_batch_kernel(const BatchB
 // Host side
 // ---------------------------------------------------------------------------
 int64_t forward_partial_count(const Geometry &g) {
-    if (g.fwd_design == 2) return (int64_t)g.na * ((g.ny + 64 * g.strip_rpl - 1) / (64 * g.strip_rpl)) * g.nstrips;
+    if (g.fwd_design >= 2) return (int64_t)g.na * ((g.ny + 64 * g.strip_rpl - 1) / (64 * g.strip_rpl)) * g.nstrips;
     return (int64_t)g.na * g.fwd_chunks * g.nseg;
 }
 
@@ -1025,7 +1254,10 @@ int geometry_replan_forward(Geometry &g, int rpl, int min_groups, hipStream_t st
     return upload(&g.d_seg, g.seg, st);
 }
 
+static int build_pair_tables(Geometry &g, hipStream_t st);
+
 static int build_strip_tables(Geometry &g, hipStream_t st) {
+    if (g.strip_unit == 2) return build_pair_tables(g, st);
     geometry_strip_count(g);
     const int64_t total = g.strip_base[g.na];
     if (total * (int64_t)sizeof(uint4) > kMaxStripTableBytes) {
@@ -1076,6 +1308,62 @@ static int build_strip_tables(Geometry &g, hipStream_t st) {
     return 0;
 }
 
+// Design 3's table: the pair ordering on the host, the weights on the device.
+static int build_pair_tables(Geometry &g, hipStream_t st) {
+    if ((g.nx + kStripW - 1) / kStripW > 256) {
+        set_error("angle-pair forward table: at most 256 strips (detector width 4096)");
+        return AREC_ESIZE;
+    }
+    geometry_pair_count(g);
+    const int nu = (g.na + 1) / 2;
+    const int64_t total = g.strip_base[nu];
+    if (2 * total * (int64_t)sizeof(uint4) > kMaxStripTableBytes) {
+        set_error("angle-pair forward table too large for this geometry");
+        return AREC_ESIZE;
+    }
+    int rc;
+    AREC_HIP(hipMalloc((void **)&g.d_strip, 2 * sizeof(uint4) * std::max<int64_t>(1, total)));
+    if ((rc = upload(&g.d_strip_base, g.strip_base, st))) return rc;
+    AREC_HIP(hipMemsetAsync(g.d_err, 0, sizeof(int), st));
+    std::vector<uint32_t> order;
+    uint32_t *d_order = nullptr;
+    int64_t cap = 0;
+    for (int u0 = 0; u0 < nu;) {
+        int u1 = u0 + 1;
+        while (u1 < nu && g.strip_base[u1 + 1] - g.strip_base[u0] <= (int64_t)(64 << 20)) u1++;
+        geometry_pair_units(g, u0, u1, order);
+        AREC_HIP(hipStreamSynchronize(st));  // previous batch done with d_order
+        if ((int64_t)order.size() > cap) {
+            hipFree(d_order);
+            cap = (int64_t)order.size();
+            AREC_HIP(hipMalloc((void **)&d_order, sizeof(uint32_t) * std::max<int64_t>(1, cap)));
+        }
+        AREC_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(uint32_t) * order.size(), hipMemcpyHostToDevice, st));
+        int64_t most = 0;
+        for (int u = u0; u < u1; u++) most = std::max(most, g.strip_base[u + 1] - g.strip_base[u]);
+        dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((most + 255) / 256, 1024)), u1 - u0);
+        if (g.method == kSQ)
+            build_pair_table<kSQ><<<grid, 256, 0, st>>>(d_order, g.d_strip_base, u0, u1 - u0, g.na, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_strip, g.d_err);
+        else if (g.method == kKB)
+            build_pair_table<kKB><<<grid, 256, 0, st>>>(d_order, g.d_strip_base, u0, u1 - u0, g.na, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_strip, g.d_err);
+        else
+            build_pair_table<kLinear><<<grid, 256, 0, st>>>(d_order, g.d_strip_base, u0, u1 - u0, g.na, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_strip, g.d_err);
+        AREC_HIP(hipGetLastError());
+        u0 = u1;
+    }
+    AREC_HIP(hipStreamSynchronize(st));
+    hipFree(d_order);
+    if ((rc = upload(&g.d_strip_off, g.strip_off, st))) return rc;
+    AREC_HIP(hipStreamSynchronize(st));
+    int herr = 0;
+    AREC_HIP(hipMemcpy(&herr, g.d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) {
+        set_error("angle-pair table: device taps disagree with the host ordering");
+        return AREC_EGEOM;
+    }
+    return 0;
+}
+
 // Frees the depth-strip tables (device and host).
 static void release_strip_tables(Geometry &g) {
     hipFree(g.d_strip);
@@ -1088,8 +1376,8 @@ static void release_strip_tables(Geometry &g) {
     std::vector<int>().swap(g.strip_off);
 }
 
-// Selecting design 1 releases design 2's table (rebuilt when design 2 is
-// selected again).
+// Selecting design 1 releases design 2's or 3's table (rebuilt when that
+// design is selected again); switching between designs 2 and 3 rebuilds it.
 int geometry_set_forward(Geometry &g, int design, int rpl, int min_groups, hipStream_t st) {
     if (design == 1) {
         int rc = geometry_replan_forward(g, rpl, min_groups, st);
@@ -1102,32 +1390,57 @@ int geometry_set_forward(Geometry &g, int design, int rpl, int min_groups, hipSt
         }
         return rc;
     }
-    if (design != 2 || !g.device_ready) {
-        set_error("forward design must be 1 (sweep) or 2 (depth strips) on an uploaded geometry");
+    if ((design != 2 && design != 3) || !g.device_ready) {
+        set_error("forward design must be 1 (sweep), 2 (depth strips) or 3 (depth strips, angle pairs) on an "
+                  "uploaded geometry");
         return AREC_EINVAL;
     }
-    if (rpl == 0) rpl = (g.ny >= 256 && g.ny % 4 == 0) ? 4 : (g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1;
-    if (!fwd_rpl_allowed(g.ny, rpl)) {
-        set_error("forward plan: rows per lane must be 1, 2 or 4 and divide the row count");
+    if (rpl == 0)
+        rpl = design == 3 ? ((g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1)
+                          : (g.ny >= 256 && g.ny % 4 == 0) ? 4 : (g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1;
+    if (!fwd_rpl_allowed(g.ny, rpl) || (design == 3 && rpl == 4)) {
+        set_error("forward plan: rows per lane must be 1, 2 or 4 (design 3: 1 or 2) and divide the row count");
         return AREC_EINVAL;
     }
+    const int unit = design == 3 ? 2 : 1;
+    if (g.d_strip && g.strip_unit != unit) {
+        AREC_HIP(hipStreamSynchronize(st));
+        release_strip_tables(g);
+    }
+    g.strip_unit = unit;
     if (!g.d_strip) {
         int rc = build_strip_tables(g, st);
         if (rc) {
             hipStreamSynchronize(st);
             release_strip_tables(g);
+            if (g.fwd_design != 1) g.fwd_design = 1;  // its table is gone; the sweep's tables always exist
             return rc;
         }
     }
     // design 1's plan keeps its own rows per lane and chunks
     g.strip_rpl = rpl;
-    g.fwd_design = 2;
+    g.fwd_design = design;
     return 0;
 }
 
 int launch_forward(const Geometry &g, const float *x, float *p, double *partials, hipStream_t st,
                    const int *skip) {
     if (g.nvalid == 0) return fill(p, 0.0f, g.nproj, st, skip);
+    if (g.fwd_design == 3) {
+        const int rpl = g.strip_rpl, nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
+        const int ngroups = ((g.na + 1) / 2 + kStripAngles - 1) / kStripAngles;
+        dim3 grid((unsigned)((int64_t)ngroups * g.nstrips * nchunks));
+        if (rpl == 2)
+            forward_pair_kernel<2><<<grid, 64 * kStripAngles, 0, st>>>(x, p, g.d_strip, g.d_strip_base, g.d_strip_off,
+                                                                      partials, g.nstrips, g.nbins, g.nx, g.ny, g.na,
+                                                                      nchunks, skip);
+        else
+            forward_pair_kernel<1><<<grid, 64 * kStripAngles, 0, st>>>(x, p, g.d_strip, g.d_strip_base, g.d_strip_off,
+                                                                      partials, g.nstrips, g.nbins, g.nx, g.ny, g.na,
+                                                                      nchunks, skip);
+        AREC_HIP(hipGetLastError());
+        return 0;
+    }
     if (g.fwd_design == 2) {
         const int rpl = g.strip_rpl, nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
         dim3 grid((unsigned)((int64_t)((g.na + kStripAngles - 1) / kStripAngles) * g.nstrips * nchunks));

@@ -100,9 +100,11 @@ int arec_geom_get_plan(const arec_geom *geom, int32_t *rows_per_lane, int32_t *m
  * sweep (the default; arec_geom_plan sets its plan), 2 = depth strips (16-pixel
  * detector strips walked in depth order by 4 neighbouring angles in
  * lockstep; its table, ~1.125 x the sweep table, is built when design 2 is
- * selected, refused above 32 GiB, and released when design 1 is selected).
- * Results agree to FP32 summation order.  arec_geom_get_plan reports the
- * selected design's rows per lane. */
+ * selected, refused above 32 GiB, and released when design 1 is selected),
+ * 3 = depth strips walked by angle pairs (each wave carries two angles, one
+ * table entry per column of either, rows per lane 1 or 2; table ~1.13 x the
+ * sweep table, same limits).  Results agree to FP32 summation order.
+ * arec_geom_get_plan reports the selected design's rows per lane. */
 int arec_geom_set_forward(arec_ctx *ctx, arec_geom *geom, int32_t design, int32_t rows_per_lane,
                           int32_t min_groups);
 int32_t arec_geom_forward_design(const arec_geom *geom);

@@ -178,14 +178,19 @@ def test_back_tap_table_and_computed_taps_agree(ctx, method, monkeypatch):
     assert np.array_equal(x1.download(), x2.download())
 
 
+@pytest.mark.parametrize("design", [2, 3])
 @pytest.mark.parametrize("method", ["sq", "kb", "linear"])
 @pytest.mark.parametrize("nx,ny,na,rpl", [(33, 5, 17, 1), (64, 70, 45, 2), (129, 132, 90, 4), (256, 3, 7, 1)])
-def test_forward_depth_strips_vs_oracle(ctx, method, nx, ny, na, rpl):
-    """Forward design 2 (depth strips): the same parity bar as the sweep
-    against the oracle, and the sweep's projections to FP32 summation order."""
+def test_forward_depth_strips_vs_oracle(ctx, design, method, nx, ny, na, rpl):
+    """Forward designs 2 (depth strips) and 3 (depth strips walked by angle
+    pairs; odd angle counts leave the last pair with one angle): the same
+    parity bar as the sweep against the oracle, and the sweep's projections
+    to FP32 summation order."""
     import arec3d_amd as A
 
-    print(f"\n[case] depth strips {method} nx={nx} ny={ny} na={na} rpl={rpl}", flush=True)
+    if design == 3:
+        rpl = min(rpl, 2)
+    print(f"\n[case] depth strips design {design} {method} nx={nx} ny={ny} na={na} rpl={rpl}", flush=True)
     rng = np.random.default_rng(nx * 7 + ny + na)
     ang = (np.arange(na) * np.pi / na).astype(np.float32)
     g = A.Geometry(ctx, method, nx, ny, ang)
@@ -195,12 +200,17 @@ def test_forward_depth_strips_vs_oracle(ctx, method, nx, ny, na, rpl):
     dx, dp = ctx.to_device(x), ctx.empty(g.proj_shape)
     g.project(dx, dp)
     sweep = dp.download()
-    g.set_forward(2, rpl)
-    assert g.forward_design == 2
+    g.set_forward(design, rpl)
+    assert g.forward_design == design
     g.project(dx, dp)
     strips = dp.download()
     op = O.project(method, x, ang, nx, ny)
     assert rel_views(strips, op).max() <= TOL
     assert rel_views(strips, sweep).max() <= TOL
+    other = 5 - design  # switching between designs 2 and 3 rebuilds the table
+    g.set_forward(other, min(rpl, 2))
+    assert g.forward_design == other
+    g.project(dx, dp)
+    assert rel_views(dp.download(), strips).max() <= TOL
     g.set_forward(1)
     assert g.forward_design == 1

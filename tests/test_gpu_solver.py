@@ -217,22 +217,23 @@ def test_sirt_config1_shape_rows_vs_oracle(ctx):
     assert relnorm(x.download()[:, r0:r0 + 4], xo) <= 1e-4
 
 
+@pytest.mark.parametrize("design", [2, 3])
 @pytest.mark.parametrize("method", ["sq", "kb", "linear"])
-def test_solvers_with_depth_strip_forward_vs_oracle(ctx, method):
-    """SIRT and CGLS with forward design 2 (depth strips, whose kernel also
-    delivers the residual-norm partials): the same bar against the oracle
-    as with the sweep."""
+def test_solvers_with_depth_strip_forward_vs_oracle(ctx, method, design):
+    """SIRT and CGLS with forward designs 2 and 3 (depth strips, angle
+    pairs; their kernels also deliver the residual-norm partials): the same
+    bar against the oracle as with the sweep."""
     import arec3d_amd as A
 
-    nx, ny, na = 65, 40, 60
-    print(f"\n[case] solvers with depth strips {method}", flush=True)
+    nx, ny, na = 65, 40, 61 if design == 3 else 60
+    print(f"\n[case] solvers with depth strips design {design} {method}", flush=True)
     b, ang = problem(method, nx, ny, na, 1)
     lam = 1.0 / (na * nx)
     _, xo, ho, lo = O.sirt(method, b, ang, lam, 30, 1e-6)
     _, _, ho64, _ = O.sirt(method, b, ang, lam, 30, 1e-6, dtype=np.float64)
     g = A.Geometry(ctx, method, nx, ny, ang)
-    g.set_forward(2, 0)
-    assert g.forward_design == 2
+    g.set_forward(design, 0)
+    assert g.forward_design == design
     db, dx = ctx.to_device(b), ctx.empty(g.vol_shape)
     hg, lg = g.sirt(db, dx, lam, 30, 1e-6)
     check_hist(hg, ho, ho64)
@@ -259,7 +260,7 @@ def test_autotune_designs(ctx, tmp_path):
     g = A.Geometry(ctx, "sq", nx, ny, ang)
     g.set_forward(1, 4, 0)
     base = g.device_bytes()
-    assert {c[0] for c in candidates(g)} == {1, 2}
+    assert {c[0] for c in candidates(g)} == {1, 2, 3}
     cache = str(tmp_path / "plans.json")
     d, r, m, ms = autotune(ctx, g, cache=cache, reps=1)
     assert g.forward_design == d and g.get_plan()[0] == r and ms > 0
