@@ -493,13 +493,23 @@ __device__ __forceinline__ void strip_add_batch(float (&acc)[kStripW + 4][RPL], 
     }
 }
 
-template <int D, int RPL>
+// strip_add_batch for every offset D in [LO, HI) whose bit is set in
+// present, by halving the range: a batch's one or two offsets cost about
+// 2 log2(kStripGroups) scalar tests instead of one per offset.
+template <int LO, int HI, int RPL>
 __device__ __forceinline__ void strip_add_present(float (&acc)[kStripW + 4][RPL], const unsigned present,
                                                   const unsigned (&dk)[kStripBatch],
                                                   const float (&w)[kStripBatch][3],
                                                   const float (&v)[kStripBatch][RPL]) {
-    if (present & (1u << D)) strip_add_batch<D, RPL>(acc, dk, w, v);
-    if constexpr (D + 1 < kStripGroups) strip_add_present<D + 1, RPL>(acc, present, dk, w, v);
+    if constexpr (HI - LO == 1) {
+        strip_add_batch<LO, RPL>(acc, dk, w, v);
+    } else {
+        constexpr int MID = (LO + HI) / 2;
+        constexpr unsigned ML = ((1u << MID) - 1u) & ~((1u << LO) - 1u);
+        constexpr unsigned MH = ((1u << HI) - 1u) & ~((1u << MID) - 1u);
+        if (present & ML) strip_add_present<LO, MID, RPL>(acc, present, dk, w, v);
+        if (present & MH) strip_add_present<MID, HI, RPL>(acc, present, dk, w, v);
+    }
 }
 
 // One batch: entries i .. i + kStripBatch - 1 of the strip (indices clamped
@@ -549,7 +559,7 @@ __device__ __forceinline__ void strip_batch(StripRing &R, const int i, const flo
 #pragma unroll
         for (int q = 0; q < 3; q++) w[k][q] = __uint_as_float(strip_ring_word(R, e, 1 + q));
     }
-    strip_add_present<0, RPL>(acc, present, dk, w, v);
+    strip_add_present<0, kStripGroups, RPL>(acc, present, dk, w, v);
 }
 
 // The wave's walk over its strip's entries [o[0], o[nbins]) in batches, with
@@ -726,8 +736,8 @@ __device__ __forceinline__ void pair_batch(PairRing &R, const int i, const float
             w1[k][q] = __uint_as_float(pair_ring_word(R, e, 4 + q));
         }
     }
-    strip_add_present<0, RPL>(acc0, p0, d0, w0, v);
-    strip_add_present<0, RPL>(acc1, p1, d1, w1, v);
+    strip_add_present<0, kStripGroups, RPL>(acc0, p0, d0, w0, v);
+    strip_add_present<0, kStripGroups, RPL>(acc1, p1, d1, w1, v);
 }
 
 template <int RPL, bool FULL>
