@@ -892,12 +892,13 @@ __device__ __forceinline__ int owner_of(const P *__restrict__ probs, const int *
 // Back projector
 // ---------------------------------------------------------------------------
 constexpr int kBpThreads = kTileCols;  // one voxel column per thread
-// Rows per workgroup: 16 keeps the kernel at 94 VGPRs (5 waves per SIMD);
-// measured at 512^3/720: 64 rows 27.5 ms, 32 rows 22.8 ms, 16 rows 20.8 ms,
-// 8 rows 22.0 ms; 2 or 8 angles per LDS pass were slower than 4.
-constexpr int kBpRows = 16;
+// Rows per workgroup and angles per LDS pass, measured at 512^3/720 with the
+// pipelined loop below: 32 rows x 2 angles 12.5 ms (128 VGPRs), 32 x 4
+// 12.9 ms, 16 x 4 14.3 ms, 16 x 8 14.9 ms, 64 x 2 14.3 ms, 24 x 4 15.8 ms,
+// 8 x 4 17.3 ms; the unpipelined loop took 20.8 ms at 16 x 4.
+constexpr int kBpRows = 32;
 constexpr int kBpRowPad = kBpRows + 4; // LDS row stride (16-byte aligned)
-constexpr int kBpAngBlk = 4;           // angles staged per LDS pass
+constexpr int kBpAngBlk = 2;           // angles staged per LDS pass
 constexpr int kBpWin = 32;             // staged pixels per (angle)
 
 template <int M>
@@ -923,9 +924,17 @@ __device__ __forceinline__ void backward_body(const float *__restrict__ p, float
                                               const AngleConst *__restrict__ ang, double *__restrict__ partials,
                                               int *__restrict__ err, const double radius, const int xcent,
                                               const int nx, const int ny, const int na, const int nrowblk, int b) {
-    __shared__ __align__(16) float s_p[kBpAngBlk][kBpWin][kBpRowPad];
-    __shared__ int s_u0[kBpAngBlk];
+    // Software pipeline over blocks of kBpAngBlk angles, one barrier per
+    // block: while block k is accumulated from s_p[k & 1], the footprints of
+    // block k + 1 are in flight to registers (then stored to s_p[(k+1) & 1])
+    // and the windows of block k + 2 are computed into s_u0[(k+2) % 3].  A
+    // buffer written in iteration k was last read before the barrier that
+    // ended iteration k - 1.
+    __shared__ __align__(16) float s_p[2][kBpAngBlk][kBpWin][kBpRowPad];
+    __shared__ int s_u0[3][kBpAngBlk];
     __shared__ double s_red[kBpThreads / 64];
+    constexpr int kStage = kBpRows * kBpWin / kBpThreads;  // staged values per thread and angle
+    static_assert(kStage * kBpThreads == kBpRows * kBpWin, "staging pattern");
 
     const int tid = threadIdx.x;
     const int tile = b / nrowblk;
@@ -942,10 +951,9 @@ __device__ __forceinline__ void backward_body(const float *__restrict__ p, float
 #pragma unroll
     for (int q = 0; q < kBpRows / 4; q++) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
 
-    for (int a0 = 0; a0 < na; a0 += kBpAngBlk) {
-        const int nb = min(kBpAngBlk, na - a0);
-        // footprint window of the box for each staged angle
-        if (tid < nb) {
+    // footprint windows of block a0's angles (threads < kBpAngBlk)
+    auto windows = [&](const int a0, int *su) __attribute__((always_inline)) {
+        if (tid < min(kBpAngBlk, na - a0)) {
             AngleConst A = ang[a0 + tid];
             double lo = 1e300, hi = -1e300;
 #pragma unroll
@@ -959,10 +967,52 @@ __device__ __forceinline__ void backward_body(const float *__restrict__ p, float
                 }
             int u0 = (int)floor(lo) - 2;
             if ((int)floor(hi) + 3 - u0 >= kBpWin) atomicOr(err, 2);
-            s_u0[tid] = u0;
+            su[tid] = u0;
         }
-        // this column's taps for the staged angles (registers): from the tap
-        // table (one coalesced 16-byte load per thread and angle) or computed
+    };
+    // block a0's footprints (rows fastest in LDS) into registers, then LDS
+    float stg[kBpAngBlk][kStage];
+    auto load_stage = [&](const int a0, const int *su) __attribute__((always_inline)) {
+        const int nb = min(kBpAngBlk, na - a0);
+#pragma unroll
+        for (int ab = 0; ab < kBpAngBlk; ab++) {
+            const int u0 = ab < nb ? su[ab] : 0;
+            const float *pa = p + (size_t)(a0 + (ab < nb ? ab : 0)) * ny * nx;
+#pragma unroll
+            for (int q = 0; q < kStage; q++) {
+                const int e = q * kBpThreads + tid;
+                const int row = e >> 5, w = e & (kBpWin - 1);
+                const int y = ybase + row, u = u0 + w;
+                float v = 0.0f;
+                if (ab < nb && y < ny && u >= 0 && u < nx) v = pa[(size_t)y * nx + u];
+                stg[ab][q] = v;
+            }
+        }
+    };
+    auto store_stage = [&](float (*sp)[kBpWin][kBpRowPad]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int ab = 0; ab < kBpAngBlk; ab++)
+#pragma unroll
+            for (int q = 0; q < kStage; q++) {
+                const int e = q * kBpThreads + tid;
+                sp[ab][e & (kBpWin - 1)][e >> 5] = stg[ab][q];
+            }
+    };
+
+    windows(0, s_u0[0]);
+    if (kBpAngBlk < na) windows(kBpAngBlk, s_u0[1]);
+    __syncthreads();
+    load_stage(0, s_u0[0]);
+    store_stage(s_p[0]);
+    __syncthreads();
+
+    int k = 0;
+    for (int a0 = 0; a0 < na; a0 += kBpAngBlk, k++) {
+        const int nb = min(kBpAngBlk, na - a0);
+        const int cur = k & 1, u_cur = k % 3;
+        const bool more = a0 + kBpAngBlk < na;
+        // this column's taps for block k (registers): from the tap table
+        // (one coalesced 16-byte load per thread and angle) or computed
         Taps t[kBpAngBlk];
 #pragma unroll
         for (int ab = 0; ab < kBpAngBlk; ab++) {
@@ -976,35 +1026,21 @@ __device__ __forceinline__ void backward_body(const float *__restrict__ p, float
                 }
             }
         }
-        __syncthreads();
-        // stage the footprints, rows fastest in LDS
-        for (int ab = 0; ab < nb; ab++) {
-            const int u0 = s_u0[ab];
-            const float *pa = p + (size_t)(a0 + ab) * ny * nx;
-#pragma unroll
-            for (int e0 = 0; e0 < kBpRows * kBpWin; e0 += kBpThreads) {
-                int e = e0 + tid;
-                int row = e >> 5, w = e & (kBpWin - 1);
-                int y = ybase + row, u = u0 + w;
-                float v = 0.0f;
-                if (y < ny && u >= 0 && u < nx) v = pa[(size_t)y * nx + u];
-                s_p[ab][w][row] = v;
-            }
-        }
-        __syncthreads();
+        if (more) load_stage(a0 + kBpAngBlk, s_u0[(k + 1) % 3]);
+        if (a0 + 2 * kBpAngBlk < na) windows(a0 + 2 * kBpAngBlk, s_u0[(k + 2) % 3]);
         if (j >= 0) {
 #pragma unroll
             for (int ab = 0; ab < kBpAngBlk; ab++) {
                 if (ab >= nb) break;
-                int o = t[ab].p0 - s_u0[ab];
+                int o = t[ab].p0 - s_u0[u_cur][ab];
                 if (o < 0 || o > kBpWin - 3) {  // outside the staged window
                     atomicOr(err, 4);
                     continue;
                 }
                 const float w0 = t[ab].w0, w1 = t[ab].w1, w2 = t[ab].w2;
-                const float4 *c0 = reinterpret_cast<const float4 *>(&s_p[ab][o][0]);
-                const float4 *c1 = reinterpret_cast<const float4 *>(&s_p[ab][o + 1][0]);
-                const float4 *c2 = reinterpret_cast<const float4 *>(&s_p[ab][o + 2][0]);
+                const float4 *c0 = reinterpret_cast<const float4 *>(&s_p[cur][ab][o][0]);
+                const float4 *c1 = reinterpret_cast<const float4 *>(&s_p[cur][ab][o + 1][0]);
+                const float4 *c2 = reinterpret_cast<const float4 *>(&s_p[cur][ab][o + 2][0]);
 #pragma unroll
                 for (int q = 0; q < kBpRows / 4; q++) {
                     float4 v0 = c0[q], v1 = c1[q], v2 = c2[q];
@@ -1017,6 +1053,7 @@ __device__ __forceinline__ void backward_body(const float *__restrict__ p, float
                 }
             }
         }
+        if (more) store_stage(s_p[cur ^ 1]);
         __syncthreads();
     }
 
