@@ -31,6 +31,8 @@ namespace arec {
 
 typedef short s4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
 typedef uint16_t bf16_t;
 
 __device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
@@ -84,11 +86,51 @@ __global__ __launch_bounds__(kCvThreads) void conv3_mfma_kernel(
     // of an output run in the order (tap row, channel block, dw tap): with
     // one block (Ct = 16) that is the tap order, otherwise the FP32 sums are
     // reassociated.
+    const bf8v zero8 = __builtin_bit_cast(bf8v, u4v{0u, 0u, 0u, 0u});
     for (int row = 0; row < 9; row++) {
         const int dz = row / 3 - 1, dh = row % 3 - 1;
         const int zz = z + dz;
         if (zz < 0 || zz >= d.D) continue;  // uniform
-        for (int cb = 0; cb < Ct; cb += 16) {
+        // 32-channel blocks on v_mfma_f32_16x16x32_bf16: lane l supplies
+        // channels 8 (l / 16) .. + 7 of the block (16 bytes per operand load)
+        int cb = 0;
+        for (; cb + 32 <= Ct; cb += 32) {
+            const int ci = cb + (lane >> 4) * 8;  // 8 channels within one 16-group: in1 or in2
+            bf8v a[3][R], b[3][NB];
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                const int pp = pa + t - 1;
+                const bool pv = pp >= 0 && pp < d.W;
+                const bf16_t *wt_tap = w + (size_t)(row * 3 + t) * COUT * Ct;
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++) {
+                    const int co = nb * 16 + (lane & 15);
+                    b[t][nb] = *reinterpret_cast<const bf8v *>(wt_tap + (size_t)co * Ct + ci);
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int hh = h0 + r + dh;
+                    a[t][r] = zero8;
+                    if (pv && hh >= 0 && hh < d.H) {
+                        const bf16_t *src;
+                        if (ci < C1)
+                            src = in1 + (((size_t)zz * d.H + hh) * d.W + pp) * C1 + ci;
+                        else
+                            src = in2 + (((size_t)(zz >> 1) * H2 + (hh >> 1)) * W2 + (pp >> 1)) * C2 + (ci - C1);
+                        a[t][r] = *reinterpret_cast<const bf8v *>(src);
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 3; t++)
+#pragma unroll
+                for (int r = 0; r < R; r++)
+#pragma unroll
+                    for (int nb = 0; nb < NB; nb++)
+                        acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t][r], b[t][nb], acc[r][nb], 0, 0, 0);
+        }
+        // a remaining 16-channel block on v_mfma_f32_16x16x16bf16_1k
+        for (; cb < Ct; cb += 16) {
             const int ci = cb + kq;  // the block lies in in1 or in in2 (C1 is a multiple of 16)
             s4v a[3][R], b[3][NB];
 #pragma unroll
