@@ -43,7 +43,8 @@ def test_cylvol_brute_force(r):
     assert O.cyl_geometry(r).tolist() == [list(t) for t in ref]
 
 
-def test_cylvol_counts_survey():
+def test_cylvol_counts_128_and_512():
+    # column counts of the 128- and 512-wide cylinders (radius (n-1)/2)
     assert len(O.cyl_geometry(63.5)) == 12900
     assert len(O.cyl_geometry(255.5)) == 206124
 
