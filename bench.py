@@ -21,6 +21,7 @@ CPU baseline (the oracle, test infrastructure, timed on a bounded sub-slab).
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -31,7 +32,54 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time to 1e-3 rel. residual, 512^3/720 angles: warm-start vs plain SIRT"
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_FLOPS = 157.3e12  # FP32 vector
+VALU_PEAK_FLOPS = 157.3e12  # FP32 vector (MI355X published; half of it without packed FP32 math)
+# Phantom and noise seed of the headline problem: disjoint from the seeds the
+# warm start's input iteration count was chosen on (49000-49002).
+HEADLINE_SEED = 49010
+
+
+def roofline(kernel, ms, vu, g, plan):
+    """Roofline of one projector launch (SQ: 3 taps, one FMA each = 6 flop
+    per voxel-angle update).  The binding bound is the FP32 vector pipe:
+    frac = (6 flop x VU) / launch time / 157.3 TFLOP/s.  Beside it, the
+    compulsory HBM bytes (read the input once, write the output once:
+    4 (nnz + nproj)) as a fraction of 8 TB/s, and -- when a rocprofv3
+    counter summary of the same kernel, workload and plan is committed under
+    profiles/ -- the counted HBM bytes per launch and their ratio to the
+    compulsory bytes (re-reads through L2).  The per-view streaming model
+    4 na (nnz + nx ny) is kept as a labelled model figure only: kernels that
+    reuse a column across angles move less than it."""
+    t = ms * 1e-3
+    flops = 6.0 * vu
+    compulsory = 4.0 * (g.nnz + g.nproj)
+    model = 4.0 * g.na * (g.nnz + g.nx * g.ny)
+    out = {
+        "bound": "valu",
+        "kernel": kernel,
+        "achieved": round(flops / t / 1e12, 3),
+        "peak": VALU_PEAK_FLOPS / 1e12,
+        "unit": "TFLOP/s",
+        "frac": round(flops / t / VALU_PEAK_FLOPS, 4),
+        "traffic": None,
+        "ms_per_launch": round(ms, 3),
+        "flop_per_launch": flops,
+        "flop_model": "6 flop per voxel-angle update (3 taps x FMA), SQ",
+        "frac_of_fp32_without_packed_math": round(flops / t / (VALU_PEAK_FLOPS / 2), 4),
+        "compulsory_bytes": compulsory,
+        "compulsory_frac_of_hbm": round(compulsory / t / (HBM_PEAK_GBS * 1e9), 5),
+        "model_bytes_per_view_streaming": model,
+        "model_rate_GBs": round(model / t / 1e9, 1),
+    }
+    pmc = os.path.join(ROOT, "profiles", f"pmc_sq_{g.nx}x{g.nx}x{g.ny}_{g.na}.json")
+    if os.path.exists(pmc):
+        summary = json.load(open(pmc))
+        k = summary.get(kernel)
+        if k and list(summary.get("plan", [])) == list(plan):
+            out["traffic"] = k["hbm_bytes_per_launch"]
+            out["traffic_over_compulsory"] = round(k["hbm_bytes_per_launch"] / compulsory, 2)
+            out["traffic_rate_frac_of_hbm"] = round(k["hbm_bytes_per_launch"] / t / (HBM_PEAK_GBS * 1e9), 4)
+            out["traffic_source"] = os.path.relpath(pmc, ROOT)
+    return out
 
 
 def log(*a):
@@ -57,20 +105,68 @@ def power_iteration(A, ctx, g, iters=20, seed=0):
     return lam
 
 
-def cpu_baseline(nx, na, iters_gpu, ny_full, rows_1t=2, rows_per_thread=1, threads=None):
-    """The FP32 oracle (the reference's loop order: angle -> column -> row)
-    timed on a sub-slab of rows of the same geometry -- rows are independent
-    and cost the same, so the time scales exactly with the row count.  One
-    thread on rows_1t rows (the reference's single-rank path), then T threads
-    on rows_per_thread rows each (the reference's MPI Y-slab split, as
-    threads: ctypes releases the GIL).  Reported at T threads, scaled to the
-    full height and to the GPU's plain-SIRT iteration count."""
+def host_cpu():
+    """lscpu's model, sockets and physical cores, and the cores this process
+    may use: its affinity set, capped by OMP_NUM_THREADS where the host sets
+    it (a GPU box gives each GPU a share of its cores that way)."""
+    info = {}
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.CalledProcessError):
+        pass
+    sockets = int(info.get("Socket(s)", "1") or 1)
+    per = int(info.get("Core(s) per socket", "0") or 0)
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if share > 0:
+        usable = min(usable, share)
+    physical = sockets * per or (os.cpu_count() or 1)
+    return {"model": info.get("Model name", "unknown"), "sockets": sockets, "physical_cores": physical,
+            "threads_per_core": int(info.get("Thread(s) per core", "1") or 1),
+            "usable_cores": max(1, min(usable, physical))}
+
+
+def timing_oracle():
+    """Build the oracle with the reference's release flags (-O3 -DNDEBUG
+    -march=native, AREC3D/Makefile:19-20) on this host, into a scratch
+    directory, and route the oracle through it.  Returns (description,
+    previous binding or None)."""
+    import tempfile
+
+    import pyoracle as O
+
+    d = tempfile.mkdtemp(prefix="arec_oracle_timing_")
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "fast", f"FASTDIR={d}"],
+                       capture_output=True, text=True)
+    lib = os.path.join(d, "liboracle.so")
+    if r.returncode != 0 or not os.path.exists(lib):
+        return f"parity build (-O2; the -O3 -march=native build failed: {r.stderr.strip()[-200:]})", None
+    return "-O3 -DNDEBUG -march=native, built on this host", O.use_build(lib)
+
+
+def cpu_baseline(nx, na, iters_gpu, ny_full, rows_1t=2, rows_per_thread=2):
+    """The FP32 oracle (the reference's loop order: angle -> column -> row,
+    built with the reference's release flags) timed on a sub-slab of rows of
+    the same geometry -- rows are independent and cost the same, so the time
+    scales exactly with the row count.  One thread on rows_1t rows (the
+    reference's single-rank path), then T threads on rows_per_thread rows
+    each (the reference's MPI Y-slab split, as threads: ctypes releases the
+    GIL), T = the cores this process may use.  Reported at T threads, scaled
+    to the full height and to the GPU's plain-SIRT iteration count."""
     import concurrent.futures as cf
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
-    T = threads or max(1, min(16, os.cpu_count() or 1))
+    cpu = host_cpu()
+    T = cpu["usable_cores"]
+    build, prev = timing_oracle()
+    log(f"CPU baseline: oracle ({build}) on {T} of {cpu['physical_cores']} physical cores ({cpu['model']})")
     ang = (np.arange(na) * np.pi / na).astype(np.float32)
     nr = len(O.cyl_geometry(0.5 * (nx - 1)))
     rng = np.random.default_rng(1)
@@ -83,24 +179,36 @@ def cpu_baseline(nx, na, iters_gpu, ny_full, rows_1t=2, rows_per_thread=1, threa
     t0 = time.perf_counter()
     one(rows_1t)
     dt1 = time.perf_counter() - t0
+    log(f"CPU baseline: 1 thread, {rows_1t} rows: {dt1:.2f} s")
     iter_1t = dt1 * ny_full / rows_1t
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(max_workers=T) as ex:
         list(ex.map(one, [rows_per_thread] * T))
     dtT = time.perf_counter() - t0
+    log(f"CPU baseline: {T} threads, {rows_per_thread * T} rows: {dtT:.2f} s")
+    if prev is not None:
+        O.restore_build(prev)
     iter_T = dtT * ny_full / (rows_per_thread * T)
     vu_row = 2 * nr * na  # forward + back, one row
+    rate_T = vu_row * rows_per_thread * T / dtT
     return {
         "value": iter_T * iters_gpu,
         "unit": "s",
         "cores": T,
         "kind": "port",
-        "sample": f"oracle SQ forward+back (one SIRT iteration) at {nx} wide, {na} angles: 1 thread on {rows_1t} "
-                  f"rows {dt1:.2f} s ({vu_row * rows_1t / dt1 / 1e6:.1f} MVU/s, {iter_1t * iters_gpu:.0f} s scaled); "
-                  f"{T} threads on {rows_per_thread * T} rows {dtT:.2f} s ({vu_row * rows_per_thread * T / dtT / 1e6:.1f} "
-                  f"MVU/s); scaled to {ny_full} rows x {iters_gpu} iterations (plain SIRT's count on the GPU)",
+        "sample": f"oracle SQ forward+back (one SIRT iteration) at {nx} wide, {na} angles, {build}: 1 thread on "
+                  f"{rows_1t} rows {dt1:.2f} s ({vu_row * rows_1t / dt1 / 1e6:.1f} MVU/s, {iter_1t * iters_gpu:.0f} s "
+                  f"scaled); {T} threads on {rows_per_thread * T} rows {dtT:.2f} s ({rate_T / 1e6:.1f} MVU/s); "
+                  f"scaled to {ny_full} rows x {iters_gpu} iterations (plain SIRT's count on the GPU)",
         "seconds_per_iteration_1_thread": iter_1t,
         "seconds_per_iteration": iter_T,
+        "build": build,
+        "cpu_model": cpu["model"],
+        "sockets": cpu["sockets"],
+        "physical_cores": cpu["physical_cores"],
+        "threads_per_core": cpu["threads_per_core"],
+        "extrapolated_all_physical_cores_s": iter_T * iters_gpu * T / cpu["physical_cores"],
+        "extrapolation_note": "linear in cores from the T-thread rate (rows are independent); not measured",
     }
 
 
@@ -390,12 +498,19 @@ def cgls_workload(args):
     gs = A.Geometry(ctx, "sq", nx, nr, ang)
     xs = ctx.empty(gs.vol_shape)
     hs = gs.cgls(ctx.to_device(bsub), xs, iters, 0.0)
+    log(f"GPU CGLS on rows {r0}..{r0 + nr} done; FP64 and FP32 oracle solves of those rows follow "
+        f"(one thread each, about {45 * nr} s)")
+    import concurrent.futures as cf
+
     t1 = time.perf_counter()
-    _, x64, h64 = O.cgls("sq", bsub, ang, iters, 0.0, dtype=np.float64)
+    with cf.ThreadPoolExecutor(max_workers=2) as ex:
+        f64 = ex.submit(O.cgls, "sq", bsub, ang, iters, 0.0, dtype=np.float64)
+        f32 = ex.submit(O.cgls, "sq", bsub, ang, iters, 0.0, dtype=np.float32)
+        _, x32, h32 = f32.result()
+        log(f"FP32 oracle on rows {r0}..{r0 + nr}: {time.perf_counter() - t1:.1f} s")
+        _, x64, h64 = f64.result()
     t_cpu = time.perf_counter() - t1
     log(f"FP64 oracle on rows {r0}..{r0 + nr}: {t_cpu:.1f} s")
-    _, x32, h32 = O.cgls("sq", bsub, ang, iters, 0.0, dtype=np.float32)
-    log("FP32 oracle done")
     xg = xs.download().astype(np.float64)
 
     def rel(a, b):
@@ -458,7 +573,7 @@ def main():
                          "batch: 64 uneven SIRT problems in one batch; cgls: CGLS 512x512x256/720 with an "
                          "FP64 check; rotate: the rotational alignment step at the full slab's view shape")
     ap.add_argument("--samples", type=int, default=64, help="batch workload: problems")
-    ap.add_argument("--cpu-rows-cgls", type=int, default=2, help="cgls workload: rows of the FP64 check")
+    ap.add_argument("--cpu-rows-cgls", type=int, default=1, help="cgls workload: rows of the FP64 check")
     ap.add_argument("--outer", type=int, default=5)
     ap.add_argument("--sirt-iters", type=int, default=20)
     ap.add_argument("--ellipsoids", type=int, default=200)
@@ -512,11 +627,12 @@ def main():
     log(f"forward design {plan[0]}: {plan[1]} rows/lane, >= {plan[2]} workgroups")
     support = 250.0 * nx / 512
     x_true = ctx.empty(g.vol_shape)
-    prm = P.ellipsoid_params(49000, 40, support, ny, axes=(8.0 * nx / 512, 120.0 * nx / 512), dens=(0.05, 1.0))
+    prm = P.ellipsoid_params(HEADLINE_SEED, 40, support, ny, axes=(8.0 * nx / 512, 120.0 * nx / 512),
+                             dens=(0.05, 1.0))
     P.phantom_device(ctx, x_true, g.radius, ny, support, prm)
     b = ctx.empty(g.proj_shape)
     g.project(x_true, b)
-    P.add_noise(ctx, b, 49000, rel_sigma=0.01)
+    P.add_noise(ctx, b, HEADLINE_SEED, rel_sigma=0.01)
     ctx.synchronize()
     sig2 = power_iteration(A, ctx, g)
     lam = 0.9 / sig2
@@ -574,17 +690,15 @@ def main():
     t_plain = float(np.median(times))
     n_it = int(np.median(iters))
 
-    # roofline of the projector pair: per-view streaming model
-    B = 4.0 * na * (g.nnz + nx * ny)
+    # roofline of the dominant projector kernel (timed with HIP events on
+    # the library's stream)
     vu = g.nnz * na
     p_tmp = ctx.empty(g.proj_shape)
     ms_f = g.time_operator(0, x, p_tmp, 5)
     ms_b = g.time_operator(1, p_tmp, x, 5)
     fwd_name = {1: "forward_kernel", 2: "forward_strip_kernel", 3: "forward_pair_kernel"}[plan[0]]
     dom, ms = (fwd_name, ms_f) if ms_f >= ms_b else ("backward_kernel", ms_b)
-    achieved = B / (ms * 1e-3) / 1e9
     log(f"forward {ms_f:.2f} ms ({vu / ms_f / 1e6:.1f} GVU/s), back {ms_b:.2f} ms ({vu / ms_b / 1e6:.1f} GVU/s)")
-
     if args.plain_only:
         value, n_steps = t_plain, len(times)
         workload = f"plain SIRT (SQ) to relnrm {args.tol:g}"
@@ -606,6 +720,7 @@ def main():
         "voxel_updates_per_s": round(2 * vu / ((ms_f + ms_b) * 1e-3), 1),
         "valu_frac_forward": round(6.0 * vu / (ms_f * 1e-3) / VALU_PEAK_FLOPS, 4),
         "valu_frac_backward": round(6.0 * vu / (ms_b * 1e-3) / VALU_PEAK_FLOPS, 4),
+        "phantom_seed": HEADLINE_SEED,
     }
     if not args.plain_only:
         cfg.update({
@@ -631,25 +746,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded ellipsoid phantom, 1% Gaussian noise)",
         "config": cfg,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "model": "algorithmic bytes per launch 4*na*(nnz + nx*ny) (per-view streaming)",
-        },
+        "roofline": roofline(dom, ms, vu, g, plan),
     }
-    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of the
-    # same kernel, workload and forward plan), when such a summary exists
-    pmc = os.path.join(ROOT, "profiles", f"pmc_sq_{nx}x{nx}x{ny}_{na}.json")
-    if os.path.exists(pmc):
-        summary = json.load(open(pmc))
-        t = summary.get(dom)
-        if t and list(summary.get("plan", [])) == list(plan):
-            out["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nx, na, n_it, ny)
     print(json.dumps(out), flush=True)

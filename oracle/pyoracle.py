@@ -37,68 +37,91 @@ def build():
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
 
 
+def _bind(path):
+    L = C.CDLL(path)
+    L.orc_cyl_geometry.argtypes = [C.c_double, C.c_int, C.c_void_p]
+    L.orc_cyl_geometry.restype = C.c_int
+    for sfx, arr in (("f", _F), ("d", _D)):
+        getattr(L, f"orc_project_{sfx}").argtypes = [C.c_int, _I, C.c_int, C.c_double, C.c_int, C.c_int, arr, _F, C.c_int, arr]
+        getattr(L, f"orc_backproject_{sfx}").argtypes = [C.c_int, _I, C.c_int, C.c_double, C.c_int, C.c_int, arr, _F, C.c_int, arr]
+    L.orc_sirt_f.argtypes = [C.c_int, _F, C.c_int, C.c_int, C.c_int, _F, C.c_float, C.c_int, C.c_float, _F,
+                             C.POINTER(Hist), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]
+    L.orc_sirt_d.argtypes = [C.c_int, _D, C.c_int, C.c_int, C.c_int, _F, C.c_double, C.c_int, C.c_double, _D,
+                             C.POINTER(Hist), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.orc_cgls_f.argtypes = [C.c_int, _F, C.c_int, C.c_int, C.c_int, _F, C.c_int, C.c_float, C.c_int, _F,
+                             C.POINTER(Hist), C.c_int, C.POINTER(C.c_int)]
+    L.orc_cgls_d.argtypes = [C.c_int, _D, C.c_int, C.c_int, C.c_int, _F, C.c_int, C.c_double, C.c_int, _D,
+                             C.POINTER(Hist), C.c_int, C.POINTER(C.c_int)]
+    L.orc_cgls_stat_f.argtypes = L.orc_cgls_f.argtypes
+    L.orc_cgls_stat_d.argtypes = L.orc_cgls_d.argtypes
+    L.orc_init_weights_f.argtypes = [_F, C.c_int, C.c_int, C.c_int, _F]
+    L.orc_ccorr2d.argtypes = [C.c_int, C.c_int, _F, _F, _D]
+    L.orc_quad2dfit.argtypes = [_F, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_speak.argtypes = [C.c_int, C.c_int, _F, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_fshift2d.argtypes = [C.c_int, C.c_int, _F, _F, C.c_float, C.c_float]
+    L.orc_cc_images.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, _F, _F]
+    L.orc_shift_images.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, _F]
+    L.orc_soft_noneg.argtypes = [_F, C.c_long]
+    L.orc_take_log.argtypes = [_F, C.c_long]
+    L.orc_periodic_smooth.argtypes = [C.c_int, C.c_int, _F, _F]
+    L.orc_periodic_decomposition.argtypes = [C.c_int, C.c_int, C.c_int, _F]
+    L.orc_radial_profile.argtypes = [C.c_int, C.c_int, _F, C.c_int, C.c_int, _F]
+    L.orc_ccorr1d.argtypes = [C.c_int, _F, _F, _D]
+    L.orc_findpeak1d.argtypes = [C.c_int, _F]
+    L.orc_findpeak1d.restype = C.c_float
+    L.orc_rot_cc_images.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, C.c_int, _F]
+    L.orc_shear_x.argtypes = [_F, C.c_float, C.c_int, C.c_int]
+    L.orc_shear_y.argtypes = [_F, C.c_float, C.c_int, C.c_int]
+    L.orc_rotate2d.argtypes = [C.c_int, C.c_int, _F, _F, C.c_float]
+    L.orc_rotate_skew.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F]
+    L.orc_tukey_ncc.argtypes = [_F, _F, C.c_int, C.c_int]
+    L.orc_tukey_ncc.restype = C.c_double
+    L.orc_rotate_skew_safe.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, _F, C.POINTER(C.c_double)]
+    L.orc_rotate_skew_safe.restype = C.c_int
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        L.orc_cyl_geometry.argtypes = [C.c_double, C.c_int, C.c_void_p]
-        L.orc_cyl_geometry.restype = C.c_int
-        for sfx, arr in (("f", _F), ("d", _D)):
-            getattr(L, f"orc_project_{sfx}").argtypes = [C.c_int, _I, C.c_int, C.c_double, C.c_int, C.c_int, arr, _F, C.c_int, arr]
-            getattr(L, f"orc_backproject_{sfx}").argtypes = [C.c_int, _I, C.c_int, C.c_double, C.c_int, C.c_int, arr, _F, C.c_int, arr]
-        L.orc_sirt_f.argtypes = [C.c_int, _F, C.c_int, C.c_int, C.c_int, _F, C.c_float, C.c_int, C.c_float, _F,
-                                 C.POINTER(Hist), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]
-        L.orc_sirt_d.argtypes = [C.c_int, _D, C.c_int, C.c_int, C.c_int, _F, C.c_double, C.c_int, C.c_double, _D,
-                                 C.POINTER(Hist), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
-        L.orc_cgls_f.argtypes = [C.c_int, _F, C.c_int, C.c_int, C.c_int, _F, C.c_int, C.c_float, C.c_int, _F,
-                                 C.POINTER(Hist), C.c_int, C.POINTER(C.c_int)]
-        L.orc_cgls_d.argtypes = [C.c_int, _D, C.c_int, C.c_int, C.c_int, _F, C.c_int, C.c_double, C.c_int, _D,
-                                 C.POINTER(Hist), C.c_int, C.POINTER(C.c_int)]
-        L.orc_cgls_stat_f.argtypes = L.orc_cgls_f.argtypes
-        L.orc_cgls_stat_d.argtypes = L.orc_cgls_d.argtypes
-        L.orc_init_weights_f.argtypes = [_F, C.c_int, C.c_int, C.c_int, _F]
-        L.orc_ccorr2d.argtypes = [C.c_int, C.c_int, _F, _F, _D]
-        L.orc_quad2dfit.argtypes = [_F, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        L.orc_speak.argtypes = [C.c_int, C.c_int, _F, C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        L.orc_fshift2d.argtypes = [C.c_int, C.c_int, _F, _F, C.c_float, C.c_float]
-        L.orc_cc_images.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, _F, _F]
-        L.orc_shift_images.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, _F]
-        L.orc_soft_noneg.argtypes = [_F, C.c_long]
-        L.orc_take_log.argtypes = [_F, C.c_long]
-        L.orc_periodic_smooth.argtypes = [C.c_int, C.c_int, _F, _F]
-        L.orc_periodic_decomposition.argtypes = [C.c_int, C.c_int, C.c_int, _F]
-        L.orc_radial_profile.argtypes = [C.c_int, C.c_int, _F, C.c_int, C.c_int, _F]
-        L.orc_ccorr1d.argtypes = [C.c_int, _F, _F, _D]
-        L.orc_findpeak1d.argtypes = [C.c_int, _F]
-        L.orc_findpeak1d.restype = C.c_float
-        L.orc_rot_cc_images.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, C.c_int, _F]
-        L.orc_shear_x.argtypes = [_F, C.c_float, C.c_int, C.c_int]
-        L.orc_shear_y.argtypes = [_F, C.c_float, C.c_int, C.c_int]
-        L.orc_rotate2d.argtypes = [C.c_int, C.c_int, _F, _F, C.c_float]
-        L.orc_rotate_skew.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F]
-        L.orc_tukey_ncc.argtypes = [_F, _F, C.c_int, C.c_int]
-        L.orc_tukey_ncc.restype = C.c_double
-        L.orc_rotate_skew_safe.argtypes = [C.c_int, C.c_int, C.c_int, _F, _F, _F, C.POINTER(C.c_double)]
-        L.orc_rotate_skew_safe.restype = C.c_int
-        _lib = L
+        _lib = _bind(LIB_PATH)
     return _lib
+
+
+def use_build(path):
+    """Route every oracle function through another build of the same source
+    (the -O3 -march=native timing build of ``make fast``); returns the
+    binding it replaces.  Parity checks use the default build."""
+    global _lib
+    prev = lib()
+    _lib = _bind(path)
+    return prev
+
+
+def restore_build(prev):
+    global _lib
+    _lib = prev
 
 
 def ref_available():
     return os.path.exists(REF_PATH)
 
 
-def ref():
+def ref(path=None):
+    """The reference build (default) or, with a path, another build of the
+    same reference sources (``make ref-fast``), bound afresh."""
     global _ref
-    if _ref is None:
-        R = C.CDLL(REF_PATH)
+    if path is not None or _ref is None:
+        R = C.CDLL(path or REF_PATH)
         R.ref_cyl_geometry.argtypes = [C.c_double, C.c_int, C.POINTER(C.c_int), C.c_void_p]
         R.ref_project.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, _F, _F, C.c_int, _F]
         R.ref_backproject.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, _F, _F, C.c_int, _F]
         R.ref_solve.argtypes = [C.c_int, C.c_int, _F, C.c_int, C.c_int, C.c_int, _F, C.c_float, C.c_int,
                                 C.c_float, C.c_int, _F, C.POINTER(C.c_int)]
+        if path is not None:
+            return R
         _ref = R
     return _ref
 

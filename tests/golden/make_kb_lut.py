@@ -5,9 +5,9 @@ The KB projector (AREC3D/AREC3D/arecproject.cpp:192-198) interpolates a fixed
 to 5 significant digits, with round-off noise in the tail), so it cannot be
 regenerated from a formula; the projector needs the same numbers.  This script
 reads the decimal literals from the reference source once and writes them as
-float32 bit patterns to ``kb_lut.txt`` beside it.  It is run by hand in a
-container that holds the reference; the committed ``kb_lut.txt`` is what the
-build uses.
+float32 bit patterns to ``arec3d_amd/data/kb_lut.txt``.  It is a data recipe,
+run by hand in a container that holds the reference; the committed
+``kb_lut.txt`` is what the build uses.
 """
 import os
 import struct
@@ -16,7 +16,10 @@ import sys
 REF = "/root/reference/AREC3D/AREC3D/arecproject.cpp"
 
 
-def main(out=os.path.join(os.path.dirname(os.path.abspath(__file__)), "kb_lut.txt")):
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+
+def main(out=os.path.join(ROOT, "arec3d_amd", "data", "kb_lut.txt")):
     src = open(REF).read()
     start = src.index("float LUT[LUT_SIZE] = {") + len("float LUT[LUT_SIZE] = {")
     stop = src.index("};", start)

@@ -225,6 +225,55 @@ def test_cgls_matches_textbook_and_lstsq():
     np.testing.assert_allclose(xc.reshape(-1), xl, rtol=1e-3, atol=1e-4)
 
 
+def test_weighted_cgls_is_cgls_on_the_weighted_system():
+    """cyl_cgls*_stat (areccgls_stat.cpp:49-275) keeps s = w.(b - A x)
+    (:148-154), divides by sum w q^2 (:197-204) and updates s -= alpha w.q
+    (:212).  With W = diag(w) that is textbook CGLS on (sqrt(W) A,
+    sqrt(W) b): the same g = A^T W (b - A x), the same alpha and beta.  So
+    the first iterates equal that recurrence on the dense FP64 matrix, and
+    the converged volume is the weighted least-squares solution.  A weight
+    dropped from the residual, the norm or the update fails both checks."""
+    A, b, ang, nr = _small_problem(nx=11, ny=1, na=30, seed=8)
+    b = np.abs(b) / np.abs(b).max() * 1.5  # path integrals of 0..1.5: w = exp(-b) spans 0.22..1
+    w = O.init_weights(b.astype(np.float32)).astype(np.float64).reshape(-1)
+    assert w.min() == 0.0 and w.max() > 0.5 and np.ptp(w[w > 0]) > 0.3  # Tukey zeros, exp(-b) spread
+    sw = np.sqrt(w)
+    Aw, bw = A * sw[:, None], b.reshape(-1) * sw
+    x = np.zeros(A.shape[1])
+    s = bw.copy()
+    g = Aw.T @ s
+    p = g.copy()
+    gam = g @ g
+    gnorms = []
+    for k in range(4):
+        q = Aw @ p
+        alpha = gam / (q @ q)
+        x = x + alpha * p
+        s = s - alpha * q
+        g = Aw.T @ s
+        gam_new = g @ g
+        p = g + (gam_new / gam) * p
+        gam = gam_new
+        gnorms.append(np.sqrt(gam))
+    assert all(a > c for a, c in zip(gnorms, gnorms[1:]))  # no best-iterate restore at 4 iterations
+    st, xo, hist = O.cgls("sq", b, ang, 4, 1e-13, dtype=np.float64, weighted=True)
+    assert st == 0
+    np.testing.assert_allclose(xo.reshape(-1), x, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose([h[3] for h in hist[1:]], gnorms, rtol=1e-9)
+    # the FP32 solver follows the same recurrence to FP32 accuracy
+    st, xf, hf = O.cgls("sq", b.astype(np.float32), ang, 4, 1e-13, weighted=True)
+    assert st == 0 and np.linalg.norm(xf.reshape(-1) - x) <= 1e-4 * np.linalg.norm(x)
+    # run to convergence: the weighted normal equations are solved
+    st, xc, hist = O.cgls("sq", b, ang, 3 * A.shape[1], 1e-13, dtype=np.float64, weighted=True)
+    xl = np.linalg.lstsq(Aw, bw, rcond=None)[0]
+    r = np.linalg.norm(Aw.T @ (bw - Aw @ xc.reshape(-1))) / np.linalg.norm(Aw.T @ bw)
+    assert r < 1e-9
+    np.testing.assert_allclose(xc.reshape(-1), xl, rtol=1e-3, atol=1e-4)
+    # and the unweighted solution is a different point (the weights matter)
+    xu = np.linalg.lstsq(A, b.reshape(-1), rcond=None)[0]
+    assert np.linalg.norm(xu - xl) > 1e-3 * np.linalg.norm(xl)
+
+
 def test_sirt_restart_chain():
     A, b, ang, nr = _small_problem()
     st, x, hist, lam = O.sirt("sq", b.astype(np.float32), ang, 1e3, 10, 1e-9)
