@@ -73,6 +73,7 @@ def _load():
         "arec_add_noise": ([vp, vp, i64, C.c_uint64, i32, dbl, dbl, dbl], C.c_int),
         "arec_align_ccorr": ([vp, vp, vp, i32, i32, i32, vp], C.c_int),
         "arec_align_cc": ([vp, vp, vp, i32, i32, i32, vp, vp], C.c_int),
+        "arec_align_cc_noalloc": ([vp, vp, vp, i32, i32, i32, vp, vp], C.c_int),
         "arec_align_peak": ([vp, vp, i32, i32, i32, vp, vp], C.c_int),
         "arec_align_shift": ([vp, vp, i32, i32, i32, vp, vp, flt], C.c_int),
         "arec_h_cc_images": ([i32, i32, i32, vp, i32, i32, i32, vp, vp, vp], C.c_int),
@@ -132,7 +133,8 @@ EXPORTED = [
     "arec_cyl_nrays", "arec_geom_create", "arec_geom_destroy", "arec_geom_info",
     "arec_geom_device_bytes", "arec_project", "arec_backproject", "arec_sirt", "arec_cgls",
     "arec_init_weights", "arec_soft_noneg", "arec_dot", "arec_max", "arec_copy", "arec_time_operator", "arec_phantom",
-    "arec_add_noise", "arec_align_ccorr", "arec_align_cc", "arec_align_peak", "arec_align_shift",
+    "arec_add_noise", "arec_align_ccorr", "arec_align_cc", "arec_align_cc_noalloc", "arec_align_peak",
+    "arec_align_shift",
     "arec_h_cc_images", "arec_h_shift_images", "arec_sirt_warm", "arec_unet_create",
     "arec_unet_destroy", "arec_unet_nparams", "arec_unet_get_params", "arec_unet_set_params",
     "arec_unet_set_scale", "arec_unet_forward", "arec_unet_train_step", "arec_unet_gradient",
@@ -487,6 +489,13 @@ def align_cc(ctx, proj, meas, sx, sy):
     """Sub-pixel shifts of meas against proj (arecCCImages) into sx, sy."""
     nv, ny, nx = proj.shape
     _check(lib.arec_align_cc(ctx.h, proj.ptr, meas.ptr, nx, ny, nv, sx.ptr, sy.ptr))
+
+
+def align_cc_noalloc(ctx, proj, meas, sx, sy):
+    """align_cc through the entry point that allocates nothing (its work
+    buffers live in a fixed device array; calls overlapping on one GPU race)."""
+    nv, ny, nx = proj.shape
+    _check(lib.arec_align_cc_noalloc(ctx.h, proj.ptr, meas.ptr, nx, ny, nv, sx.ptr, sy.ptr))
 
 
 def align_peak(ctx, cmap, sx, sy):

@@ -17,6 +17,8 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "arec_internal.hpp"
@@ -41,16 +43,21 @@ static std::map<int, Twiddles> &twiddle_cache() {
     return c;
 }
 
+// w[k] = exp(-2 pi i k / n) in FP64, rounded to FP32, k < n.
+static void fill_twiddles(float2 *h, int n) {
+    const double pi = 3.141592653589793238462643;
+    for (int k = 0; k < n; k++) {
+        double a = -2.0 * pi * (double)k / (double)n;
+        h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+}
+
 static int get_twiddles(int n, hipStream_t st, float2 **out) {
     auto &c = twiddle_cache();
     auto it = c.find(n);
     if (it == c.end()) {
         std::vector<float2> h(n);
-        const double pi = 3.141592653589793238462643;
-        for (int k = 0; k < n; k++) {
-            double a = -2.0 * pi * (double)k / (double)n;
-            h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
-        }
+        fill_twiddles(h.data(), n);
         Twiddles t;
         t.n = n;
         AREC_HIP(hipMalloc((void **)&t.d, sizeof(float2) * n));
@@ -134,12 +141,16 @@ __global__ void copy2_kernel(float2 *__restrict__ dst, const float2 *__restrict_
         dst[i] = src[i];
 }
 
-// One batch of 1-D transforms along one axis.
+// One batch of 1-D transforms along one axis.  tw: the length's twiddles,
+// or NULL for the cached table.
 static int fft_axis(float2 *data, float2 *tmp, int n, int64_t estride, int64_t tstride,
-                    int64_t bstride, int ntrans, int nbatch, int dir, hipStream_t st) {
-    float2 *tw;
-    int rc = get_twiddles(n, st, &tw);
-    if (rc) return rc;
+                    int64_t bstride, int ntrans, int nbatch, int dir, hipStream_t st,
+                    const float2 *tw_given = nullptr) {
+    float2 *tw = const_cast<float2 *>(tw_given);
+    if (!tw) {
+        int rc = get_twiddles(n, st, &tw);
+        if (rc) return rc;
+    }
     if (n == 1) return 0;
     if ((n & (n - 1)) == 0 && n <= kFftLds) {
         int logn = 0;
@@ -161,10 +172,12 @@ static int fft_axis(float2 *data, float2 *tmp, int n, int64_t estride, int64_t t
     return 0;
 }
 
-// 2-D transform of nv views of [ny][nx] complex values.
-static int fft2(float2 *data, float2 *tmp, int nx, int ny, int nv, int dir, hipStream_t st) {
-    int rc = fft_axis(data, tmp, nx, 1, nx, (int64_t)nx * ny, ny, nv, dir, st);
-    if (!rc) rc = fft_axis(data, tmp, ny, nx, 1, (int64_t)nx * ny, nx, nv, dir, st);
+// 2-D transform of nv views of [ny][nx] complex values (twx, twy: the
+// twiddles of nx and ny, or NULL for the cached tables).
+static int fft2(float2 *data, float2 *tmp, int nx, int ny, int nv, int dir, hipStream_t st,
+                const float2 *twx = nullptr, const float2 *twy = nullptr) {
+    int rc = fft_axis(data, tmp, nx, 1, nx, (int64_t)nx * ny, ny, nv, dir, st, twx);
+    if (!rc) rc = fft_axis(data, tmp, ny, nx, 1, (int64_t)nx * ny, nx, nv, dir, st, twy);
     return rc;
 }
 
@@ -357,6 +370,35 @@ static int views_per_batch(int64_t nx, int64_t ny, int64_t nv) {
     return (int)std::min<int64_t>(vb, nv);
 }
 
+// The correlation of views [v0, v0 + n) in batches of vb views through the
+// work buffers a, b, t (vb*nx*ny complex values each) and cbuf (vb*nx*ny
+// floats, used when cmap_out is NULL).  Every buffer element a batch reads
+// it has written first.
+static int cc_batches(const float *proj, const float *meas, int nx, int ny, int nv, float *cmap_out, float *sx,
+                      float *sy, hipStream_t st, int vb, float2 *a, float2 *b, float2 *t, float *cbuf,
+                      const float2 *twx, const float2 *twy) {
+    const int64_t per = (int64_t)nx * ny;
+    int rc = 0;
+    for (int v0 = 0; v0 < nv; v0 += vb) {
+        int n = std::min(vb, nv - v0);
+        int64_t cnt = (int64_t)n * per;
+        pack_real_kernel<<<ew_grid(cnt), 256, 0, st>>>(proj + v0 * per, a, cnt);
+        pack_real_kernel<<<ew_grid(cnt), 256, 0, st>>>(meas + v0 * per, b, cnt);
+        if ((rc = fft2(a, t, nx, ny, n, -1, st, twx, twy)) || (rc = fft2(b, t, nx, ny, n, -1, st, twx, twy))) break;
+        conj_mul_kernel<<<ew_grid(cnt), 256, 0, st>>>(a, b, cnt);
+        if ((rc = fft2(a, t, nx, ny, n, +1, st, twx, twy))) break;
+        float *dst = cmap_out ? cmap_out + v0 * per : cbuf;
+        real_part_kernel<<<ew_grid(cnt), 256, 0, st>>>(a, dst, cnt, 1.0f);
+        if (sx && sy) speak_kernel<<<n, 256, 0, st>>>(dst, nx, ny, sx + v0, sy + v0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "cc_views");
+            break;
+        }
+    }
+    return rc;
+}
+
 // c (optional, [nv][ny][nx]) = correlation maps; sx, sy (optional) = peaks.
 int cc_views(const float *proj, const float *meas, int nx, int ny, int nv, float *cmap_out,
              float *sx, float *sy, hipStream_t st) {
@@ -367,25 +409,95 @@ int cc_views(const float *proj, const float *meas, int nx, int ny, int nv, float
     if ((rc = w.ensure((size_t)vb * per))) return rc;
     float *cbuf = nullptr;
     if (!cmap_out) AREC_HIP(hipMalloc((void **)&cbuf, sizeof(float) * vb * per));
-    for (int v0 = 0; v0 < nv; v0 += vb) {
-        int n = std::min(vb, nv - v0);
-        int64_t cnt = (int64_t)n * per;
-        pack_real_kernel<<<ew_grid(cnt), 256, 0, st>>>(proj + v0 * per, w.a, cnt);
-        pack_real_kernel<<<ew_grid(cnt), 256, 0, st>>>(meas + v0 * per, w.b, cnt);
-        if ((rc = fft2(w.a, w.t, nx, ny, n, -1, st)) || (rc = fft2(w.b, w.t, nx, ny, n, -1, st))) break;
-        conj_mul_kernel<<<ew_grid(cnt), 256, 0, st>>>(w.a, w.b, cnt);
-        if ((rc = fft2(w.a, w.t, nx, ny, n, +1, st))) break;
-        float *dst = cmap_out ? cmap_out + v0 * per : cbuf;
-        real_part_kernel<<<ew_grid(cnt), 256, 0, st>>>(w.a, dst, cnt, 1.0f);
-        if (sx && sy) speak_kernel<<<n, 256, 0, st>>>(dst, nx, ny, sx + v0, sy + v0);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) {
-            rc = hip_fail(e, "cc_views");
-            break;
-        }
-    }
+    rc = cc_batches(proj, meas, nx, ny, nv, cmap_out, sx, sy, st, vb, w.a, w.b, w.t, cbuf, nullptr, nullptr);
     hipStreamSynchronize(st);
     hipFree(cbuf);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Cross-correlation alignment without allocation.  Its work buffers and
+// twiddles live in `scratch`, a static device array of kNoAllocScratchBytes
+// that every GPU the library runs on holds until the process ends.  In host
+// code `scratch` names a host-side stand-in, so its device address is looked
+// up once per device (noalloc_setup, called by arec_create with the device
+// current) and only that address is handed to kernels and copies.  Calls
+// that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+constexpr int kNoAllocMaxSide = 32768;
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static void *g_scratch[kNoAllocMaxDevices];           // device address of scratch, per device
+static std::mutex g_scratch_mu;                        // guards g_scratch and g_tw_host
+static float2 g_tw_host[2 * kNoAllocMaxSide];          // twiddle staging (no host allocation)
+
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) {
+        set_error("no-allocation scratch: device index beyond the table");
+        return AREC_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return 0;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch[device] = p;
+    return 0;
+}
+
+static size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+// Bytes of scratch a batch of vb views of nx x ny needs: both lengths'
+// twiddles, three complex work buffers and one real correlation map.
+static size_t noalloc_bytes(int nx, int ny, int64_t vb) {
+    const size_t per = (size_t)nx * ny;
+    return align256(sizeof(float2) * ((size_t)nx + ny)) + 3 * align256(sizeof(float2) * per * vb) +
+           align256(sizeof(float) * per * vb);
+}
+
+int cc_views_noalloc(int device, const float *proj, const float *meas, int nx, int ny, int nv, float *sx, float *sy,
+                     hipStream_t st) {
+    if (nx > kNoAllocMaxSide || ny > kNoAllocMaxSide || noalloc_bytes(nx, ny, 1) > kNoAllocScratchBytes) {
+        set_error("arec_align_cc_noalloc: a view of " + std::to_string(nx) + " x " + std::to_string(ny) +
+                  " needs " + std::to_string(noalloc_bytes(nx, ny, 1)) + " bytes of scratch; the fixed scratch holds " +
+                  std::to_string(kNoAllocScratchBytes) + " (and sides up to " + std::to_string(kNoAllocMaxSide) + ")");
+        return AREC_ESIZE;
+    }
+    if (device < 0 || device >= kNoAllocMaxDevices) {
+        set_error("arec_align_cc_noalloc: device index beyond the scratch table");
+        return AREC_EINVAL;
+    }
+    if (!g_scratch[device]) {  // arec_create's lookup failed: once more, with the context's device current
+        AREC_HIP(hipSetDevice(device));
+        int rc = noalloc_setup(device);
+        if (rc) return rc;
+    }
+    const size_t per = (size_t)nx * ny;
+    const size_t room = kNoAllocScratchBytes - align256(sizeof(float2) * ((size_t)nx + ny)) - 4 * 256;
+    int64_t vb = std::max<int64_t>(1, std::min<int64_t>(nv, (int64_t)(room / (28 * per))));
+    while (vb > 1 && noalloc_bytes(nx, ny, vb) > kNoAllocScratchBytes) vb--;
+    unsigned char *base = static_cast<unsigned char *>(g_scratch[device]);
+    float2 *twx = reinterpret_cast<float2 *>(base);
+    float2 *twy = twx + nx;
+    unsigned char *q = base + align256(sizeof(float2) * ((size_t)nx + ny));
+    float2 *a = reinterpret_cast<float2 *>(q);
+    float2 *b = reinterpret_cast<float2 *>(q + align256(sizeof(float2) * per * vb));
+    float2 *t = reinterpret_cast<float2 *>(q + 2 * align256(sizeof(float2) * per * vb));
+    float *cbuf = reinterpret_cast<float *>(q + 3 * align256(sizeof(float2) * per * vb));
+    {
+        // the same FP32 twiddles as the cached tables, staged through a
+        // static host array; the copy completes before the lock is released
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        fill_twiddles(g_tw_host, nx);
+        fill_twiddles(g_tw_host + nx, ny);
+        AREC_HIP(hipMemcpyAsync(twx, g_tw_host, sizeof(float2) * ((size_t)nx + ny), hipMemcpyHostToDevice, st));
+        AREC_HIP(hipStreamSynchronize(st));
+    }
+    int rc = cc_batches(proj, meas, nx, ny, nv, nullptr, sx, sy, st, (int)vb, a, b, t, cbuf, twx, twy);
+    hipStreamSynchronize(st);
     return rc;
 }
 
@@ -579,6 +691,15 @@ int arec_align_cc(arec_ctx *ctx, const float *proj_dev, const float *meas_dev, i
         return AREC_EINVAL;
     }
     return cc_views(proj_dev, meas_dev, nx, ny, nviews, nullptr, sx_dev, sy_dev, ctx->stream);
+}
+
+int arec_align_cc_noalloc(arec_ctx *ctx, const float *proj_dev, const float *meas_dev, int32_t nx, int32_t ny,
+                          int32_t nviews, float *sx_dev, float *sy_dev) {
+    if (!ctx || !proj_dev || !meas_dev || !sx_dev || !sy_dev || nx < 1 || ny < 1 || nviews < 1) {
+        set_error("arec_align_cc_noalloc: bad argument");
+        return AREC_EINVAL;
+    }
+    return cc_views_noalloc(ctx->device, proj_dev, meas_dev, nx, ny, nviews, sx_dev, sy_dev, ctx->stream);
 }
 
 int arec_align_peak(arec_ctx *ctx, const float *c_dev, int32_t nx, int32_t ny, int32_t nviews,

@@ -221,6 +221,12 @@ bool use_graphs();
 
 // Alignment (align.hip): per-view cross-correlation maps and/or sub-pixel
 // peaks; Fourier shift of every view by sgn*(sx, sy).
+// Device address of the no-allocation scratch array for `device` (which must
+// be current), looked up once per device; cc_views_noalloc runs cc_views's
+// kernels on it (sx, sy required).
+int noalloc_setup(int device);
+int cc_views_noalloc(int device, const float *proj, const float *meas, int nx, int ny, int nv, float *sx, float *sy,
+                     hipStream_t st);
 int cc_views(const float *proj, const float *meas, int nx, int ny, int nv, float *cmap_out, float *sx,
              float *sy, hipStream_t st);
 int shift_views(float *data, int nx, int ny, int nv, const float *sx, const float *sy, float sgn,

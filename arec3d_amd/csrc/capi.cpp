@@ -47,6 +47,9 @@ int arec_create(arec_ctx **ctx, int device) {
     AREC_HIP(hipGetDeviceCount(&n));
     CHECK_ARG(device >= 0 && device < n, "no such HIP device");
     AREC_HIP(hipSetDevice(device));
+    // the no-allocation entry point's scratch address on this device; a
+    // failure here is reported by that entry point, which tries again
+    noalloc_setup(device);
     arec_ctx *c = new arec_ctx;
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -194,6 +194,18 @@ int arec_align_ccorr(arec_ctx *ctx, const float *x_dev, const float *y_dev, int3
  * align2dstack.cpp:1059-1166).  sx_dev, sy_dev: nviews floats each. */
 int arec_align_cc(arec_ctx *ctx, const float *proj_dev, const float *meas_dev, int32_t nx, int32_t ny,
                   int32_t nviews, float *sx_dev, float *sy_dev);
+/* arec_align_cc without allocation (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and results as arec_align_cc, bit for bit.  Its work buffers and
+ * FFT twiddles live in a fixed 128 MiB device array that each GPU the
+ * library runs on holds until the process ends; views run in batches that
+ * fit it.  A view needing more than the array (nx*ny above about 4.7 M
+ * pixels, or a side above 32768) is AREC_ESIZE.  It calls no hipMalloc or
+ * hipFree; every call on one GPU shares the array, so calls that overlap on
+ * one GPU race. */
+int arec_align_cc_noalloc(arec_ctx *ctx, const float *proj_dev, const float *meas_dev, int32_t nx, int32_t ny,
+                          int32_t nviews, float *sx_dev, float *sy_dev);
 /* speak (align2dstack.cpp:1059-1105) on nviews maps [ny][nx]. */
 int arec_align_peak(arec_ctx *ctx, const float *c_dev, int32_t nx, int32_t ny, int32_t nviews,
                     float *sx_dev, float *sy_dev);
