@@ -74,6 +74,10 @@ def _load():
         "arec_align_ccorr": ([vp, vp, vp, i32, i32, i32, vp], C.c_int),
         "arec_align_cc": ([vp, vp, vp, i32, i32, i32, vp, vp], C.c_int),
         "arec_align_cc_noalloc": ([vp, vp, vp, i32, i32, i32, vp, vp], C.c_int),
+        "arec_h_ccorr2d": ([i32, i32, vp, vp, vp], C.c_int),
+        "arec_h_fshift2d": ([i32, i32, vp, vp, C.c_float, C.c_float], C.c_int),
+        "arec_h_speak": ([i32, i32, vp, vp, vp], C.c_int),
+        "arec_quad2dfit": ([vp, vp, vp, vp], C.c_int),
         "arec_align_peak": ([vp, vp, i32, i32, i32, vp, vp], C.c_int),
         "arec_align_shift": ([vp, vp, i32, i32, i32, vp, vp, flt], C.c_int),
         "arec_h_cc_images": ([i32, i32, i32, vp, i32, i32, i32, vp, vp, vp], C.c_int),
@@ -134,7 +138,7 @@ EXPORTED = [
     "arec_geom_device_bytes", "arec_project", "arec_backproject", "arec_sirt", "arec_cgls",
     "arec_init_weights", "arec_soft_noneg", "arec_dot", "arec_max", "arec_copy", "arec_time_operator", "arec_phantom",
     "arec_add_noise", "arec_align_ccorr", "arec_align_cc", "arec_align_cc_noalloc", "arec_align_peak",
-    "arec_align_shift",
+    "arec_align_shift", "arec_h_ccorr2d", "arec_h_fshift2d", "arec_h_speak", "arec_quad2dfit",
     "arec_h_cc_images", "arec_h_shift_images", "arec_sirt_warm", "arec_unet_create",
     "arec_unet_destroy", "arec_unet_nparams", "arec_unet_get_params", "arec_unet_set_params",
     "arec_unet_set_scale", "arec_unet_forward", "arec_unet_train_step", "arec_unet_gradient",
@@ -508,6 +512,35 @@ def align_shift(ctx, data, sx, sy, sign=-1.0):
     """Fourier shift of every view by (sign*sx, sign*sy) (arecShiftImages: sign -1)."""
     nv, ny, nx = data.shape
     _check(lib.arec_align_shift(ctx.h, data.ptr, nx, ny, nv, sx.ptr, sy.ptr, sign))
+
+
+def ccorr2d(nx, ny, x, y, c):
+    """align2dstack.cpp:480-521 on host arrays [ny][nx]: c = IDFT(conj(DFT x) DFT y)."""
+    _check(lib.arec_h_ccorr2d(nx, ny, _ptr(_f32c(x)), _ptr(_f32c(y)), _ptr(c)))
+
+
+def fshift2d(nx, ny, imagein, imageout, sx, sy):
+    """align2dstack.cpp:566-630: imageout = imagein shifted by (+sx, +sy)."""
+    _check(lib.arec_h_fshift2d(nx, ny, _ptr(_f32c(imagein)), _ptr(imageout), float(sx), float(sy)))
+
+
+def speak(nx, ny, data):
+    """align2dstack.cpp:1059-1105: the sub-pixel peak (xt, yt) of one map."""
+    xt, yt = C.c_float(), C.c_float()
+    _check(lib.arec_h_speak(nx, ny, _ptr(_f32c(data)), C.byref(xt), C.byref(yt)))
+    return xt.value, yt.value
+
+
+def quad2dfit(z):
+    """align2dstack.cpp:1125-1166 on 9 values (host): (xsh, ysh, peakv); peakv
+    is NaN for a singular fit (the reference leaves it unset)."""
+    xs, ys, pv = C.c_float(), C.c_float(), C.c_float(float("nan"))
+    _check(lib.arec_quad2dfit(_ptr(_f32c(z)), C.byref(xs), C.byref(ys), C.byref(pv)))
+    return xs.value, ys.value, pv.value
+
+
+def _f32c(a):
+    return np.ascontiguousarray(a, np.float32)
 
 
 def arecCCImages(comm, images1, images2, sx, sy):

@@ -203,10 +203,13 @@ __global__ void real_part_kernel(const float2 *__restrict__ c, float *__restrict
 // Sub-pixel peak of a 3x3 neighbourhood (quad2dfit, align2dstack.cpp:1125-
 // 1166), FP64 on FP32 samples z[k], k = 3*row + column.  Least-squares fit
 // of q(u,v) = a + b u + c v + d u^2 + e u v + f v^2 on u, v in {-1, 0, 1}
-// (u along x): the grid's moments give b = Su/6, c = Sv/6, e = Suv/4,
-// d + f = (Suu + Svv)/2 - 2 S0/3 and d - f = (Suu - Svv)/2; the stationary
-// point solves [2d e; e 2f] (u, v) = -(b, c) and is clamped to [-1, 1].
-__device__ void quad2dfit_dev(const float *z, float *xsh, float *ysh) {
+// (u along x): the grid's moments give a = (S0 - 6 (d + f))/9,
+// b = Su/6, c = Sv/6, e = Suv/4, d + f = (Suu + Svv)/2 - 2 S0/3 and
+// d - f = (Suu - Svv)/2; the stationary point solves [2d e; e 2f] (u, v) =
+// -(b, c) and is clamped to [-1, 1]; *peakv (when not NULL) gets q at the
+// unclamped stationary point.  With a singular system the shifts are 0 and
+// *peakv is left as it was, as in the reference.  Host and device share it.
+__host__ __device__ inline void quad2dfit_dev(const float *z, float *xsh, float *ysh, float *peakv = nullptr) {
     double s0 = 0, su = 0, sv = 0, suu = 0, svv = 0, suv = 0;
     for (int k = 0; k < 9; k++) {
         double u = (double)(k % 3 - 1), v = (double)(k / 3 - 1), w = (double)z[k];
@@ -223,8 +226,13 @@ __device__ void quad2dfit_dev(const float *z, float *xsh, float *ysh) {
     double det = 4.0 * d * f - e * e;
     float xs = 0.0f, ys = 0.0f;
     if (det != 0.0) {
-        xs = (float)fmin(1.0, fmax(-1.0, (e * c - 2.0 * f * b) / det));
-        ys = (float)fmin(1.0, fmax(-1.0, (e * b - 2.0 * d * c) / det));
+        const double u = (e * c - 2.0 * f * b) / det, v = (e * b - 2.0 * d * c) / det;
+        xs = (float)fmin(1.0, fmax(-1.0, u));
+        ys = (float)fmin(1.0, fmax(-1.0, v));
+        if (peakv) {
+            const double a = (s0 - 6.0 * dpf) / 9.0;
+            *peakv = (float)(a + b * u + c * v + d * u * u + e * u * v + f * v * v);
+        }
     }
     *xsh = xs;
     *ysh = ys;
@@ -669,6 +677,37 @@ int view_moments(const float *views, int nx, int ny, int nv, double *mom_dev, hi
     return 0;
 }
 
+// One view through device copies: n floats of in1 (and of in2, when given)
+// are uploaded to a (b); `run` computes into the device buffers (c holds at
+// least max(n, n_out, 2) floats) and returns the device pointer whose n_out
+// floats are downloaded to `out`.
+template <typename F>
+static int one_view(const float *in1, const float *in2, int64_t n, float *out, int64_t n_out, const char *what,
+                    F run) {
+    hipStream_t st;
+    AREC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    float *a = nullptr, *b = nullptr, *c = nullptr;
+    int rc = 0;
+    if (hipMalloc((void **)&a, n * 4) != hipSuccess || (in2 && hipMalloc((void **)&b, n * 4) != hipSuccess) ||
+        hipMalloc((void **)&c, std::max<int64_t>(std::max<int64_t>(n, n_out), 2) * 4) != hipSuccess) {
+        rc = hip_fail(hipErrorOutOfMemory, what);
+    } else {
+        hipMemcpyAsync(a, in1, n * 4, hipMemcpyHostToDevice, st);
+        if (in2) hipMemcpyAsync(b, in2, n * 4, hipMemcpyHostToDevice, st);
+        const float *res = nullptr;
+        rc = run(a, b, c, st, &res);
+        if (!rc) {
+            hipMemcpyAsync(out, res, n_out * 4, hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) rc = hip_fail(hipGetLastError(), what);
+        }
+    }
+    hipFree(a);
+    hipFree(b);
+    hipFree(c);
+    hipStreamDestroy(st);
+    return rc;
+}
+
 }  // namespace arec
 
 using namespace arec;
@@ -804,6 +843,69 @@ int arec_h_shift_images(int32_t nx, int32_t ny, int32_t nz, float *data, const f
     hipFree(s);
     hipStreamDestroy(st);
     return rc;
+}
+
+// Host-pointer versions of the reference's per-view alignment helpers
+// (align2dstack.h:9,13,24-25, the drop-in's ccorr2d, fshift2d, speak and
+// quad2dfit): upload, compute on the GPU with the stack kernels above (one
+// view), download.  quad2dfit is 9 values in, 3 out: the same FP64
+// arithmetic as the speak kernel's fit, on the host.
+int arec_quad2dfit(const float *z, float *xsh, float *ysh, float *peakv) {
+    if (!z || !xsh || !ysh || !peakv) {
+        set_error("arec_quad2dfit: bad argument");
+        return AREC_EINVAL;
+    }
+    quad2dfit_dev(z, xsh, ysh, peakv);
+    return 0;
+}
+
+int arec_h_ccorr2d(int32_t nx, int32_t ny, const float *x, const float *y, float *c) {
+    if (!x || !y || !c || nx < 1 || ny < 1) {
+        set_error("arec_h_ccorr2d: bad argument");
+        return AREC_EINVAL;
+    }
+    const int64_t n = (int64_t)nx * ny;
+    return one_view(x, y, n, c, n, "arec_h_ccorr2d",
+                    [&](float *a, float *b, float *cm, hipStream_t st, const float **res) {
+                        *res = cm;
+                        return cc_views(a, b, nx, ny, 1, cm, nullptr, nullptr, st);
+                    });
+}
+
+int arec_h_speak(int32_t nx, int32_t ny, const float *data, float *xt, float *yt) {
+    if (!data || !xt || !yt || nx < 1 || ny < 1) {
+        set_error("arec_h_speak: bad argument");
+        return AREC_EINVAL;
+    }
+    float s[2];
+    int rc = one_view(data, nullptr, (int64_t)nx * ny, s, 2, "arec_h_speak",
+                      [&](float *a, float *, float *o, hipStream_t st, const float **res) {
+                          speak_kernel<<<1, 256, 0, st>>>(a, nx, ny, o, o + 1);
+                          *res = o;
+                          AREC_HIP(hipGetLastError());
+                          return 0;
+                      });
+    if (!rc) {
+        *xt = s[0];
+        *yt = s[1];
+    }
+    return rc;
+}
+
+int arec_h_fshift2d(int32_t nx, int32_t ny, const float *imagein, float *imageout, float sx, float sy) {
+    if (!imagein || !imageout || nx < 1 || ny < 1) {
+        set_error("arec_h_fshift2d: bad argument");
+        return AREC_EINVAL;
+    }
+    const int64_t n = (int64_t)nx * ny;
+    return one_view(imagein, nullptr, n, imageout, n, "arec_h_fshift2d",
+                    [&](float *a, float *, float *o, hipStream_t st, const float **res) {
+                        const float shift[2] = {sx, sy};
+                        AREC_HIP(hipMemcpyAsync(o, shift, sizeof shift, hipMemcpyHostToDevice, st));
+                        AREC_HIP(hipStreamSynchronize(st));  // shift is a stack array
+                        *res = a;
+                        return shift_views(a, nx, ny, 1, o, o + 1, 1.0f, st);
+                    });
 }
 
 // Host-pointer versions of the preprocessing (the drop-in's

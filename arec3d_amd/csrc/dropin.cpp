@@ -144,40 +144,93 @@ int arecAllocateCylImage(arecImage *cylimage, double radius, int height) {
     return cylimage->data ? 0 : -1;
 }
 
-// cube[iz][iy][ix] -> cyl[j][iy] (arecImage.cpp:88-161); status 1 for an
-// unset cylinder, -1 unallocated, -2 inconsistent counts.
+// cube[iz][iy][ix] -> cyl[j][iy] (arecImage.cpp:88-161).  As the reference:
+// the cube's own nx, ny, nz are walked, iz outer, ix inner; a column is in
+// the cylinder when (ix - c)^2 + (iz - c)^2 <= (int)(r^2) with the centre
+// c = (int)r (for a half-integer radius a different set than
+// arecAllocateCylImage's truncated mask, which the count check then
+// reports); every kept column takes the cube's ny values and rewrites cord.
+// Status 1 for an unset cylinder, -1 unallocated, -2 when the counts differ
+// from the cylinder's nnz / nrays.  Where the reference would write past the
+// cylinder's buffers (more columns in the cube than the cylinder holds) the
+// extra values are counted but not stored.
 int ImageCB2Cyl(arecImage cbimage, arecImage cylimage, int, int, double, int) {
     if (!cylimage.data) {
         fprintf(stderr, "ImageCB2Cyl: cylimage not properly allocated\n");
         return -1;
     }
-    if (cylimage.nnz <= 0 || cylimage.nrays <= 0 || cylimage.radius <= 0 || cylimage.height <= 0) return 1;
-    if (!cylimage.cord) return -1;
-    int nx = cbimage.nx, ny = cbimage.ny;
-    std::vector<int> cord(2 * (size_t)arec_cyl_nrays(cylimage.radius, NULL));
-    arec_cyl_nrays(cylimage.radius, cord.data());
-    int nrays = (int)(cord.size() / 2);
-    for (int j = 0; j < nrays && j < cylimage.nrays; j++) {
-        int ix = cord[2 * j], iz = cord[2 * j + 1];
-        for (int iy = 0; iy < ny && iy < cylimage.height; iy++)
-            cylimage.data[(size_t)j * ny + iy] = cbimage.data[((size_t)iz * ny + iy) * nx + ix];
-        cylimage.cord[2 * j] = ix;
-        cylimage.cord[2 * j + 1] = iz;
+    const double radius = cylimage.radius;
+    const int height = cylimage.height, nnz0 = cylimage.nnz, nrays0 = cylimage.nrays;
+    if (nnz0 <= 0 || nrays0 <= 0 || radius <= 0 || height <= 0) {
+        fprintf(stderr, "ImageCB2Cy: cylimage not properly created!\n");
+        return 1;
     }
-    return (nrays * ny != cylimage.nnz || nrays != cylimage.nrays) ? -2 : 0;
+    if (!cylimage.cord) {
+        fprintf(stderr, "ImageCB2Cyl: cylimage not properly allocated\n");
+        return -1;
+    }
+    const int nx = cbimage.nx, ny = cbimage.ny, nz = cbimage.nz;
+    const int cent = (int)radius, r2 = (int)(radius * radius);
+    int64_t nnz = 0, nrays = 0;
+    for (int iz = 0; iz < nz; iz++) {
+        const int zs = iz - cent;
+        for (int ix = 0; ix < nx; ix++) {
+            const int xs = ix - cent;
+            if (xs * xs + zs * zs > r2) continue;
+            for (int iy = 0; iy < ny; iy++, nnz++)
+                if (nnz < nnz0) cylimage.data[nnz] = cbimage.data[((size_t)iz * ny + iy) * nx + ix];
+            if (nrays < nrays0) {
+                cylimage.cord[2 * nrays] = ix;
+                cylimage.cord[2 * nrays + 1] = iz;
+            }
+            nrays++;
+        }
+    }
+    if (nnz != nnz0 || nrays != nrays0) {
+        fprintf(stderr, "ImageCB2Cyl: inconsistency!\n");
+        fprintf(stderr, "nnz, nnz0, nrays, nrays0 = %lld, %d, %lld, %d\n", (long long)nnz, nnz0, (long long)nrays,
+                nrays0);
+        return -2;
+    }
+    return 0;
 }
 
-// cyl -> cube (arecImage.cpp:163-232)
+// cyl[j][iy] -> cube[iz][iy][ix] (arecImage.cpp:163-232): status -1 for
+// unallocated data, 1 for sizes that disagree with the cube or the
+// cylinder, -2 for a column outside the cube or more values than nnz (the
+// reference's check lets one value past nnz through; here none is read).
 int ImageCyl2CB(arecImage cylimage, arecImage cbimage, int nx, int ny, int nz) {
-    if (!cylimage.data) return -1;
-    if (nx != cbimage.nx || ny != cbimage.ny || nz != cbimage.nz) return 1;
-    if (nx != 2 * cylimage.radius + 1 || nz != 2 * cylimage.radius + 1 || ny != cylimage.height) return 1;
-    if (!cbimage.data) return -1;
+    if (!cylimage.data) {
+        fprintf(stderr, "ImageCyl2CB: cylimage not properly allocated!\n");
+        return -1;
+    }
+    if (nx != cbimage.nx || ny != cbimage.ny || nz != cbimage.nz) {
+        fprintf(stderr, "ImageCyl2CB: inconsistent dimension\n");
+        return 1;
+    }
+    if (nx != 2 * cylimage.radius + 1 || nz != 2 * cylimage.radius + 1 || ny != cylimage.height) {
+        fprintf(stderr, "ImageCyl2CB: inconsistent dimension\n");
+        return 1;
+    }
+    if (!cbimage.data) {
+        fprintf(stderr, "ImageCyl2CB: cbimage not properly allocated!\n");
+        return -1;
+    }
+    int64_t nnz = 0;
     for (int j = 0; j < cylimage.nrays; j++) {
-        int ix = cylimage.cord[2 * j], iz = cylimage.cord[2 * j + 1];
-        if (ix >= nx || iz >= nz) return -2;
-        for (int iy = 0; iy < ny; iy++)
-            cbimage.data[((size_t)iz * ny + iy) * nx + ix] = cylimage.data[(size_t)j * ny + iy];
+        const int ix = cylimage.cord[2 * j], iz = cylimage.cord[2 * j + 1];
+        if (!(iz < nz && ix < nx)) {
+            printf("ImageCyl2CB: ix, iz, out of bound\n");
+            printf("ImageCyl2CB: ix = %d, iz = %d \n", ix, iz);
+            return -2;
+        }
+        for (int iy = 0; iy < ny; iy++, nnz++) {
+            if (nnz >= cylimage.nnz) {
+                printf("ImageCyl2CB: iz out of bound, nnz = %lld, nnz0 = %d\n", (long long)nnz, cylimage.nnz);
+                return -2;
+            }
+            cbimage.data[((size_t)iz * ny + iy) * nx + ix] = cylimage.data[nnz];
+        }
     }
     return 0;
 }
@@ -258,6 +311,22 @@ int arecCCImages(MPI_Comm comm, arecImage images1, arecImage images2, float *sx,
 
 void arecShiftImages(arecImage images, float *sx, float *sy) {
     arec_h_shift_images(images.nx, images.ny, images.nz, images.data, sx, sy);
+}
+
+// align2dstack.h:9,13,24-25: one view at a time, as the reference's callers
+// use them.
+void ccorr2d(int nx, int ny, float *x, float *y, float *c) {
+    if (arec_h_ccorr2d(nx, ny, x, y, c)) fprintf(stderr, "ccorr2d: %s\n", arec_last_error());
+}
+
+void fshift2d(int nx, int ny, float *imagein, float *imageout, float sx, float sy) {
+    if (arec_h_fshift2d(nx, ny, imagein, imageout, sx, sy)) fprintf(stderr, "fshift2d: %s\n", arec_last_error());
+}
+
+void quad2dfit(float *Z, float *XSH, float *YSH, float *PEAKV) { arec_quad2dfit(Z, XSH, YSH, PEAKV); }
+
+void speak(int nx, int ny, float *data, float *xt, float *yt) {
+    if (arec_h_speak(nx, ny, data, xt, yt)) fprintf(stderr, "speak: %s\n", arec_last_error());
 }
 
 // ---- rotational alignment -------------------------------------------------------

@@ -215,6 +215,19 @@ int arec_align_peak(arec_ctx *ctx, const float *c_dev, int32_t nx, int32_t ny, i
 int arec_align_shift(arec_ctx *ctx, float *data_dev, int32_t nx, int32_t ny, int32_t nviews,
                      const float *sx_dev, const float *sy_dev, float sign);
 
+/* The reference's per-view helpers on host arrays (align2dstack.h:9,13,
+ * 24-25), the drop-in's versions: ccorr2d (align2dstack.cpp:480-521), the
+ * correlation map c of one view pair [ny][nx]; fshift2d (:566-630), one
+ * image shifted by (+sx, +sy) into imageout (which may be imagein);
+ * speak (:1059-1105), the sub-pixel peak of one map; quad2dfit
+ * (:1125-1166), the 3x3 paraboloid fit (z[3*row + column]) with its peak
+ * value, on the host (no GPU; with a singular fit the shifts are 0 and
+ * *peakv is left as it was, as in the reference). */
+int arec_h_ccorr2d(int32_t nx, int32_t ny, const float *x, const float *y, float *c);
+int arec_h_fshift2d(int32_t nx, int32_t ny, const float *imagein, float *imageout, float sx, float sy);
+int arec_h_speak(int32_t nx, int32_t ny, const float *data, float *xt, float *yt);
+int arec_quad2dfit(const float *z, float *xsh, float *ysh, float *peakv);
+
 /* ---- preprocessing of the measured views (AREC3D.cpp:81,105) ---------- */
 /* -log of every value > 0, 0 elsewhere, in place (arecImageTakeLog,
  * arecImage.cpp:358-381; n values, e.g. nx*ny*nz of a view stack). */

@@ -74,6 +74,13 @@ int cyl_cgls_SQ_stat(MPI_Comm comm, arecImage images, float *angles, arecImage *
 int arecCCImages(MPI_Comm comm, arecImage images1, arecImage images2, float *sx, float *sy);
 void arecShiftImages(arecImage images, float *sx, float *sy);
 
+/* align2dstack.h:9,13,24-25 (align2dstack.cpp:480-521,566-630,1059-1166):
+ * one view's correlation map, Fourier shift, sub-pixel peak and 3x3 fit */
+void ccorr2d(int nx, int ny, float *x, float *y, float *c);
+void fshift2d(int nx, int ny, float *imagein, float *imageout, float sx, float sy);
+void quad2dfit(float *Z, float *XSH, float *YSH, float *PEAKV);
+void speak(int nx, int ny, float *data, float *xt, float *yt);
+
 /* arecImage.h:37,46-48 (rotational alignment) */
 int arecRotCCImages(MPI_Comm comm, arecImage images1, arecImage images2, int r2, float *angle);
 void arecRotateImages(arecImage *images, float *angles);

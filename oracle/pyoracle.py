@@ -209,6 +209,25 @@ def cgls(method, b, angles, maxit, tol, x0=None, dtype=np.float32, weighted=Fals
     return st, x, _hist(H, nh.value)
 
 
+def cyl_to_cube(x, radius):
+    """The cylinder x [nrays, ny] scattered into its bounding cube, zero
+    outside: ImageCyl2CB's scatter (arecImage.cpp:163-232, cube(ix, iy, iz)
+    = cylval(j*ny + iy) over the cord pairs) with the detector rows as the
+    slowest axis, [ny][iz][ix], the layout the device conversion writes."""
+    cord = cyl_geometry(radius)
+    n = int(round(2 * radius + 1))
+    cube = np.zeros((x.shape[1], n, n), x.dtype)
+    cube[:, cord[:, 1], cord[:, 0]] = x.T
+    return cube
+
+
+def cube_to_cyl(cube, radius):
+    """The cylinder's columns gathered from a [ny][iz][ix] cube, [nrays, ny]
+    (ImageCB2Cyl's gather over the allocator's cord, arecImage.cpp:88-161)."""
+    cord = cyl_geometry(radius)
+    return np.ascontiguousarray(cube[:, cord[:, 1], cord[:, 0]].T)
+
+
 def init_weights(b):
     na, ny, nx = b.shape
     w = np.zeros_like(b, dtype=np.float32)

@@ -53,3 +53,24 @@ def test_no_gpu_reports_error_not_crash():
         pytest.skip("a GPU is present")
     with pytest.raises(A.ArecError):
         A.Context(0)
+
+
+def test_quad2dfit_host_entry_matches_oracle():
+    """arec_quad2dfit (host code shared with the speak kernel's fit,
+    align2dstack.cpp:1125-1166) equals the oracle's fit bit for bit, peak value
+    included; a singular fit leaves the peak value unset, as the reference."""
+    import numpy as np
+
+    import arec3d_amd as A
+    import pyoracle as O
+
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        z = rng.standard_normal(9).astype(np.float32)
+        assert A.quad2dfit(z) == O.quad2dfit(z)
+    z = np.array([2.0 - 0.5 * (u - 0.3) ** 2 - 0.8 * (v + 0.4) ** 2 for v in (-1, 0, 1) for u in (-1, 0, 1)],
+                 np.float32)
+    xs, ys, pv = A.quad2dfit(z)
+    assert abs(xs - 0.3) < 1e-5 and abs(ys + 0.4) < 1e-5 and abs(pv - 2.0) < 1e-5
+    xs, ys, pv = A.quad2dfit(np.ones(9, np.float32))
+    assert xs == 0.0 and ys == 0.0 and np.isnan(pv)

@@ -32,8 +32,23 @@ def test_dropin_driver_links():
                 "arecRotCCImages(int, Image, Image, int, float*)", "arecRotateImages_skew_safe(Image*, Image*, float*)",
                 "arecRotateImages_skew(Image*, float*)", "arecRotateImages(Image*, float*)",
                 "arecReadImage(char*, Image*)", "arecWriteImage(char*, Image)",
-                "arecReadImageHeader(char*, int*, int*, int*)", "periodicDecomposition(Image)"]:
+                "arecReadImageHeader(char*, int*, int*, int*)", "periodicDecomposition(Image)",
+                "ccorr2d(int, int, float*, float*, float*)", "fshift2d(int, int, float*, float*, float, float)",
+                "quad2dfit(float*, float*, float*, float*)", "speak(int, int, float*, float*, float*)",
+                "ImageCB2Cyl(Image, Image, int, int, double, int)", "ImageCyl2CB(Image, Image, int, int, int)"]:
         assert sym in out, sym
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN) or not os.path.exists(f"{MPI}/include/mpi.h"),
+                    reason="drop-in library or MPI headers absent")
+def test_dropin_driver_host_checks():
+    """The checks that need no GPU: the cylinder, ImageCyl2CB / ImageCB2Cyl
+    with the reference's semantics (AREC3D_test.cpp:326-363; -2 for a
+    half-integer radius) and quad2dfit."""
+    exe = build_driver()
+    r = subprocess.run([exe, "--host"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-3000:], r.stderr[-3000:])
+    assert r.returncode == 0 and "all checks passed" in r.stdout
 
 
 @pytest.mark.gpu

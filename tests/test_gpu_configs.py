@@ -3,9 +3,9 @@
 * CGLS 512 x 512 x 256, 720 angles, Beer-Lambert data at 1e4 photons
   (areccgls.cpp:245-478): the full problem runs 30 iterations; a one-row
   instance of the same problem (CGLS couples rows through its scalars, so
-  the instance is solved as its own problem on both sides) matches the FP32
-  oracle's history and lies within 4x the FP32-vs-FP64 oracle spread of the
-  FP64 oracle's volume.
+  the instance is solved as its own problem on both sides) follows the FP64
+  oracle's history and volume within 4x the FP32 oracle's distance from it
+  (FP32 CGLS drifts on this noisy system, whatever the summation order).
 * Batched SIRT, 64 samples of uneven width, height and angle count, 100
   iterations: 4 sampled problems equal their single solves bit for bit, and
   the cheapest problem's rows equal the oracle's row solves.
@@ -25,7 +25,7 @@ import pytest
 
 import pyoracle as O
 from test_gpu_full_rows import column_blocks, oracle_back_rows, oracle_forward_rows, rel_views, workers
-from test_gpu_solver import check_hist, relnorm
+from test_gpu_solver import relnorm
 
 pytestmark = pytest.mark.gpu
 
@@ -61,7 +61,16 @@ def test_cgls_512x512x256_720(ctx):
     xg = xs.download()
     spread = relnorm(x32, x64)
     print(f"row {r0}: GPU vs FP64 {relnorm(xg, x64):.3e}, FP32 oracle vs FP64 {spread:.3e}")
-    check_hist(hs, h32, h64)
+    # FP32 CGLS on this noisy system drifts from FP64 and the drift does not
+    # shrink again: each printed relnrm of the GPU lies within 4x the largest
+    # FP32-oracle-vs-FP64 deviation so far of the FP64 value (iteration
+    # numbers and events exact)
+    assert len(hs) == len(h64) == len(h32)
+    env = 0.0
+    for k, (g, o, d) in enumerate(zip(hs, h32, h64)):
+        assert g[0] == d[0] and g[1] == d[1], (k, g, d)
+        env = max(env, abs(o[2] - d[2]) / d[2])
+        assert abs(g[2] - d[2]) / d[2] <= max(1e-5, 4 * env), (k, g, o, d, env)
     assert relnorm(xg, x64) <= max(1e-4, 4 * spread)
 
 

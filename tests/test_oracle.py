@@ -49,6 +49,20 @@ def test_cylvol_counts_128_and_512():
     assert len(O.cyl_geometry(255.5)) == 206124
 
 
+def test_cube_round_trip_reference_case():
+    """AREC3D_test.cpp:326-363: a radius-1, height-2 cylinder holding 0..9
+    scattered into its cube keeps its sum, and gathers back unchanged; the
+    rows are the cube's slowest axis and every other cube element is 0."""
+    x = np.arange(10, dtype=np.float32).reshape(5, 2)
+    cube = O.cyl_to_cube(x, 1.0)
+    assert cube.shape == (2, 3, 3) and cube.sum() == x.sum()
+    assert np.array_equal(O.cube_to_cyl(cube, 1.0), x)
+    # cord (1,0),(0,1),(1,1),(2,1),(1,2): column j at [row][iz][ix]
+    assert cube[1, 0, 1] == x[0, 1] and cube[0, 1, 0] == x[1, 0] and cube[1, 2, 1] == x[4, 1]
+    assert np.count_nonzero(cube) == 9  # x[0, 0] == 0 is the tenth value
+    assert cube[:, 0, 0].tolist() == [0, 0] and cube[:, 2, 2].tolist() == [0, 0]
+
+
 # ---- SQ known answers: AREC3D_test.cpp:97-189 ------------------------------
 
 def test_sq_projection_odd():
