@@ -6,10 +6,12 @@ designs to FP32 summation order; its speed depends on both.  Design 1 (the
 detector sweep) has plans (detector rows per lane, minimum workgroup count):
 rows per lane sets how many rows one wavefront carries (1, 2 or 4: one, two
 or four FP32 per lane per column read), the workgroup count how finely each
-angle's detector is split into segments.  Designs 2 (depth strips) and 3
-(depth strips walked by angle pairs) have only rows per lane; their table is
-built while timed and released again when design 1 wins.  :func:`autotune` times every candidate on the geometry with
-HIP events and keeps the fastest; results are cached in a JSON file keyed by
+angle's detector is split into segments.  Designs 2 (depth strips), 3
+(depth strips walked by angle pairs) and 4 (staged strips: groups of 8 angles
+through LDS-staged column chunks) have only rows per lane; their tables are
+built while timed and released again when another design wins.
+:func:`autotune` times every candidate on the geometry with HIP events and
+keeps the fastest; results are cached in a JSON file keyed by
 (method, nx, ny, na) so a shape is timed once.
 """
 import json
@@ -23,6 +25,8 @@ ROWS_PER_LANE = (4, 2, 1)
 MIN_GROUPS = (512, 1024, 2048, 4096)
 STRIP_ROWS_PER_LANE = (4, 2)
 PAIR_ROWS_PER_LANE = (2, 1)
+STAGED_ROWS_PER_LANE = (4, 2)
+DESIGNS = (1, 2, 3, 4)
 
 
 def default_cache():
@@ -31,7 +35,7 @@ def default_cache():
 
 
 def _key(g):
-    return f"designs:{g.method}:{g.nx}:{g.ny}:{g.na}"
+    return f"designs1-4:{g.method}:{g.nx}:{g.ny}:{g.na}"
 
 
 def _load(path):
@@ -42,7 +46,7 @@ def _load(path):
         return {}
 
 
-def candidates(g, designs=(1, 2, 3)):
+def candidates(g, designs=DESIGNS):
     """(design, rows_per_lane, min_groups) plans allowed for geometry g."""
     out = []
     if 1 in designs:
@@ -51,6 +55,8 @@ def candidates(g, designs=(1, 2, 3)):
         out += [(2, r, 0) for r in STRIP_ROWS_PER_LANE if g.ny % r == 0]
     if 3 in designs:
         out += [(3, r, 0) for r in PAIR_ROWS_PER_LANE if g.ny % r == 0]
+    if 4 in designs:
+        out += [(4, r, 0) for r in STAGED_ROWS_PER_LANE if g.ny % r == 0]
     return out
 
 
@@ -59,7 +65,7 @@ def apply(g, plan):
     g.set_forward(design, r, m)
 
 
-def autotune(ctx, g, cache=None, reps=3, log=None, designs=(1, 2, 3)):
+def autotune(ctx, g, cache=None, reps=3, log=None, designs=DESIGNS):
     """Pick and set the fastest forward design and plan for geometry g;
     returns (design, rows_per_lane, min_groups, ms)."""
     path = cache or default_cache()

@@ -49,6 +49,15 @@ constexpr int kDepthBin = 16;
 constexpr int kStripGroups = kStripW + 2;
 constexpr int kStripAngles = 4;  // angles (wavefronts) per depth-strip workgroup
 constexpr int kStripBatch = 4;   // depth-strip table entries whose loads are in flight together (8: 3% slower)
+// Staged-strip forward design (4): workgroups of kSgAngles neighbouring
+// angles walk one detector strip (kStripW pixels) through chunks of
+// kSgChunk voxel columns staged in LDS.  A group's columns that reach the
+// strip at any of its angles are listed once (the stage list), ordered by
+// the central angle's first tap and then by depth along its ray, and cut
+// into chunks; each angle's entries reference a chunk slot.
+constexpr int kSgAngles = 8;
+constexpr int kSgChunk = 32;
+constexpr int kSgEnt = 48;  // most entries of one (chunk, angle), padding included
 // The back projector's tap table is built when it takes at most this much.
 constexpr int64_t kMaxBackTableBytes = 16ll << 30;
 // The depth-strip forward table is built (on request) up to this size.
@@ -109,6 +118,20 @@ struct Geometry {
     uint4 *d_strip = nullptr;
     int64_t *d_strip_base = nullptr;
     int *d_strip_off = nullptr;
+    // Design 4 (staged strips; rows per lane in strip_rpl).  Angle group G
+    // holds angles [G*kSgAngles, ...); chunk K of (G, s) is global chunk
+    // sg_chunk_base[G*nstrips + s] + k.  sg_stage[K*kSgChunk + q] is the
+    // column in slot q of chunk K (-1: empty).  Entries are stored chunk
+    // after chunk, inside a chunk angle after angle (i = a - G*kSgAngles),
+    // inside an angle by d: sg_off[K*kSgAngles + i] is the first entry of
+    // (K, i), sg_off[nchunks*kSgAngles] the total.  Entry = {slot | d << 8,
+    // w0, w1, w2}, d = first tap - (s*kStripW - 2).
+    std::vector<int64_t> sg_chunk_base;  // ngroups * nstrips + 1
+    std::vector<int64_t> sg_group_ent;   // ngroups + 1: first entry of each group
+    int *d_sg_stage = nullptr;
+    int64_t *d_sg_chunk_base = nullptr;
+    int64_t *d_sg_off = nullptr;
+    uint4 *d_sg_tab = nullptr;
     bool device_ready = false;
 };
 
@@ -133,6 +156,17 @@ void geometry_strip_angles(Geometry &g, int a0, int a1, std::vector<uint32_t> &o
 // | present flags << 30) per entry, pair after pair.
 void geometry_pair_count(Geometry &g);
 void geometry_pair_units(Geometry &g, int u0, int u1, std::vector<uint32_t> &order);
+// Staged-strip ordering (design 4).  staged_count fills nstrips,
+// sg_chunk_base and sg_group_ent (host counting pass) and returns the most
+// entries of one (chunk, angle), padding included.  staged_groups fills,
+// for angle groups [G0, G1): stage (all chunks of those groups, kSgChunk
+// slots each, starting at chunk sg_chunk_base[G0*nstrips]), off (the groups'
+// sg_off values, kSgAngles per chunk) and, per entry of the groups in table
+// order, col (column; 0xffffffff for a zero-weight padding entry), ang
+// (angle) and meta (slot | d << 8 | strip << 16).
+int geometry_staged_count(Geometry &g);
+void geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage, std::vector<int64_t> &off,
+                            std::vector<uint32_t> &col, std::vector<uint32_t> &ang, std::vector<uint32_t> &meta);
 // Segment table of segments of width segw (nseg of them) for g's forward table.
 void segment_table(const Geometry &g, int segw, int nseg, std::vector<int> &seg);
 

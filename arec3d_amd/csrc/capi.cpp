@@ -157,7 +157,11 @@ int64_t arec_geom_device_bytes(const arec_geom *geom) {
     if (!geom) return 0;
     const Geometry &g = geom->g;
     const int64_t btab = (g.d_btab ? (int64_t)sizeof(uint4) * g.ntiles * g.na * kTileCols : 0) +
-                         (g.d_strip ? (int64_t)sizeof(uint4) * g.strip_unit * g.strip_base.back() : 0);
+                         (g.d_strip ? (int64_t)sizeof(uint4) * g.strip_unit * g.strip_base.back() : 0) +
+                         (g.d_sg_tab ? (int64_t)sizeof(uint4) * (g.sg_group_ent.back() + kStripBatch) +
+                                           g.sg_chunk_base.back() * (int64_t)(sizeof(int) * kSgChunk +
+                                                                              sizeof(int64_t) * kSgAngles)
+                                     : 0);
     return (int64_t)sizeof(uint4) * g.na * g.nvalid + btab + (int64_t)sizeof(AngleConst) * g.na +
            (int64_t)sizeof(int) * (g.coord.size() + g.tile_cols.size() + g.tile_org.size() + g.seg.size() +
                                    g.pixstart.size());

@@ -378,6 +378,205 @@ void geometry_pair_units(Geometry &g, int u0, int u1, std::vector<uint32_t> &ord
     for (auto &th : pool) th.join();
 }
 
+// Staged strips (design 4).  Angle group G holds angles [G*kSgAngles,
+// min(na, (G+1)*kSgAngles)); its central angle is the middle one.  A column
+// belongs to the stage list of (G, s) when strip s is one of its strips at
+// any of the group's angles, and to one entry of (G, s) per such angle, so
+// every (angle, strip, column) contribution of design 2 is in exactly one
+// entry.  The stage list is ordered by the central angle's first tap (bands
+// one pixel wide along its rays) and inside a band by depth along the ray:
+// a chunk of kSgChunk consecutive columns then spans a short stretch of one
+// or two bands, where each angle of the group sees one or two first-tap
+// offsets.  A (chunk, angle)'s entries are ordered by offset and every
+// offset's run is padded with zero-weight entries (column -1) to a multiple
+// of kStripBatch, so each batch of the walk has one offset; a chunk ends
+// early when one more column would give an angle more than kSgEnt entries
+// (groups of widely spaced angles see many offsets in a chunk).
+namespace {
+
+struct StagedItem {
+    int s, band;
+    double t;
+    int j;
+};
+
+// One angle group's chunks and entries, in table order: stage holds
+// kSgChunk slots per chunk of strips 0, 1, ...; nchunk[s] the chunks of
+// strip s; cnt kSgAngles entry counts per chunk; ent the entries {slot,
+// d, column (-1: padding)}, angle i's after angle i - 1's in each chunk.
+struct StagedGroup {
+    std::vector<int> p0;
+    std::vector<StagedItem> items;
+    std::vector<int> stage, nchunk, cnt;
+    std::vector<int4> ent;  // x slot, y d, z column, w angle
+};
+
+void staged_group(const Geometry &g, int G, StagedGroup &o) {
+    const int ns = g.nstrips, a0 = G * kSgAngles, na = std::min(kSgAngles, g.na - a0), ac = a0 + (na - 1) / 2;
+    o.p0.resize((size_t)g.nvalid * kSgAngles);
+    o.items.clear();
+    int sa[2], un[2 * kSgAngles];
+    for (int j = 0; j < g.nvalid; j++) {
+        int nu = 0;
+        for (int i = 0; i < na; i++) {
+            const int q = host_first_tap(g, a0 + i, j);
+            o.p0[(size_t)j * kSgAngles + i] = q;
+            const int n = strips_of(q, ns, sa);
+            for (int k = 0; k < n; k++) {
+                bool seen = false;
+                for (int u = 0; u < nu; u++) seen = seen || un[u] == sa[k];
+                if (!seen) un[nu++] = sa[k];
+            }
+        }
+        if (!nu) continue;
+        const int band = host_first_tap(g, ac, j);
+        const double t = host_depth(g, ac, j);
+        for (int u = 0; u < nu; u++) o.items.push_back({un[u], band, t, j});
+    }
+    std::sort(o.items.begin(), o.items.end(), [](const StagedItem &x, const StagedItem &y) {
+        if (x.s != y.s) return x.s < y.s;
+        if (x.band != y.band) return x.band < y.band;
+        if (x.t != y.t) return x.t < y.t;
+        return x.j < y.j;
+    });
+    o.stage.clear();
+    o.nchunk.assign(ns, 0);
+    o.cnt.clear();
+    o.ent.clear();
+    int runs[kStripGroups + 1];
+    int fill[kSgAngles][kStripGroups];  // per angle and offset: entries of the chunk being filled
+    size_t idx = 0;
+    for (int s = 0; s < ns; s++) {
+        size_t n = 0;
+        while (idx + n < o.items.size() && o.items[idx + n].s == s) n++;
+        for (size_t q0 = idx; q0 < idx + n;) {
+            // the chunk takes up to kSgChunk columns, fewer when one more
+            // column would give an angle more than kSgEnt entries once its
+            // offset runs are padded to whole batches (widely spaced angles)
+            int nq = 0;
+            std::fill(&fill[0][0], &fill[0][0] + kSgAngles * kStripGroups, 0);
+            while (nq < kSgChunk && q0 + nq < idx + n) {
+                const int j = o.items[q0 + nq].j;
+                bool fits = true;
+                for (int i = 0; i < na && fits; i++) {
+                    const int d = o.p0[(size_t)j * kSgAngles + i] - (s * kStripW - 2);
+                    if (d < 0 || d >= kStripGroups) continue;
+                    int padded = 0;
+                    for (int e = 0; e < kStripGroups; e++) {
+                        const int c = fill[i][e] + (e == d);
+                        padded += (c + kStripBatch - 1) / kStripBatch * kStripBatch;
+                    }
+                    fits = padded <= kSgEnt;
+                }
+                if (!fits) break;
+                for (int i = 0; i < na; i++) {
+                    const int d = o.p0[(size_t)j * kSgAngles + i] - (s * kStripW - 2);
+                    if (d >= 0 && d < kStripGroups) fill[i][d]++;
+                }
+                nq++;
+            }
+            o.nchunk[s]++;
+            for (int q = 0; q < kSgChunk; q++) o.stage.push_back(q < nq ? o.items[q0 + q].j : -1);
+            for (int i = 0; i < kSgAngles; i++) {
+                const size_t e0 = o.ent.size();
+                if (i < na) {
+                    std::fill(runs, runs + kStripGroups + 1, 0);
+                    for (int q = 0; q < nq; q++) {
+                        const int d = o.p0[(size_t)o.items[q0 + q].j * kSgAngles + i] - (s * kStripW - 2);
+                        if (d >= 0 && d < kStripGroups) runs[d]++;
+                    }
+                    for (int d = 0; d < kStripGroups; d++) {
+                        if (!runs[d]) continue;
+                        for (int q = 0; q < nq; q++) {
+                            const int j = o.items[q0 + q].j;
+                            if (o.p0[(size_t)j * kSgAngles + i] - (s * kStripW - 2) == d)
+                                o.ent.push_back(make_int4(q, d, j, a0 + i));
+                        }
+                        for (int k = runs[d]; k % kStripBatch; k++) o.ent.push_back(make_int4(0, d, -1, a0 + i));
+                    }
+                }
+                o.cnt.push_back((int)(o.ent.size() - e0));
+            }
+            q0 += nq;
+        }
+        idx += n;
+    }
+}
+
+unsigned staged_threads(int ngroups) {
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    return std::min<unsigned>(nth, (unsigned)std::max(1, ngroups));
+}
+
+}  // namespace
+
+int geometry_staged_count(Geometry &g) {
+    g.nstrips = (g.nx + kStripW - 1) / kStripW;
+    const int ns = g.nstrips, ngroups = (g.na + kSgAngles - 1) / kSgAngles;
+    std::vector<int64_t> chunks((size_t)ngroups * ns, 0), ent(ngroups, 0);
+    std::vector<int> most(ngroups, 0);
+    const unsigned nth = staged_threads(ngroups);
+    auto work = [&](unsigned t) {
+        StagedGroup o;
+        for (int G = (int)t; G < ngroups; G += (int)nth) {
+            staged_group(g, G, o);
+            for (int s = 0; s < ns; s++) chunks[(size_t)G * ns + s] = o.nchunk[s];
+            ent[G] = (int64_t)o.ent.size();
+            for (int c : o.cnt) most[G] = std::max(most[G], c);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; t++) pool.emplace_back(work, t);
+    for (auto &th : pool) th.join();
+    g.sg_chunk_base.assign((size_t)ngroups * ns + 1, 0);
+    for (size_t k = 0; k < chunks.size(); k++) g.sg_chunk_base[k + 1] = g.sg_chunk_base[k] + chunks[k];
+    g.sg_group_ent.assign((size_t)ngroups + 1, 0);
+    for (int G = 0; G < ngroups; G++) g.sg_group_ent[G + 1] = g.sg_group_ent[G] + ent[G];
+    return *std::max_element(most.begin(), most.end());
+}
+
+void geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage, std::vector<int64_t> &off,
+                            std::vector<uint32_t> &col, std::vector<uint32_t> &ang, std::vector<uint32_t> &meta) {
+    const int ns = g.nstrips;
+    const int64_t K0 = g.sg_chunk_base[(size_t)G0 * ns], K1 = g.sg_chunk_base[(size_t)G1 * ns];
+    const int64_t E0 = g.sg_group_ent[G0], E1 = g.sg_group_ent[G1];
+    stage.assign((size_t)(K1 - K0) * kSgChunk, -1);
+    off.assign((size_t)(K1 - K0) * kSgAngles, 0);
+    col.resize((size_t)(E1 - E0));
+    ang.resize((size_t)(E1 - E0));
+    meta.resize((size_t)(E1 - E0));
+    const unsigned nth = staged_threads(G1 - G0);
+    auto work = [&](unsigned t) {
+        StagedGroup o;
+        for (int G = G0 + (int)t; G < G1; G += (int)nth) {
+            staged_group(g, G, o);
+            const int64_t Kg = g.sg_chunk_base[(size_t)G * ns] - K0;
+            std::copy(o.stage.begin(), o.stage.end(), stage.begin() + Kg * kSgChunk);
+            int64_t e = g.sg_group_ent[G];
+            for (size_t k = 0; k < o.cnt.size(); k++) {
+                off[(size_t)Kg * kSgAngles + k] = e;
+                e += o.cnt[k];
+            }
+            // the strip of each chunk, for the device's check of d
+            std::vector<int> strip_of;
+            for (int s = 0; s < ns; s++) strip_of.insert(strip_of.end(), (size_t)o.nchunk[s], s);
+            size_t k = 0;
+            int64_t o0 = g.sg_group_ent[G] - E0;
+            for (size_t c = 0; c < strip_of.size(); c++)
+                for (int i = 0; i < kSgAngles; i++)
+                    for (int m = 0; m < o.cnt[c * kSgAngles + i]; m++, k++) {
+                        const int4 v = o.ent[k];
+                        col[o0 + k] = (uint32_t)v.z;
+                        ang[o0 + k] = (uint32_t)v.w;
+                        meta[o0 + k] = (uint32_t)v.x | ((uint32_t)v.y << 8) | ((uint32_t)strip_of[c] << 16);
+                    }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; t++) pool.emplace_back(work, t);
+    for (auto &th : pool) th.join();
+}
+
 // Stable counting sort of the contributing columns of angles [a0, a1) by
 // their first tap; order receives (a1-a0)*nvalid column indices.  Also
 // fills pixstart: for each q, the first entry whose first tap is >=

@@ -240,6 +240,31 @@ __global__ void build_pair_table(const uint32_t *__restrict__ order, const int64
     }
 }
 
+// Staged-strip table (design 4) for n entries: entry e (column col[e], angle
+// ang[e], meta[e] = slot | d << 8 | strip << 16) -> {slot | d << 8, w0, w1,
+// w2}; the device's first tap must give the host's d.  A padding entry
+// (column 0xffffffff) gets zero weights.
+template <int M>
+__global__ void build_staged_table(const uint32_t *__restrict__ col, const uint32_t *__restrict__ angi,
+                                   const uint32_t *__restrict__ meta, int64_t n, const int *__restrict__ coord,
+                                   const AngleConst *__restrict__ ang, double radius, int xcent,
+                                   uint4 *__restrict__ tab, int *__restrict__ err) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t m = meta[e];
+        const int d = (int)((m >> 8) & 31u), st = (int)(m >> 16);
+        Taps t = Taps{0, 0.0f, 0.0f, 0.0f};  // padding entry (column 0xffffffff): zero weights
+        if (col[e] != 0xffffffffu) {
+            const int j = (int)col[e], a = (int)angi[e];
+            t = tap_weights<M>(coord[2 * j], coord[2 * j + 1], ang[a], radius, xcent);
+            if (t.p0 - (st * kStripW - 2) != d) {
+                atomicOr(err, 16);
+                t.w0 = t.w1 = t.w2 = 0.0f;
+            }
+        }
+        tab[e] = make_uint4(m & 0xffffu, __float_as_uint(t.w0), __float_as_uint(t.w1), __float_as_uint(t.w2));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Forward projector
 // ---------------------------------------------------------------------------
@@ -863,6 +888,198 @@ __global__ __launch_bounds__(64 * kStripAngles) void forward_pair_kernel(
     forward_pair_body<RPL>(x, p, tab, base, off, partials, nstrips, nbins, nx, ny, na, nchunks, b);
 }
 
+// Forward projection, staged strips (design 4).  Workgroup b: angle group
+// G = b % ngroups (kSgAngles waves, one angle each), then strip s, then row
+// chunk.  The group's stage list of (G, s) is walked chunk by chunk: the
+// kSgChunk columns x 64*RPL rows of the next chunk are loaded (each wave
+// one column in kSgAngles, one coalesced 64*RPL*4-byte segment) while the
+// current chunk is read from LDS by every wave's entries, then stored to
+// the other LDS buffer; one barrier per chunk (a buffer a wave writes in
+// iteration k was last read before the barrier that ended iteration k - 1).
+// Each entry adds its column's rows, read from LDS, to the strip's pixel
+// registers as in design 2 (batches of kStripBatch entries, every offset
+// present in the batch accumulated with the others' weights zeroed); the
+// entries of a (chunk, angle) are ordered by offset, and a chunk's columns
+// lie along one band of the central angle's rays, so a batch mostly holds
+// one offset.  Table entries are wave-uniform (scalar loads).
+constexpr int kSgThreads = 64 * kSgAngles;
+
+// acc[D..D+2] += w * v for every entry of a batch, for the one offset D whose
+// bit is set in present (the batch's real entries all start at D; entries
+// staged past the range end have zero weights, so none needs a mask).
+// Sequential bit tests, as strip_add_present: an if/else on D would let the
+// compiler merge the leaves' stores into one store through a selected
+// pointer, which puts acc in scratch memory.
+template <int LO, int HI, int RPL>
+__device__ __forceinline__ void strip_add_single(float (&acc)[kStripW + 4][RPL], const unsigned present,
+                                                 const float (&w)[kStripBatch][3],
+                                                 const float (&v)[kStripBatch][RPL]) {
+    if constexpr (HI - LO == 1) {
+#pragma unroll
+        for (int k = 0; k < kStripBatch; k++) strip_add<LO, RPL>(acc, w[k][0], w[k][1], w[k][2], v[k]);
+    } else {
+        constexpr int MID = (LO + HI) / 2;
+        constexpr unsigned ML = ((1u << MID) - 1u) & ~((1u << LO) - 1u);
+        constexpr unsigned MH = ((1u << HI) - 1u) & ~((1u << MID) - 1u);
+        if (present & ML) strip_add_single<LO, MID, RPL>(acc, present, w, v);
+        if (present & MH) strip_add_single<MID, HI, RPL>(acc, present, w, v);
+    }
+}
+
+// The walk over chunks [K0, K1) of one (group, strip) for the wave's rows;
+// FULL: all 64*RPL rows lie inside the volume.  Per chunk, staged into LDS
+// one chunk ahead: the kSgChunk columns' rows (an empty slot, stage -1,
+// stages column 0: no entry reads it) and each wave's entries of the chunk
+// (at most kSgEnt, whole batches of one offset each; lane l fetches entry
+// l).  With whole chunks at 4 rows per lane a column goes from global
+// memory straight to its LDS slot (one global_load_lds_dwordx4 per column:
+// 64 lanes x 16 bytes), else through registers.  A batch reads its four
+// entries from LDS (the same address in every lane), takes its offset from
+// the first, and adds each entry's column rows with the entry's weights.
+template <int RPL, bool FULL>
+__device__ __forceinline__ void staged_walk(float (*s_col)[kSgChunk][64 * RPL], uint4 (*s_ent)[kSgAngles][kSgEnt],
+                                            const float *__restrict__ x, const uint4 *__restrict__ tab,
+                                            const int *__restrict__ stage, const int64_t *__restrict__ off,
+                                            const int64_t K0, const int64_t K1, const int wave, const int lane,
+                                            const int ny, const int y0, float (&acc)[kStripW + 4][RPL]) {
+    constexpr int kPer = kSgChunk / kSgAngles;  // columns each wave stages per chunk
+    static_assert(kPer * kSgAngles == kSgChunk, "chunk staging pattern");
+    static_assert(kSgEnt <= 64 && kSgEnt % kStripBatch == 0, "one entry per lane, whole batches");
+    constexpr bool kDma = FULL && RPL == 4;
+    float stg[kDma ? 1 : kPer][RPL];
+    uint4 ent;      // lane l: entry l of the wave's range in the chunk being staged
+    int nent_next;  // its entry count (a multiple of kStripBatch)
+    auto load_stage = [&](const int64_t K, const int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < kPer; c++) {
+            const int j = max(0, stage[K * kSgChunk + wave + c * kSgAngles]);
+            const float *__restrict__ xj = x + (size_t)j * ny + y0;
+            if constexpr (kDma) {
+                __builtin_amdgcn_global_load_lds(xj, &s_col[buf][wave + c * kSgAngles][0], 16, 0, 0);
+            } else if constexpr (FULL && RPL == 2) {
+                const float2 q = *reinterpret_cast<const float2 *>(xj);
+                stg[c][0] = q.x;
+                stg[c][RPL > 1 ? 1 : 0] = q.y;
+            } else if constexpr (FULL) {
+                stg[c][0] = xj[0];
+            } else {
+#pragma unroll
+                for (int r = 0; r < RPL; r++) stg[c][r] = y0 + r < ny ? xj[r] : 0.0f;
+            }
+        }
+        // this wave's entry range of chunk K: lanes 0 and 1 fetch its bounds
+        const int64_t bnd = off[K * kSgAngles + wave + (lane & 1)];
+        const int64_t e0 = (int64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)bnd) |
+                           ((int64_t)__builtin_amdgcn_readfirstlane((int)(bnd >> 32)) << 32);
+        const int64_t e1l = __shfl(bnd, 1, 64);
+        nent_next = __builtin_amdgcn_readfirstlane((int)(e1l - e0));
+        ent = lane < nent_next ? tab[e0 + lane] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    auto store_stage = [&](const int buf) __attribute__((always_inline)) {
+        if constexpr (!kDma) {
+#pragma unroll
+            for (int c = 0; c < kPer; c++) {
+                float *d = &s_col[buf][wave + c * kSgAngles][lane * RPL];
+                if constexpr (RPL == 4) {
+                    *reinterpret_cast<float4 *>(d) = make_float4(stg[c][0], stg[c][RPL > 1 ? 1 : 0],
+                                                                 stg[c][RPL > 2 ? 2 : 0], stg[c][RPL > 3 ? 3 : 0]);
+                } else if constexpr (RPL == 2) {
+                    *reinterpret_cast<float2 *>(d) = make_float2(stg[c][0], stg[c][RPL > 1 ? 1 : 0]);
+                } else {
+                    d[0] = stg[c][0];
+                }
+            }
+        }
+        if (lane < kSgEnt) s_ent[buf][wave][lane] = ent;
+        // the columns' direct loads (and the entry fetch) complete before the barrier
+        if constexpr (kDma) __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+    };
+
+    int nent = 0;
+    if (K0 < K1) {
+        load_stage(K0, 0);
+        store_stage(0);
+        nent = nent_next;
+    }
+    __syncthreads();
+    for (int64_t K = K0; K < K1; K++) {
+        const int buf = (int)((K - K0) & 1);
+        const bool more = K + 1 < K1;
+        if (more) load_stage(K + 1, buf ^ 1);
+        const uint4 *E = s_ent[buf][wave];
+        for (int i = 0; i < nent; i += kStripBatch) {
+            uint4 ec[kStripBatch];
+#pragma unroll
+            for (int k = 0; k < kStripBatch; k++) ec[k] = E[i + k];
+            float v[kStripBatch][RPL];
+#pragma unroll
+            for (int k = 0; k < kStripBatch; k++) {
+                const float *src = &s_col[buf][ec[k].x & (kSgChunk - 1)][lane * RPL];
+                if constexpr (RPL == 4) {
+                    const float4 q = *reinterpret_cast<const float4 *>(src);
+                    v[k][0] = q.x;
+                    v[k][RPL > 1 ? 1 : 0] = q.y;
+                    v[k][RPL > 2 ? 2 : 0] = q.z;
+                    v[k][RPL > 3 ? 3 : 0] = q.w;
+                } else if constexpr (RPL == 2) {
+                    const float2 q = *reinterpret_cast<const float2 *>(src);
+                    v[k][0] = q.x;
+                    v[k][RPL > 1 ? 1 : 0] = q.y;
+                } else {
+                    v[k][0] = src[0];
+                }
+            }
+            float w[kStripBatch][3];
+#pragma unroll
+            for (int k = 0; k < kStripBatch; k++) {
+                w[k][0] = __uint_as_float(ec[k].y);
+                w[k][1] = __uint_as_float(ec[k].z);
+                w[k][2] = __uint_as_float(ec[k].w);
+            }
+            const unsigned d = ((unsigned)__builtin_amdgcn_readfirstlane(ec[0].x) >> 8) & 31u;
+            strip_add_single<0, kStripGroups, RPL>(acc, 1u << d, w, v);
+        }
+        if (more) {
+            store_stage(buf ^ 1);
+            nent = nent_next;
+        }
+        __syncthreads();
+    }
+}
+
+// At least 4 waves per SIMD, so that two workgroups share a CU.
+template <int RPL>
+__global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void forward_staged_kernel(
+    const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ tab, const int *__restrict__ stage,
+    const int64_t *__restrict__ chunk_base, const int64_t *__restrict__ off, double *__restrict__ partials,
+    int nstrips, int nx, int ny, int na, int nchunks, const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    constexpr int ROWS = 64 * RPL;
+    __shared__ __align__(16) float s_col[2][kSgChunk][ROWS];
+    __shared__ uint4 s_ent[2][kSgAngles][kSgEnt];
+    const int nblk = gridDim.x;
+    int b = blockIdx.x;
+    if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+    const int ngroups = (na + kSgAngles - 1) / kSgAngles;
+    const int G = b % ngroups, rest = b / ngroups, st = rest % nstrips, chunk = rest / nstrips;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int a = G * kSgAngles + wave;
+    const bool active = a < na;
+    const int ybase = chunk * ROWS, y0 = ybase + lane * RPL;
+    const int64_t K0 = chunk_base[(int64_t)G * nstrips + st], K1 = chunk_base[(int64_t)G * nstrips + st + 1];
+    float acc[kStripW + 4][RPL];
+#pragma unroll
+    for (int i = 0; i < kStripW + 4; i++)
+#pragma unroll
+        for (int r = 0; r < RPL; r++) acc[i][r] = 0.0f;
+    if (ybase + ROWS <= ny)
+        staged_walk<RPL, true>(s_col, s_ent, x, tab, stage, off, K0, K1, wave, lane, ny, y0, acc);
+    else
+        staged_walk<RPL, false>(s_col, s_ent, x, tab, stage, off, K0, K1, wave, lane, ny, y0, acc);
+    if (!active) return;
+    strip_store<RPL>(acc, p, partials, a, st, chunk, nchunks, nstrips, nx, ny, y0, lane);
+}
+
 // Index of the problem that owns work item `it`: the last k with
 // item0(k) <= it (item0 ascending).  Uniform across the workgroup.  The
 // item0 values of the first kBatchLdsProbs problems are staged in LDS once
@@ -1200,6 +1417,8 @@ static int upload(T **dst, const std::vector<T> &src, hipStream_t st) {
     return 0;
 }
 
+static void release_staged_tables(Geometry &g);
+
 void geometry_free_device(Geometry &g) {
     hipFree(g.d_ang);
     hipFree(g.d_coord);
@@ -1214,6 +1433,7 @@ void geometry_free_device(Geometry &g) {
     g.d_strip = nullptr;
     g.d_strip_base = nullptr;
     g.d_strip_off = nullptr;
+    release_staged_tables(g);
     hipFree(g.d_err);
     hipFree(g.d_seg);
     hipFree(g.d_pixstart);
@@ -1418,6 +1638,93 @@ static int build_pair_tables(Geometry &g, hipStream_t st) {
     return 0;
 }
 
+// Design 4's tables: the staged ordering on the host, in batches of angle
+// groups of at most 32 M entries, and the weights on the device.
+static int build_staged_tables(Geometry &g, hipStream_t st) {
+    if (g.na > 65535 || (g.nx + kStripW - 1) / kStripW > 65535) {
+        set_error("staged-strip forward table: at most 65535 angles and strips");
+        return AREC_ESIZE;
+    }
+    if (geometry_staged_count(g) > kSgEnt) {
+        set_error("staged-strip forward table: a chunk holds more entries for one angle than the kernel stages");
+        return AREC_ESIZE;
+    }
+    const int ns = g.nstrips, ngroups = (g.na + kSgAngles - 1) / kSgAngles;
+    const int64_t nchunk = g.sg_chunk_base.back(), total = g.sg_group_ent.back();
+    if ((total + kStripBatch) * (int64_t)sizeof(uint4) + nchunk * (int64_t)(kSgChunk * 4 + kSgAngles * 8) >
+        kMaxStripTableBytes) {
+        set_error("staged-strip forward table too large for this geometry");
+        return AREC_ESIZE;
+    }
+    int rc;
+    AREC_HIP(hipMalloc((void **)&g.d_sg_tab, sizeof(uint4) * (size_t)(total + kStripBatch)));
+    AREC_HIP(hipMemsetAsync(g.d_sg_tab + total, 0, sizeof(uint4) * kStripBatch, st));
+    AREC_HIP(hipMalloc((void **)&g.d_sg_stage, sizeof(int) * (size_t)std::max<int64_t>(1, nchunk * kSgChunk)));
+    AREC_HIP(hipMalloc((void **)&g.d_sg_off, sizeof(int64_t) * (size_t)(nchunk * kSgAngles + 1)));
+    if ((rc = upload(&g.d_sg_chunk_base, g.sg_chunk_base, st))) return rc;
+    AREC_HIP(hipMemsetAsync(g.d_err, 0, sizeof(int), st));
+    std::vector<int> stage;
+    std::vector<int64_t> off;
+    std::vector<uint32_t> col, angi, meta;
+    uint32_t *d_tmp = nullptr;
+    int64_t cap = 0;
+    for (int G0 = 0; G0 < ngroups;) {
+        int G1 = G0 + 1;
+        while (G1 < ngroups && g.sg_group_ent[G1 + 1] - g.sg_group_ent[G0] <= (int64_t)(32 << 20)) G1++;
+        geometry_staged_groups(g, G0, G1, stage, off, col, angi, meta);
+        AREC_HIP(hipStreamSynchronize(st));  // the previous batch is done with d_tmp
+        const int64_t n = (int64_t)col.size();
+        if (n > cap) {
+            hipFree(d_tmp);
+            cap = n;
+            AREC_HIP(hipMalloc((void **)&d_tmp, 3 * sizeof(uint32_t) * (size_t)std::max<int64_t>(1, cap)));
+        }
+        const int64_t K0 = g.sg_chunk_base[(size_t)G0 * ns], E0 = g.sg_group_ent[G0];
+        AREC_HIP(hipMemcpyAsync(g.d_sg_stage + K0 * kSgChunk, stage.data(), sizeof(int) * stage.size(),
+                                hipMemcpyHostToDevice, st));
+        AREC_HIP(hipMemcpyAsync(g.d_sg_off + K0 * kSgAngles, off.data(), sizeof(int64_t) * off.size(),
+                                hipMemcpyHostToDevice, st));
+        if (n > 0) {
+            AREC_HIP(hipMemcpyAsync(d_tmp, col.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+            AREC_HIP(hipMemcpyAsync(d_tmp + cap, angi.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+            AREC_HIP(hipMemcpyAsync(d_tmp + 2 * cap, meta.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+            dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 65536));
+            if (g.method == kSQ)
+                build_staged_table<kSQ><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_sg_tab + E0, g.d_err);
+            else if (g.method == kKB)
+                build_staged_table<kKB><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_sg_tab + E0, g.d_err);
+            else
+                build_staged_table<kLinear><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_sg_tab + E0, g.d_err);
+            AREC_HIP(hipGetLastError());
+        }
+        G0 = G1;
+    }
+    AREC_HIP(hipMemcpyAsync(g.d_sg_off + nchunk * kSgAngles, &g.sg_group_ent.back(), sizeof(int64_t),
+                            hipMemcpyHostToDevice, st));
+    AREC_HIP(hipStreamSynchronize(st));
+    hipFree(d_tmp);
+    int herr = 0;
+    AREC_HIP(hipMemcpy(&herr, g.d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) {
+        set_error("staged-strip table: device taps disagree with the host ordering");
+        return AREC_EGEOM;
+    }
+    return 0;
+}
+
+static void release_staged_tables(Geometry &g) {
+    hipFree(g.d_sg_tab);
+    hipFree(g.d_sg_stage);
+    hipFree(g.d_sg_off);
+    hipFree(g.d_sg_chunk_base);
+    g.d_sg_tab = nullptr;
+    g.d_sg_stage = nullptr;
+    g.d_sg_off = nullptr;
+    g.d_sg_chunk_base = nullptr;
+    std::vector<int64_t>().swap(g.sg_chunk_base);
+    std::vector<int64_t>().swap(g.sg_group_ent);
+}
+
 // Frees the depth-strip tables (device and host).
 static void release_strip_tables(Geometry &g) {
     hipFree(g.d_strip);
@@ -1432,7 +1739,40 @@ static void release_strip_tables(Geometry &g) {
 
 // Selecting design 1 releases design 2's or 3's table (rebuilt when that
 // design is selected again); switching between designs 2 and 3 rebuilds it.
+// Design 4's tables exist only while it is selected.
 int geometry_set_forward(Geometry &g, int design, int rpl, int min_groups, hipStream_t st) {
+    if (design == 4) {
+        if (!g.device_ready) {
+            set_error("forward design 4 needs an uploaded geometry");
+            return AREC_EINVAL;
+        }
+        if (rpl == 0) rpl = (g.ny >= 256 && g.ny % 4 == 0) ? 4 : (g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1;
+        if (!fwd_rpl_allowed(g.ny, rpl)) {
+            set_error("forward plan: rows per lane must be 1, 2 or 4 and divide the row count");
+            return AREC_EINVAL;
+        }
+        if (g.d_strip) {
+            AREC_HIP(hipStreamSynchronize(st));
+            release_strip_tables(g);
+        }
+        if (!g.d_sg_tab) {
+            int rc = build_staged_tables(g, st);
+            if (rc) {
+                hipStreamSynchronize(st);
+                release_staged_tables(g);
+                g.fwd_design = 1;
+                return rc;
+            }
+        }
+        g.strip_rpl = rpl;
+        g.fwd_design = 4;
+        return 0;
+    }
+    if (g.d_sg_tab) {
+        AREC_HIP(hipStreamSynchronize(st));
+        release_staged_tables(g);
+        if (g.fwd_design == 4) g.fwd_design = 1;
+    }
     if (design == 1) {
         int rc = geometry_replan_forward(g, rpl, min_groups, st);
         if (!rc) {
@@ -1480,6 +1820,24 @@ int geometry_set_forward(Geometry &g, int design, int rpl, int min_groups, hipSt
 int launch_forward(const Geometry &g, const float *x, float *p, double *partials, hipStream_t st,
                    const int *skip) {
     if (g.nvalid == 0) return fill(p, 0.0f, g.nproj, st, skip);
+    if (g.fwd_design == 4) {
+        const int rpl = g.strip_rpl, nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
+        const int ngroups = (g.na + kSgAngles - 1) / kSgAngles;
+        dim3 grid((unsigned)((int64_t)ngroups * g.nstrips * nchunks));
+#define AREC_FWDG(R)                                                                                          \
+    forward_staged_kernel<R><<<grid, kSgThreads, 0, st>>>(x, p, g.d_sg_tab, g.d_sg_stage, g.d_sg_chunk_base,   \
+                                                          g.d_sg_off, partials, g.nstrips, g.nx, g.ny, g.na, \
+                                                          nchunks, skip)
+        if (rpl == 4)
+            AREC_FWDG(4);
+        else if (rpl == 2)
+            AREC_FWDG(2);
+        else
+            AREC_FWDG(1);
+#undef AREC_FWDG
+        AREC_HIP(hipGetLastError());
+        return 0;
+    }
     if (g.fwd_design == 3) {
         const int rpl = g.strip_rpl, nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
         const int ngroups = ((g.na + 1) / 2 + kStripAngles - 1) / kStripAngles;

@@ -2,9 +2,9 @@
 
 Every forward design has a branch for a wave whose 64 x rows-per-lane rows
 all lie inside the volume (vector loads of 2 or 4 rows per lane): design 1's
-RPL == 4 / RPL == 2 loads, design 2's strip_walk<RPL, true> and design 3's
-pair_walk<RPL, true>.  They run wherever the row count is a multiple of the
-chunk height -- the 512^3/720 solve runs design 2 at 4 rows per lane, the
+RPL == 4 / RPL == 2 loads, design 2's strip_walk<RPL, true>, design 3's
+pair_walk<RPL, true> and design 4's staged_walk<RPL, true>.  They run
+wherever the row count is a multiple of the chunk height -- the 512^3/720 solve runs design 2 at 4 rows per lane, the
 2048^2 x 1024/1800 slab design 1 at 4 -- so these tests use 256 rows.
 
 The oracle is the reference's loop order in C (angle -> column -> row,
@@ -25,7 +25,7 @@ import pyoracle as O
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
 ROWS = [0, 1, 63, 64, 130, 191, 254, 255]  # lane and chunk boundaries of every rows-per-lane plan
-PLANS = [(2, 4), (2, 2), (2, 1), (1, 4), (1, 2), (1, 1), (3, 2), (3, 1)]
+PLANS = [(2, 4), (2, 2), (2, 1), (1, 4), (1, 2), (1, 1), (3, 2), (3, 1), (4, 4), (4, 2), (4, 1)]
 
 
 def workers(n):
@@ -151,7 +151,7 @@ def sirt_256(ctx):
     return g, b, lam, rows, xo
 
 
-@pytest.mark.parametrize("design,rpl", [(2, 4), (1, 4), (3, 2)])
+@pytest.mark.parametrize("design,rpl", [(2, 4), (1, 4), (3, 2), (4, 4)])
 def test_sirt_full_row_chunks_rows_vs_oracle(ctx, sirt_256, design, rpl):
     """30 SIRT iterations with the forward design's whole-chunk branch: with
     a fixed lam, no restart and no tolerance stop every row is its own

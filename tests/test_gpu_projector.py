@@ -178,14 +178,16 @@ def test_back_tap_table_and_computed_taps_agree(ctx, method, monkeypatch):
     assert np.array_equal(x1.download(), x2.download())
 
 
-@pytest.mark.parametrize("design", [2, 3])
+@pytest.mark.parametrize("design", [2, 3, 4])
 @pytest.mark.parametrize("method", ["sq", "kb", "linear"])
 @pytest.mark.parametrize("nx,ny,na,rpl", [(33, 5, 17, 1), (64, 70, 45, 2), (129, 132, 90, 4), (256, 3, 7, 1)])
 def test_forward_depth_strips_vs_oracle(ctx, design, method, nx, ny, na, rpl):
-    """Forward designs 2 (depth strips) and 3 (depth strips walked by angle
-    pairs; odd angle counts leave the last pair with one angle): the same
-    parity bar as the sweep against the oracle, and the sweep's projections
-    to FP32 summation order."""
+    """Forward designs 2 (depth strips), 3 (depth strips walked by angle
+    pairs; odd angle counts leave the last pair with one angle) and 4
+    (staged strips: angle groups of 8 through LDS-staged column chunks;
+    angle counts that leave a partial last group): the same parity bar as
+    the sweep against the oracle, and the sweep's projections to FP32
+    summation order."""
     import arec3d_amd as A
 
     if design == 3:
@@ -207,7 +209,7 @@ def test_forward_depth_strips_vs_oracle(ctx, design, method, nx, ny, na, rpl):
     op = O.project(method, x, ang, nx, ny)
     assert rel_views(strips, op).max() <= TOL
     assert rel_views(strips, sweep).max() <= TOL
-    other = 5 - design  # switching between designs 2 and 3 rebuilds the table
+    other = {2: 3, 3: 2, 4: 2}[design]  # switching designs rebuilds or releases the tables
     g.set_forward(other, min(rpl, 2))
     assert g.forward_design == other
     g.project(dx, dp)

@@ -217,12 +217,12 @@ def test_sirt_config1_shape_rows_vs_oracle(ctx):
     assert relnorm(x.download()[:, r0:r0 + 4], xo) <= 1e-4
 
 
-@pytest.mark.parametrize("design", [2, 3])
+@pytest.mark.parametrize("design", [2, 3, 4])
 @pytest.mark.parametrize("method", ["sq", "kb", "linear"])
 def test_solvers_with_depth_strip_forward_vs_oracle(ctx, method, design):
-    """SIRT and CGLS with forward designs 2 and 3 (depth strips, angle
-    pairs; their kernels also deliver the residual-norm partials): the same
-    bar against the oracle as with the sweep."""
+    """SIRT and CGLS with forward designs 2, 3 and 4 (depth strips, angle
+    pairs, staged strips; their kernels also deliver the residual-norm
+    partials): the same bar against the oracle as with the sweep."""
     import arec3d_amd as A
 
     nx, ny, na = 65, 40, 61 if design == 3 else 60
@@ -260,7 +260,7 @@ def test_autotune_designs(ctx, tmp_path):
     g = A.Geometry(ctx, "sq", nx, ny, ang)
     g.set_forward(1, 4, 0)
     base = g.device_bytes()
-    assert {c[0] for c in candidates(g)} == {1, 2, 3}
+    assert {c[0] for c in candidates(g)} == {1, 2, 3, 4}
     cache = str(tmp_path / "plans.json")
     d, r, m, ms = autotune(ctx, g, cache=cache, reps=1)
     assert g.forward_design == d and g.get_plan()[0] == r and ms > 0

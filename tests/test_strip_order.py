@@ -1,6 +1,8 @@
-"""Host-only check of the depth-strip (design 2) and angle-pair (design 3)
-table orderings, compiled from the library's geometry.cpp and run on the
-CPU for the three projector methods, odd and even widths and angle counts."""
+"""Host-only check of the depth-strip (design 2), angle-pair (design 3) and
+staged-strip (design 4) table orderings, compiled from the library's
+geometry.cpp and run on the CPU for the three projector methods, odd and
+even widths and angle counts (one and several angle groups, a partial
+last group)."""
 import os
 import subprocess
 
@@ -24,7 +26,7 @@ def checker(tmp_path_factory):
 
 
 @pytest.mark.parametrize("method", [1, 2, 3])
-@pytest.mark.parametrize("nx,ny,na", [(48, 4, 13), (33, 3, 20), (65, 2, 7)])
+@pytest.mark.parametrize("nx,ny,na", [(48, 4, 13), (33, 3, 20), (65, 2, 7), (128, 2, 45)])
 def test_strip_and_pair_orderings(checker, method, nx, ny, na):
     r = subprocess.run([checker, str(method), str(nx), str(ny), str(na)], capture_output=True, text=True,
                        timeout=120)
