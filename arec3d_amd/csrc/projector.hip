@@ -931,9 +931,11 @@ __device__ __forceinline__ void strip_add_single(float (&acc)[kStripW + 4][RPL],
 // one chunk ahead: the kSgChunk columns' rows (an empty slot, stage -1,
 // stages column 0: no entry reads it) and each wave's entries of the chunk
 // (at most kSgEnt, whole batches of one offset each; lane l fetches entry
-// l).  With whole chunks at 4 rows per lane a column goes from global
-// memory straight to its LDS slot (one global_load_lds_dwordx4 per column:
-// 64 lanes x 16 bytes), else through registers.  A batch reads its four
+// l straight into LDS).  With whole chunks at 4 rows per lane a column goes
+// from global memory straight to its LDS slot too (one
+// global_load_lds_dwordx4 per column: 64 lanes x 16 bytes), else through
+// registers (at 2 rows per lane one 8-byte load per lane measured faster
+// than two direct 4-byte loads: 15.9 vs 17.0 ms at 512^3/720).  A batch reads its four
 // entries from LDS (the same address in every lane), takes its offset from
 // the first, and adds each entry's column rows with the entry's weights.
 template <int RPL, bool FULL>
@@ -947,7 +949,6 @@ __device__ __forceinline__ void staged_walk(float (*s_col)[kSgChunk][64 * RPL], 
     static_assert(kSgEnt <= 64 && kSgEnt % kStripBatch == 0, "one entry per lane, whole batches");
     constexpr bool kDma = FULL && RPL == 4;
     float stg[kDma ? 1 : kPer][RPL];
-    uint4 ent;      // lane l: entry l of the wave's range in the chunk being staged
     int nent_next;  // its entry count (a multiple of kStripBatch)
     auto load_stage = [&](const int64_t K, const int buf) __attribute__((always_inline)) {
 #pragma unroll
@@ -973,7 +974,7 @@ __device__ __forceinline__ void staged_walk(float (*s_col)[kSgChunk][64 * RPL], 
                            ((int64_t)__builtin_amdgcn_readfirstlane((int)(bnd >> 32)) << 32);
         const int64_t e1l = __shfl(bnd, 1, 64);
         nent_next = __builtin_amdgcn_readfirstlane((int)(e1l - e0));
-        ent = lane < nent_next ? tab[e0 + lane] : make_uint4(0u, 0u, 0u, 0u);
+        if (lane < nent_next) __builtin_amdgcn_global_load_lds((void *)(tab + e0 + lane), &s_ent[buf][wave][0], 16, 0, 0);
     };
     auto store_stage = [&](const int buf) __attribute__((always_inline)) {
         if constexpr (!kDma) {
@@ -990,9 +991,7 @@ __device__ __forceinline__ void staged_walk(float (*s_col)[kSgChunk][64 * RPL], 
                 }
             }
         }
-        if (lane < kSgEnt) s_ent[buf][wave][lane] = ent;
-        // the columns' direct loads (and the entry fetch) complete before the barrier
-        if constexpr (kDma) __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+        __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the direct loads land before the barrier
     };
 
     int nent = 0;
