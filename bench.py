@@ -36,6 +36,10 @@ VALU_PEAK_FLOPS = 157.3e12  # FP32 vector (MI355X published; half of it without 
 # Phantom and noise seed of the headline problem: disjoint from the seeds the
 # warm start's input iteration count was chosen on (49000-49002).
 HEADLINE_SEED = 49010
+FORWARD_KERNELS = {1: "forward_kernel", 2: "forward_strip_kernel", 3: "forward_pair_kernel",
+                   4: "forward_staged_kernel"}
+FORWARD_DESIGNS = {1: "detector sweep", 2: "depth strips", 3: "depth strips, angle pairs",
+                   4: "staged strips (8 angles per workgroup, 32-column LDS chunks)"}
 
 
 def roofline(kernel, ms, vu, g, plan):
@@ -529,7 +533,7 @@ def cgls_workload(args):
         "data": "synthetic (seeded ellipsoid phantom, Beer-Lambert counts at 1e4 photons)",
         "config": {
             "workload": f"CGLS (SQ) {iters} iterations, {nx}x{nx}x{ny} volume, {na} angles",
-            "forward_plan": {"design": {1: "detector sweep", 2: "depth strips", 3: "depth strips, angle pairs"}[g.forward_design],
+            "forward_plan": {"design": FORWARD_DESIGNS[g.forward_design],
                              "rows_per_lane": g.get_plan()[0]},
             "voxel_updates_per_s": round(vu / dt, 1),
             "ms_per_iteration": round(dt * 1e3 / iters, 3),
@@ -560,7 +564,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--maxit", type=int, default=3000)
     ap.add_argument("--plan", type=int, nargs=3, metavar=("DESIGN", "ROWS_PER_LANE", "MIN_GROUPS"),
-                    help="forward projector design (1 sweep, 2 depth strips, 3 depth strips by angle pairs) "
+                    help="forward projector design (1 sweep, 2 depth strips, 3 depth strips by angle pairs, 4 staged strips) "
                          "and plan (default: autotuned)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=None, help="plain SIRT solves (default: --steps)")
@@ -696,7 +700,7 @@ def main():
     p_tmp = ctx.empty(g.proj_shape)
     ms_f = g.time_operator(0, x, p_tmp, 5)
     ms_b = g.time_operator(1, p_tmp, x, 5)
-    fwd_name = {1: "forward_kernel", 2: "forward_strip_kernel", 3: "forward_pair_kernel"}[plan[0]]
+    fwd_name = FORWARD_KERNELS[plan[0]]
     dom, ms = (fwd_name, ms_f) if ms_f >= ms_b else ("backward_kernel", ms_b)
     log(f"forward {ms_f:.2f} ms ({vu / ms_f / 1e6:.1f} GVU/s), back {ms_b:.2f} ms ({vu / ms_b / 1e6:.1f} GVU/s)")
     if args.plain_only:
@@ -712,7 +716,7 @@ def main():
         "plain_final_relnrm": final[2],
         "ms_per_iteration": round(t_plain * 1e3 / max(1, n_it), 3),
         "lam": lam,
-        "forward_plan": {"design": {1: "detector sweep", 2: "depth strips", 3: "depth strips, angle pairs"}[plan[0]], "rows_per_lane": plan[1],
+        "forward_plan": {"design": FORWARD_DESIGNS[plan[0]], "rows_per_lane": plan[1],
                          "min_groups": plan[2]},
         "geometry_setup_s": round(t_geom, 3),
         "forward_ms": round(ms_f, 3),

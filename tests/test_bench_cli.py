@@ -20,3 +20,14 @@ def test_bench_help_lists_workloads():
     assert r.returncode == 0
     for w in ("warm", "align", "batch", "cgls", "rotate"):
         assert w in r.stdout
+
+
+def test_bench_names_every_forward_design():
+    """Every design the autotuner may pick has a kernel name (for the
+    roofline's per-kernel timing) and a description in the bench line."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from arec3d_amd.autotune import DESIGNS
+
+    for d in DESIGNS:
+        assert d in bench.FORWARD_KERNELS and d in bench.FORWARD_DESIGNS
