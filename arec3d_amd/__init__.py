@@ -88,6 +88,8 @@ def _load():
         "arec_geom_plan": ([vp, vp, i32, i32], C.c_int),
         "arec_geom_set_forward": ([vp, vp, i32, i32, i32], C.c_int),
         "arec_geom_forward_design": ([vp], i32),
+        "arec_geom_set_back": ([vp, vp, i32, i32], C.c_int),
+        "arec_geom_get_back": ([vp, P(i32), P(i32)], C.c_int),
         "arec_match": ([vp, vp, vp, vp, flt, i32, i32, flt, i32, i32, vp, vp, P(Hist)], C.c_int),
         "arec_match_ex": ([vp, vp, vp, vp, flt, i32, i32, flt, i32, i32, i32, vp, vp, vp, P(Hist)], C.c_int),
         "arec_rot_cc": ([vp, vp, vp, i32, i32, i32, i32, vp], C.c_int),
@@ -145,7 +147,7 @@ EXPORTED = [
     "arec_conv3_check", "arec_conv3_wgrad_check", "arec_cyl_to_cube", "arec_cube_to_cyl", "arec_h_project", "arec_h_backproject",
     "arec_h_solve", "arec_sirt_batch", "arec_geom_plan", "arec_geom_get_plan", "arec_match", "arec_match_ex", "arec_align_centroid", "arec_take_log", "arec_periodic_decomposition",
     "arec_h_take_log", "arec_h_periodic_decomposition", "arec_scale", "arec_geom_set_forward",
-    "arec_geom_forward_design", "arec_mrc_read_header", "arec_mrc_read", "arec_mrc_write",
+    "arec_geom_forward_design", "arec_geom_set_back", "arec_geom_get_back", "arec_mrc_read_header", "arec_mrc_read", "arec_mrc_write",
     "arec_rot_cc", "arec_rotate_views", "arec_rotate_views_safe", "arec_h_rot_cc_images", "arec_h_rotate_images",
     "arec_h_rotate_images_safe",
 ]
@@ -321,13 +323,24 @@ class Geometry:
         _check(lib.arec_geom_plan(self.ctx.h, self.h, int(rows_per_lane), int(min_groups)))
 
     def set_forward(self, design, rows_per_lane=0, min_groups=0):
-        """Forward design 1 (detector sweep), 2 (depth strips) or 3 (depth
-        strips walked by angle pairs) and its plan."""
+        """Forward design 1 (detector sweep), 2 (depth strips), 3 (depth
+        strips walked by angle pairs) or 4 (staged strips) and its plan."""
         _check(lib.arec_geom_set_forward(self.ctx.h, self.h, int(design), int(rows_per_lane), int(min_groups)))
 
     @property
     def forward_design(self):
         return int(lib.arec_geom_forward_design(self.h))
+
+    def set_back(self, rows, angles):
+        """Back projector tile shape: rows per workgroup x angles per LDS
+        pass, one of (32, 2) (the default), (16, 4), (64, 2)."""
+        _check(lib.arec_geom_set_back(self.ctx.h, self.h, int(rows), int(angles)))
+
+    @property
+    def back_tile(self):
+        r, a = C.c_int32(), C.c_int32()
+        _check(lib.arec_geom_get_back(self.h, C.byref(r), C.byref(a)))
+        return r.value, a.value
 
     def get_plan(self):
         r, m, s = C.c_int32(), C.c_int32(), C.c_int32()

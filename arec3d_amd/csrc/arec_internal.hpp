@@ -59,6 +59,12 @@ constexpr int kSgAngles = 8;
 constexpr int kSgChunk = 32;
 constexpr int kSgEnt = 48;  // most entries of one (chunk, angle), padding included
 // The back projector's tap table is built when it takes at most this much.
+// Back-projector tile shapes (rows per workgroup x angles staged per LDS
+// pass) a geometry may select; shape 0 is the default (12.5 ms at 512^3/720
+// against 14.3 ms for the other two, which suit other row counts).
+constexpr int kBpShapes = 3;
+constexpr int kBpShapeRows[kBpShapes] = {32, 16, 64};
+constexpr int kBpShapeAngles[kBpShapes] = {2, 4, 2};
 constexpr int64_t kMaxBackTableBytes = 16ll << 30;
 // The depth-strip forward table is built (on request) up to this size.
 constexpr int64_t kMaxStripTableBytes = 32ll << 30;
@@ -132,6 +138,7 @@ struct Geometry {
     int64_t *d_sg_chunk_base = nullptr;
     int64_t *d_sg_off = nullptr;
     uint4 *d_sg_tab = nullptr;
+    int bp_shape = 0;  // back-projector tile shape (index into kBpShapeRows)
     bool device_ready = false;
 };
 
@@ -200,6 +207,21 @@ struct BatchBp {
     int item0, pad;
 };
 
+// One problem of a batched staged-strip forward projection (design 4): its
+// tables and sizes, a skip flag (nullptr: never skipped), and its first
+// workgroup in the launch divided by 8 (every problem takes a multiple of 8
+// workgroups, so that the 8 XCDs get equal shares of it).
+struct BatchFwd {
+    const float *x;
+    float *p;
+    const uint4 *tab;
+    const int *stage;
+    const int64_t *chunk_base, *off;
+    const int *skip;
+    int nstrips, nx, ny, na, nchunks, nblk;  // nblk: the problem's workgroups before padding
+    int64_t blk8;
+};
+
 // Device setup and operators (projector.hip).
 int geometry_upload(Geometry &g, hipStream_t st, std::string *err);
 void geometry_free_device(Geometry &g);
@@ -223,10 +245,21 @@ int64_t backward_items(const Geometry &g);
 FwdArgs fwd_args(const Geometry &g, const float *x, float *p, double *partials);
 BpArgs bp_args(const Geometry &g, const float *p, float *x, const float *bvec, double *partials);
 // Persistent back projection over nprob problems (device array of
-// descriptors, item0 ascending, `total` items, all with the same method);
-// d_counter is an int of scratch.
-int launch_backward_batch(int method, int mode, const BatchBp *d_probs, int nprob, int64_t total,
+// descriptors, item0 ascending, `total` items, all with the same method and
+// tile shape); d_counter is an int of scratch.
+int launch_backward_batch(int method, int mode, int shape, const BatchBp *d_probs, int nprob, int64_t total,
                           int *d_counter, hipStream_t st);
+// Select the back projector's tile shape (0 .. kBpShapes - 1).
+int geometry_set_back(Geometry &g, int shape);
+// Descriptor of g's staged-strip forward (g on design 4) at workgroup
+// blk8 * 8 of a batched launch; returns the workgroups it takes (a
+// multiple of 8).
+int64_t forward_batch_item(const Geometry &g, const float *x, float *p, const int *skip, int64_t blk8,
+                           BatchFwd &d);
+// One launch of the staged-strip forward over several problems, all at rpl
+// rows per lane: nblk workgroups (a multiple of 8), owner[i] = the problem
+// of workgroups 8i .. 8i + 7 (device arrays).
+int launch_forward_batch(int rpl, const BatchFwd *d_probs, const int *d_owner, int64_t nblk, hipStream_t st);
 
 // One problem of a (batched) SIRT solve (solver.hip).
 struct SirtProblem {

@@ -224,6 +224,21 @@ int arec_geom_set_forward(arec_ctx *ctx, arec_geom *geom, int32_t design, int32_
 
 int32_t arec_geom_forward_design(const arec_geom *geom) { return geom ? geom->g.fwd_design : 0; }
 
+int arec_geom_set_back(arec_ctx *ctx, arec_geom *geom, int32_t rows, int32_t angles) {
+    CHECK_ARG(ctx && geom, "NULL argument");
+    for (int s = 0; s < kBpShapes; s++)
+        if (kBpShapeRows[s] == rows && kBpShapeAngles[s] == angles) return geometry_set_back(geom->g, s);
+    set_error("back projector: tile shape must be 32 x 2, 16 x 4 or 64 x 2 (rows x angles)");
+    return AREC_EINVAL;
+}
+
+int arec_geom_get_back(const arec_geom *geom, int32_t *rows, int32_t *angles) {
+    CHECK_ARG(geom, "NULL argument");
+    if (rows) *rows = kBpShapeRows[geom->g.bp_shape];
+    if (angles) *angles = kBpShapeAngles[geom->g.bp_shape];
+    return 0;
+}
+
 int arec_geom_get_plan(const arec_geom *geom, int32_t *rows_per_lane, int32_t *min_groups, int32_t *nseg) {
     CHECK_ARG(geom, "NULL argument");
     if (rows_per_lane) *rows_per_lane = geom->g.fwd_design >= 2 ? geom->g.strip_rpl : geom->g.fwd_rpl;

@@ -927,7 +927,10 @@ __device__ __forceinline__ void strip_add_single(float (&acc)[kStripW + 4][RPL],
 }
 
 // The walk over chunks [K0, K1) of one (group, strip) for the wave's rows;
-// FULL: all 64*RPL rows lie inside the volume.  Per chunk, staged into LDS
+// FULL: all 64*RPL rows lie inside the volume and the row count is a
+// multiple of RPL (so every lane's RPL rows of a column are aligned for one
+// vector load); otherwise each row is loaded on its own, bounds-checked.
+// Per chunk, staged into LDS
 // one chunk ahead: the kSgChunk columns' rows (an empty slot, stage -1,
 // stages column 0: no entry reads it) and each wave's entries of the chunk
 // (at most kSgEnt, whole batches of one offset each; lane l fetches entry
@@ -1036,7 +1039,12 @@ __device__ __forceinline__ void staged_walk(float (*s_col)[kSgChunk][64 * RPL], 
                 w[k][2] = __uint_as_float(ec[k].w);
             }
             const unsigned d = ((unsigned)__builtin_amdgcn_readfirstlane(ec[0].x) >> 8) & 31u;
-            strip_add_single<0, kStripGroups, RPL>(acc, 1u << d, w, v);
+            unsigned present = 1u << d;
+            // opaque to the optimiser: knowing it is one bit, it turns the
+            // bit tests into a switch whose leaves copy every accumulator
+            // (at 1 row per lane past 128 VGPRs, into scratch)
+            asm volatile("" : "+s"(present));
+            strip_add_single<0, kStripGroups, RPL>(acc, present, w, v);
         }
         if (more) {
             store_stage(buf ^ 1);
@@ -1046,19 +1054,18 @@ __device__ __forceinline__ void staged_walk(float (*s_col)[kSgChunk][64 * RPL], 
     }
 }
 
-// At least 4 waves per SIMD, so that two workgroups share a CU.
+// Workgroup b of one staged-strip forward: angle group b % ngroups (one
+// angle per wave), then detector strip, then row chunk.
 template <int RPL>
-__global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void forward_staged_kernel(
-    const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ tab, const int *__restrict__ stage,
-    const int64_t *__restrict__ chunk_base, const int64_t *__restrict__ off, double *__restrict__ partials,
-    int nstrips, int nx, int ny, int na, int nchunks, const int *__restrict__ skip) {
-    if (skip && *skip) return;
+__device__ __forceinline__ void forward_staged_body(const float *__restrict__ x, float *__restrict__ p,
+                                                    const uint4 *__restrict__ tab, const int *__restrict__ stage,
+                                                    const int64_t *__restrict__ chunk_base,
+                                                    const int64_t *__restrict__ off, double *__restrict__ partials,
+                                                    const int nstrips, const int nx, const int ny, const int na,
+                                                    const int nchunks, const int b) {
     constexpr int ROWS = 64 * RPL;
     __shared__ __align__(16) float s_col[2][kSgChunk][ROWS];
     __shared__ uint4 s_ent[2][kSgAngles][kSgEnt];
-    const int nblk = gridDim.x;
-    int b = blockIdx.x;
-    if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
     const int ngroups = (na + kSgAngles - 1) / kSgAngles;
     const int G = b % ngroups, rest = b / ngroups, st = rest % nstrips, chunk = rest / nstrips;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -1071,12 +1078,49 @@ __global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(4, 8
     for (int i = 0; i < kStripW + 4; i++)
 #pragma unroll
         for (int r = 0; r < RPL; r++) acc[i][r] = 0.0f;
-    if (ybase + ROWS <= ny)
+    // whole chunk with every column's rows aligned to RPL floats: vector loads
+    if (ybase + ROWS <= ny && (ny & (RPL - 1)) == 0)
         staged_walk<RPL, true>(s_col, s_ent, x, tab, stage, off, K0, K1, wave, lane, ny, y0, acc);
     else
         staged_walk<RPL, false>(s_col, s_ent, x, tab, stage, off, K0, K1, wave, lane, ny, y0, acc);
     if (!active) return;
     strip_store<RPL>(acc, p, partials, a, st, chunk, nchunks, nstrips, nx, ny, y0, lane);
+}
+
+// At least 4 waves per SIMD, so that two workgroups share a CU.  With a
+// multiple of 8 workgroups each XCD takes a contiguous range of them
+// (neighbouring angle groups, which read the same columns).
+template <int RPL>
+__global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void forward_staged_kernel(
+    const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ tab, const int *__restrict__ stage,
+    const int64_t *__restrict__ chunk_base, const int64_t *__restrict__ off, double *__restrict__ partials,
+    int nstrips, int nx, int ny, int na, int nchunks, const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    const int nblk = gridDim.x;
+    int b = blockIdx.x;
+    if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+    forward_staged_body<RPL>(x, p, tab, stage, chunk_base, off, partials, nstrips, nx, ny, na, nchunks, b);
+}
+
+// The staged-strip forward of many problems in one launch (one grid, no
+// loop: a persistent loop kept the body's invariants in registers and
+// halved occupancy).  Problem k owns workgroups [8 blk8, 8 blk8 + 8 m) for a
+// padded count 8 m >= nblk; workgroups go round-robin over the 8 XCDs, so
+// local index l of the problem sits on XCD l & 7 and is remapped as in the
+// single launch: each XCD takes a contiguous eighth of every problem, and
+// all XCDs move through the problems together.  Padding workgroups return.
+template <int RPL>
+__global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(RPL == 2 ? 6 : 4, 8))) void forward_staged_batch_kernel(
+    const BatchFwd *__restrict__ probs, const int *__restrict__ owner) {
+    const int k = owner[blockIdx.x >> 3];
+    const BatchFwd q = probs[k];
+    if (q.skip && *q.skip) return;
+    const int l = (int)((int64_t)blockIdx.x - 8 * q.blk8);
+    const int m = ((q.nblk + 7) >> 3);  // workgroups per XCD
+    const int b = (l & 7) * m + (l >> 3);
+    if (b >= q.nblk) return;
+    forward_staged_body<RPL>(q.x, q.p, q.tab, q.stage, q.chunk_base, q.off, nullptr, q.nstrips, q.nx, q.ny, q.na,
+                             q.nchunks, b);
 }
 
 // Index of the problem that owns work item `it`: the last k with
@@ -1112,10 +1156,9 @@ constexpr int kBpThreads = kTileCols;  // one voxel column per thread
 // pipelined loop below: 32 rows x 2 angles 12.5 ms (128 VGPRs), 32 x 4
 // 12.9 ms, 16 x 4 14.3 ms, 16 x 8 14.9 ms, 64 x 2 14.3 ms, 24 x 4 15.8 ms,
 // 8 x 4 17.3 ms; the unpipelined loop took 20.8 ms at 16 x 4.
-constexpr int kBpRows = 32;
-constexpr int kBpRowPad = kBpRows + 4; // LDS row stride (16-byte aligned)
-constexpr int kBpAngBlk = 2;           // angles staged per LDS pass
-constexpr int kBpWin = 32;             // staged pixels per (angle)
+// The shape is a template parameter (kBpShapeRows / kBpShapeAngles, chosen
+// per geometry); the LDS row stride is the row count + 4 (16-byte aligned).
+constexpr int kBpWin = 32;  // staged pixels per (angle)
 
 template <int M>
 __device__ __forceinline__ double corner_xb(int X, int Z, const AngleConst &A, double radius,
@@ -1126,13 +1169,13 @@ __device__ __forceinline__ double corner_xb(int X, int Z, const AngleConst &A, d
 }
 
 // x[j][y] = sum_a sum_k w_k(a,j) p[a][y][p0+k] for the tile's columns (one
-// per thread) and kBpRows rows.  Per block of angles the tile's footprint (<= 32
-// pixels x kBpRows rows per angle) is staged in LDS with rows fastest, so one
+// per thread) and ROWS rows.  Per block of ANGB angles the tile's footprint
+// (<= 32 pixels x ROWS rows per angle) is staged in LDS with rows fastest, so one
 // 16-byte LDS read gives 4 rows of one tap.  Epilogue by mode:
 //  kBpStore      out = acc
 //  kBpSirtGrad   out = bvec - acc (FP32), partial += sum out*(double)out
 //  kBpStoreNorm  out = acc, partial += sum (double)(acc*acc)
-template <int M, int MODE>
+template <int M, int MODE, int ROWS, int ANGB>
 __device__ __forceinline__ void backward_body(const float *__restrict__ p, float *__restrict__ out,
                                               const float *__restrict__ bvec, const uint4 *__restrict__ btab,
                                               const int *__restrict__ tile_cols,
@@ -1140,6 +1183,7 @@ __device__ __forceinline__ void backward_body(const float *__restrict__ p, float
                                               const AngleConst *__restrict__ ang, double *__restrict__ partials,
                                               int *__restrict__ err, const double radius, const int xcent,
                                               const int nx, const int ny, const int na, const int nrowblk, int b) {
+    constexpr int kBpRows = ROWS, kBpAngBlk = ANGB, kBpRowPad = ROWS + 4;
     // Software pipeline over blocks of kBpAngBlk angles, one barrier per
     // block: while block k is accumulated from s_p[k & 1], the footprints of
     // block k + 1 are in flight to registers (then stored to s_p[(k+1) & 1])
@@ -1311,26 +1355,26 @@ __device__ __forceinline__ void backward_body(const float *__restrict__ p, float
     }
 }
 
-template <int M, int MODE>
+template <int M, int MODE, int S>
 __device__ __forceinline__ void backward_item(const BpArgs &B, int b) {
-    backward_body<M, MODE>(B.p, B.out, B.bvec, B.btab, B.tile_cols, B.tile_org, B.coord, B.ang, B.partials,
+    backward_body<M, MODE, kBpShapeRows[S], kBpShapeAngles[S]>(B.p, B.out, B.bvec, B.btab, B.tile_cols, B.tile_org, B.coord, B.ang, B.partials,
                            B.err, B.radius, B.xcent, B.nx, B.ny, B.na, B.nrowblk, b);
 }
 
-template <int M, int MODE>
+template <int M, int MODE, int S>
 __global__ __launch_bounds__(kBpThreads) void backward_kernel(BpArgs B, const int *__restrict__ skip) {
     if (skip && *skip) return;
     // Workgroup ids go round-robin over the XCDs, so each XCD takes the same
     // row blocks of neighbouring tiles, which share detector pixels (cheaper
     // than grouping a tile's row blocks on one XCD: 42 vs 61 GB fetched per
     // launch at 512^3/720).
-    backward_item<M, MODE>(B, blockIdx.x);
+    backward_item<M, MODE, S>(B, blockIdx.x);
 }
 
 // Persistent back projector over many problems: workgroups take work items
 // (one back-projection workgroup's job of one problem) from a queue until it
 // is empty.  *counter must be 0 at launch.
-template <int M, int MODE>
+template <int M, int MODE, int S>
 __global__ __launch_bounds__(kBpThreads) void backward_batch_kernel(const BatchBp *__restrict__ probs, int nprob,
                                                                    int total, int *__restrict__ counter) {
     __shared__ int s_item;
@@ -1345,7 +1389,7 @@ __global__ __launch_bounds__(kBpThreads) void backward_batch_kernel(const BatchB
         const int k = __builtin_amdgcn_readfirstlane(owner_of(probs, s_item0, nprob, it));
         const int *skip = probs[k].skip;
         if (skip && *skip) continue;
-        backward_item<M, MODE>(probs[k].a, it - probs[k].item0);
+        backward_item<M, MODE, S>(probs[k].a, it - probs[k].item0);
     }
 }
 
@@ -1364,7 +1408,8 @@ int64_t forward_items(const Geometry &g) {
 }
 
 int64_t backward_items(const Geometry &g) {
-    int nrowblk = (g.ny + kBpRows - 1) / kBpRows;
+    const int rows = kBpShapeRows[g.bp_shape];
+    int nrowblk = (g.ny + rows - 1) / rows;
     return (int64_t)g.ntiles * nrowblk;
 }
 
@@ -1404,7 +1449,7 @@ BpArgs bp_args(const Geometry &g, const float *p, float *x, const float *bvec, d
     B.nx = g.nx;
     B.ny = g.ny;
     B.na = g.na;
-    B.nrowblk = (g.ny + kBpRows - 1) / kBpRows;
+    B.nrowblk = (g.ny + kBpShapeRows[g.bp_shape] - 1) / kBpShapeRows[g.bp_shape];
     return B;
 }
 
@@ -1745,9 +1790,11 @@ int geometry_set_forward(Geometry &g, int design, int rpl, int min_groups, hipSt
             set_error("forward design 4 needs an uploaded geometry");
             return AREC_EINVAL;
         }
-        if (rpl == 0) rpl = (g.ny >= 256 && g.ny % 4 == 0) ? 4 : (g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1;
-        if (!fwd_rpl_allowed(g.ny, rpl)) {
-            set_error("forward plan: rows per lane must be 1, 2 or 4 and divide the row count");
+        // any row count: whole row chunks take vector loads only where the
+        // rows per lane divide the row count (column starts aligned)
+        if (rpl == 0) rpl = g.ny >= 256 ? 4 : g.ny >= 128 ? 2 : 1;
+        if (rpl != 1 && rpl != 2 && rpl != 4) {
+            set_error("forward plan: rows per lane must be 1, 2 or 4");
             return AREC_EINVAL;
         }
         if (g.d_strip) {
@@ -1886,14 +1933,76 @@ int launch_forward(const Geometry &g, const float *x, float *p, double *partials
     return 0;
 }
 
-template <int M>
-static void bp_dispatch(int mode, dim3 grid, hipStream_t st, const BpArgs &B, const int *skip) {
+template <int M, int S>
+static void bp_dispatch_shape(int mode, dim3 grid, hipStream_t st, const BpArgs &B, const int *skip) {
     if (mode == kBpSirtGrad)
-        backward_kernel<M, kBpSirtGrad><<<grid, kBpThreads, 0, st>>>(B, skip);
+        backward_kernel<M, kBpSirtGrad, S><<<grid, kBpThreads, 0, st>>>(B, skip);
     else if (mode == kBpStoreNorm)
-        backward_kernel<M, kBpStoreNorm><<<grid, kBpThreads, 0, st>>>(B, skip);
+        backward_kernel<M, kBpStoreNorm, S><<<grid, kBpThreads, 0, st>>>(B, skip);
     else
-        backward_kernel<M, kBpStore><<<grid, kBpThreads, 0, st>>>(B, skip);
+        backward_kernel<M, kBpStore, S><<<grid, kBpThreads, 0, st>>>(B, skip);
+}
+
+template <int M>
+static void bp_dispatch(int mode, int shape, dim3 grid, hipStream_t st, const BpArgs &B, const int *skip) {
+    static_assert(kBpShapes == 3, "one branch per shape");
+    if (shape == 1)
+        bp_dispatch_shape<M, 1>(mode, grid, st, B, skip);
+    else if (shape == 2)
+        bp_dispatch_shape<M, 2>(mode, grid, st, B, skip);
+    else
+        bp_dispatch_shape<M, 0>(mode, grid, st, B, skip);
+}
+
+int geometry_set_back(Geometry &g, int shape) {
+    if (shape < 0 || shape >= kBpShapes) {
+        set_error("back projector: unknown tile shape");
+        return AREC_EINVAL;
+    }
+    g.bp_shape = shape;
+    return 0;
+}
+
+int64_t forward_batch_item(const Geometry &g, const float *x, float *p, const int *skip, int64_t blk8,
+                           BatchFwd &d) {
+    const int rpl = g.strip_rpl, nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
+    const int64_t nblk = (int64_t)((g.na + kSgAngles - 1) / kSgAngles) * g.nstrips * nchunks;
+    d.x = x;
+    d.p = p;
+    d.tab = g.d_sg_tab;
+    d.stage = g.d_sg_stage;
+    d.chunk_base = g.d_sg_chunk_base;
+    d.off = g.d_sg_off;
+    d.skip = skip;
+    d.nstrips = g.nstrips;
+    d.nx = g.nx;
+    d.ny = g.ny;
+    d.na = g.na;
+    d.nchunks = nchunks;
+    d.nblk = (int)nblk;
+    d.blk8 = blk8;
+    return (nblk + 7) / 8 * 8;
+}
+
+int launch_forward_batch(int rpl, const BatchFwd *d_probs, const int *d_owner, int64_t nblk, hipStream_t st) {
+    if (nblk <= 0) return 0;
+    if (nblk % 8 || nblk > INT32_MAX) {
+        set_error("batched forward projection: workgroup count must be a multiple of 8 below 2^31");
+        return AREC_ESIZE;
+    }
+    dim3 grid((unsigned)nblk);
+    if (rpl == 4)
+        forward_staged_batch_kernel<4><<<grid, kSgThreads, 0, st>>>(d_probs, d_owner);
+    else if (rpl == 2)
+        forward_staged_batch_kernel<2><<<grid, kSgThreads, 0, st>>>(d_probs, d_owner);
+    else if (rpl == 1)
+        forward_staged_batch_kernel<1><<<grid, kSgThreads, 0, st>>>(d_probs, d_owner);
+    else {
+        set_error("batched forward projection: rows per lane must be 1, 2 or 4");
+        return AREC_EINVAL;
+    }
+    AREC_HIP(hipGetLastError());
+    return 0;
 }
 
 int launch_backward(const Geometry &g, const float *p, float *x, int mode, const float *bvec,
@@ -1907,11 +2016,11 @@ int launch_backward(const Geometry &g, const float *p, float *x, int mode, const
     dim3 grid((unsigned)backward_items(g));
     BpArgs B = bp_args(g, p, x, bvec, partials);
     if (g.method == kSQ)
-        bp_dispatch<kSQ>(mode, grid, st, B, skip);
+        bp_dispatch<kSQ>(mode, g.bp_shape, grid, st, B, skip);
     else if (g.method == kKB)
-        bp_dispatch<kKB>(mode, grid, st, B, skip);
+        bp_dispatch<kKB>(mode, g.bp_shape, grid, st, B, skip);
     else
-        bp_dispatch<kLinear>(mode, grid, st, B, skip);
+        bp_dispatch<kLinear>(mode, g.bp_shape, grid, st, B, skip);
     AREC_HIP(hipGetLastError());
     return 0;
 }
@@ -1931,24 +2040,32 @@ static int resident_grid(K kernel, int threads, int *out) {
     return 0;
 }
 
-template <int M, int MODE>
+template <int M, int MODE, int S>
 static int bp_batch(const BatchBp *d_probs, int nprob, int total, int *d_counter, hipStream_t st) {
     int grid = 0, rc;
-    if ((rc = resident_grid(backward_batch_kernel<M, MODE>, kBpThreads, &grid))) return rc;
+    if ((rc = resident_grid(backward_batch_kernel<M, MODE, S>, kBpThreads, &grid))) return rc;
     grid = std::min(grid, total);
-    backward_batch_kernel<M, MODE><<<grid, kBpThreads, 0, st>>>(d_probs, nprob, total, d_counter);
+    backward_batch_kernel<M, MODE, S><<<grid, kBpThreads, 0, st>>>(d_probs, nprob, total, d_counter);
     return 0;
 }
 
-template <int M>
+template <int M, int S>
 static int bp_batch_mode(int mode, const BatchBp *d_probs, int nprob, int total, int *d_counter,
                          hipStream_t st) {
-    if (mode == kBpSirtGrad) return bp_batch<M, kBpSirtGrad>(d_probs, nprob, total, d_counter, st);
-    if (mode == kBpStoreNorm) return bp_batch<M, kBpStoreNorm>(d_probs, nprob, total, d_counter, st);
-    return bp_batch<M, kBpStore>(d_probs, nprob, total, d_counter, st);
+    if (mode == kBpSirtGrad) return bp_batch<M, kBpSirtGrad, S>(d_probs, nprob, total, d_counter, st);
+    if (mode == kBpStoreNorm) return bp_batch<M, kBpStoreNorm, S>(d_probs, nprob, total, d_counter, st);
+    return bp_batch<M, kBpStore, S>(d_probs, nprob, total, d_counter, st);
 }
 
-int launch_backward_batch(int method, int mode, const BatchBp *d_probs, int nprob, int64_t total,
+template <int M>
+static int bp_batch_shape(int mode, int shape, const BatchBp *d_probs, int nprob, int total, int *d_counter,
+                          hipStream_t st) {
+    if (shape == 1) return bp_batch_mode<M, 1>(mode, d_probs, nprob, total, d_counter, st);
+    if (shape == 2) return bp_batch_mode<M, 2>(mode, d_probs, nprob, total, d_counter, st);
+    return bp_batch_mode<M, 0>(mode, d_probs, nprob, total, d_counter, st);
+}
+
+int launch_backward_batch(int method, int mode, int shape, const BatchBp *d_probs, int nprob, int64_t total,
                           int *d_counter, hipStream_t st) {
     if (nprob <= 0 || total <= 0) return 0;
     if (total > INT32_MAX) {
@@ -1957,9 +2074,9 @@ int launch_backward_batch(int method, int mode, const BatchBp *d_probs, int npro
     }
     zero_counter_kernel<<<1, 1, 0, st>>>(d_counter);
     int rc;
-    if (method == kSQ) rc = bp_batch_mode<kSQ>(mode, d_probs, nprob, (int)total, d_counter, st);
-    else if (method == kKB) rc = bp_batch_mode<kKB>(mode, d_probs, nprob, (int)total, d_counter, st);
-    else rc = bp_batch_mode<kLinear>(mode, d_probs, nprob, (int)total, d_counter, st);
+    if (method == kSQ) rc = bp_batch_shape<kSQ>(mode, shape, d_probs, nprob, (int)total, d_counter, st);
+    else if (method == kKB) rc = bp_batch_shape<kKB>(mode, shape, d_probs, nprob, (int)total, d_counter, st);
+    else rc = bp_batch_shape<kLinear>(mode, shape, d_probs, nprob, (int)total, d_counter, st);
     if (rc) return rc;
     AREC_HIP(hipGetLastError());
     return 0;

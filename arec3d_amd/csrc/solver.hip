@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -459,39 +461,107 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
     AREC_HIP(hipMemcpyAsync(S, s0.data(), sizeof(SirtState) * nprob, hipMemcpyHostToDevice, st));
     AREC_HIP(hipMemcpyAsync(dvec.p, vec.data(), sizeof(SirtVec) * nprob, hipMemcpyHostToDevice, st));
 
-    // Projector launches of a pass.  Batched: each problem's forward
-    // projection is its own launch with its own autotuned plan (the grid
-    // remap gives every XCD a contiguous range of neighbouring angle
-    // groups); the back projections of all problems share one persistent
-    // launch, problems in decreasing order of angle count so that the long
-    // items start first.  A persistent forward over all problems measured
-    // 0.66-0.85x of the problems solved one after the other -- its loop kept
-    // the inlined body's invariants in registers (86 instead of 44 VGPRs at
-    // 2 rows per lane), halving occupancy -- where separate forward
-    // launches make the batch 1.03x.
-    int64_t bp_total = 0;
+    // Projector launches of a pass.  Batched: the problems on the staged-strip
+    // forward design go into one forward launch per rows-per-lane class that
+    // holds at least two of them (every problem keeps its own tables and
+    // rows per lane; AREC_NO_FWD_BATCH set: one launch per problem); the
+    // others get a launch each with their own plans.  The back projections of
+    // all problems with one tile shape share one persistent launch, problems
+    // in decreasing order of angle count so that the long items start first.  (A persistent
+    // forward over all problems, a loop over work items, measured 0.66-0.85x
+    // of the problems solved one after the other: the loop kept the inlined
+    // body's invariants in registers, 86 instead of 44 VGPRs at 2 rows per
+    // lane, halving occupancy.  The batched forward launch is one grid with
+    // no loop.)
+    // Back classes: problems of one tile shape, in decreasing order of angle
+    // count, item0 counted within the class; one persistent launch each.
+    struct BpClass {
+        int shape, first, count;
+        int64_t total;
+    };
+    std::vector<BpClass> bclass;
     std::vector<BatchBp> hb0;  // A^T b (store)
     std::vector<BatchBp> hb;   // gradient pass
     if (nprob > 1) {
         std::vector<int> border(nprob);
         for (int k = 0; k < nprob; k++) border[k] = k;
         std::stable_sort(border.begin(), border.end(), [&](int a, int b) { return P[a].g->na > P[b].g->na; });
-        for (int k : border) {
-            const Geometry &g = *P[k].g;
-            BatchBp e;
-            e.a = bp_args(g, P[k].b, bdata.as<float>() + o_nnz[k], nullptr, nullptr);
-            e.skip = nullptr;
-            e.item0 = (int)bp_total;
-            e.pad = 0;
-            hb0.push_back(e);
-            e.a = bp_args(g, proj.as<float>() + o_proj[k], vec[k].grad, vec[k].bdata, bp_part.as<double>() + o_bp[k]);
-            e.skip = &S[k].skip_proj;
-            hb.push_back(e);
-            bp_total += backward_items(g);
+        for (int shape = 0; shape < kBpShapes; shape++) {
+            BpClass c{shape, (int)hb0.size(), 0, 0};
+            for (int k : border) {
+                const Geometry &g = *P[k].g;
+                if (g.bp_shape != shape) continue;
+                BatchBp e;
+                e.a = bp_args(g, P[k].b, bdata.as<float>() + o_nnz[k], nullptr, nullptr);
+                e.skip = nullptr;
+                e.item0 = (int)c.total;
+                e.pad = 0;
+                hb0.push_back(e);
+                e.a = bp_args(g, proj.as<float>() + o_proj[k], vec[k].grad, vec[k].bdata,
+                              bp_part.as<double>() + o_bp[k]);
+                e.skip = &S[k].skip_proj;
+                hb.push_back(e);
+                c.total += backward_items(g);
+                c.count++;
+            }
+            if (c.count) bclass.push_back(c);
         }
         if ((rc = alloc(dbp0, sizeof(BatchBp) * nprob)) || (rc = alloc(dbp, sizeof(BatchBp) * nprob))) return rc;
         AREC_HIP(hipMemcpyAsync(dbp0.p, hb0.data(), sizeof(BatchBp) * nprob, hipMemcpyHostToDevice, st));
         AREC_HIP(hipMemcpyAsync(dbp.p, hb.data(), sizeof(BatchBp) * nprob, hipMemcpyHostToDevice, st));
+    }
+    auto back_batch = [&](int mode, DevBuf &d) -> int {
+        for (const BpClass &c : bclass) {
+            int r = launch_backward_batch(method, mode, c.shape, d.as<BatchBp>() + c.first, c.count, c.total,
+                                          dcnt.as<int>(), st);
+            if (r) return r;
+        }
+        return 0;
+    };
+
+    struct FwdClass {
+        int rpl, prob0, own0;
+        int64_t nblk;
+    };
+    std::vector<FwdClass> fclass;
+    std::vector<int> fsolo;  // problems whose forward is launched on its own
+    DevBuf dfwd, down;
+    if (nprob > 1) {
+        const bool batch_fwd = getenv("AREC_NO_FWD_BATCH") == nullptr;
+        std::vector<BatchFwd> hf;
+        std::vector<int> own;
+        for (int rpl : {4, 2, 1}) {
+            std::vector<int> ks;
+            for (int k = 0; k < nprob; k++)
+                if (P[k].g->fwd_design == 4 && P[k].g->strip_rpl == rpl) ks.push_back(k);
+            if (!batch_fwd || ks.size() < 2) continue;
+            FwdClass c{rpl, (int)hf.size(), (int)own.size(), 0};
+            for (int i = 0; i < (int)ks.size(); i++) {
+                const int k = ks[i];
+                BatchFwd d;
+                const int64_t n = forward_batch_item(*P[k].g, P[k].x, proj.as<float>() + o_proj[k], &S[k].skip_proj,
+                                                     c.nblk / 8, d);
+                hf.push_back(d);
+                own.insert(own.end(), (size_t)(n / 8), i);
+                c.nblk += n;
+            }
+            if (c.nblk > INT32_MAX) {
+                set_error("batched SIRT: too many forward workgroups in one launch");
+                return AREC_ESIZE;
+            }
+            fclass.push_back(c);
+        }
+        for (int k = 0; k < nprob; k++) {
+            bool in = false;
+            for (const FwdClass &c : fclass) in |= P[k].g->fwd_design == 4 && P[k].g->strip_rpl == c.rpl;
+            if (!in) fsolo.push_back(k);
+        }
+        if (!hf.empty()) {
+            if ((rc = alloc(dfwd, sizeof(BatchFwd) * hf.size())) || (rc = alloc(down, sizeof(int) * own.size())))
+                return rc;
+            AREC_HIP(hipMemcpyAsync(dfwd.p, hf.data(), sizeof(BatchFwd) * hf.size(), hipMemcpyHostToDevice, st));
+            AREC_HIP(hipMemcpyAsync(down.p, own.data(), sizeof(int) * own.size(), hipMemcpyHostToDevice, st));
+        }
     }
 
     // Columns that never contribute (linear pair's last two) stay zero in
@@ -511,7 +581,7 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
     if (nprob == 1) {
         if ((rc = launch_backward(*P[0].g, P[0].b, bdata.as<float>(), kBpStore, nullptr, nullptr, st, nullptr)))
             return rc;
-    } else if ((rc = launch_backward_batch(method, kBpStore, dbp0.as<BatchBp>(), nprob, bp_total, dcnt.as<int>(), st))) {
+    } else if ((rc = back_batch(kBpStore, dbp0))) {
         return rc;
     }
 
@@ -528,10 +598,14 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
                                      bp_part.as<double>(), st, &S->skip_proj)))
                 return r;
         } else {
-            for (int k = 0; k < nprob; k++)
+            for (const FwdClass &c : fclass)
+                if ((r = launch_forward_batch(c.rpl, dfwd.as<BatchFwd>() + c.prob0, down.as<int>() + c.own0, c.nblk,
+                                              st)))
+                    return r;
+            for (int k : fsolo)
                 if ((r = launch_forward(*P[k].g, P[k].x, proj.as<float>() + o_proj[k], nullptr, st, &S[k].skip_proj)))
                     return r;
-            if ((r = launch_backward_batch(method, kBpSirtGrad, dbp.as<BatchBp>(), nprob, bp_total, dcnt.as<int>(), st)))
+            if ((r = back_batch(kBpSirtGrad, dbp)))
                 return r;
         }
         sirt_first_pass<<<dim3(gmax, nprob), kVecThreads, 0, st>>>(V, fp_part.as<double>(), S);

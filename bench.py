@@ -235,6 +235,8 @@ def align_workload(args):
     g = A.Geometry(ctx, "sq", nx, ny, ang)
     if args.plan:
         g.set_forward(*args.plan)
+        if args.back_tile:
+            g.set_back(*args.back_tile)
     t_geom = time.perf_counter() - t0
     log(f"geometry {nx}x{ny}x{na}: {t_geom:.1f} s, {g.device_bytes() / 1e9:.1f} GB of tables")
     support = 1000.0 * f
@@ -391,7 +393,7 @@ def batch_workload(args):
     rng = np.random.default_rng(3000)
     nprob, iters = args.samples, args.sirt_iters
     ctx = A.Context(0)
-    geoms, bs, lams = [], [], []
+    geoms, bs, lams, tuned = [], [], [], []
     t0 = time.perf_counter()
     for k in range(nprob):
         nx = int(rng.integers(96, 385))
@@ -400,7 +402,7 @@ def batch_workload(args):
         ne = int(rng.integers(8, 31))
         ang = (np.arange(na) * np.pi / na).astype(np.float32)
         g = A.Geometry(ctx, "sq", nx, ny, ang)
-        autotune(ctx, g)
+        tuned.append(autotune(ctx, g))
         sup = 0.47 * nx
         f = nx / 128.0
         prm = P.ellipsoid_params(3000 + k, ne, sup, ny, axes=(4.0 * f, 30.0 * f), dens=(0.1, 1.0))
@@ -432,6 +434,16 @@ def batch_workload(args):
     ctx.synchronize()
     t_seq = time.perf_counter() - t0
     log(f"one after the other: {t_seq:.2f} s")
+    # the autotuned plans, and the projector time they imply per solve (one
+    # forward and one back launch per iteration, isolated launches)
+    plans, tiles = {}, {}
+    for t in tuned:
+        kp, kt = f"design {t[0]}, {t[1]} rows/lane", f"{t[4][0]} rows x {t[4][1]} angles"
+        plans[kp] = plans.get(kp, 0) + 1
+        tiles[kt] = tiles.get(kt, 0) + 1
+    t_kernels = sum((t[3] + t[5]) * iters for t in tuned) * 1e-3
+    log(f"forward plans {plans}, back tiles {tiles}; autotuned projector times x {iters} iterations: "
+        f"{t_kernels:.2f} s")
     sample = [int(i) for i in np.random.default_rng(7).choice(nprob, 4, replace=False)]
     exact = all(res[k][0] == single[k][0][0] and np.array_equal(xs[k].download(), single[k][1].download())
                 for k in sample)
@@ -453,6 +465,9 @@ def batch_workload(args):
             "sampled_problems_bit_identical_to_single": exact,
             "sampled_problems": sample,
             "restarts": int(sum(sum(1 for h in r[0] if h[1] == 1) for r in res)),
+            "forward_plans": plans,
+            "back_tiles": tiles,
+            "sum_of_autotuned_projector_times_s": round(t_kernels, 3),
         },
     }
     print(json.dumps(out), flush=True)
@@ -476,6 +491,8 @@ def cgls_workload(args):
     g = A.Geometry(ctx, "sq", nx, ny, ang)
     if args.plan:
         g.set_forward(*args.plan)
+        if args.back_tile:
+            g.set_back(*args.back_tile)
     elif not args.no_autotune:
         from arec3d_amd.autotune import autotune
 
@@ -535,6 +552,7 @@ def cgls_workload(args):
             "workload": f"CGLS (SQ) {iters} iterations, {nx}x{nx}x{ny} volume, {na} angles",
             "forward_plan": {"design": FORWARD_DESIGNS[g.forward_design],
                              "rows_per_lane": g.get_plan()[0]},
+            "back_tile": {"rows": g.back_tile[0], "angles_per_pass": g.back_tile[1]},
             "voxel_updates_per_s": round(vu / dt, 1),
             "ms_per_iteration": round(dt * 1e3 / iters, 3),
             "final_relnrm": hist[-1][2],
@@ -566,6 +584,8 @@ def main():
     ap.add_argument("--plan", type=int, nargs=3, metavar=("DESIGN", "ROWS_PER_LANE", "MIN_GROUPS"),
                     help="forward projector design (1 sweep, 2 depth strips, 3 depth strips by angle pairs, 4 staged strips) "
                          "and plan (default: autotuned)")
+    ap.add_argument("--back-tile", type=int, nargs=2, metavar=("ROWS", "ANGLES"),
+                    help="back projector tile with --plan: 32 2 (default), 16 4 or 64 2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=None, help="plain SIRT solves (default: --steps)")
     ap.add_argument("--plain-only", action="store_true", help="time plain SIRT only (no network)")
@@ -621,6 +641,8 @@ def main():
     t_geom = time.perf_counter() - t0
     if args.plan:
         g.set_forward(*args.plan)
+        if args.back_tile:
+            g.set_back(*args.back_tile)
     elif not args.no_autotune:
         from arec3d_amd.autotune import autotune
 
@@ -718,6 +740,7 @@ def main():
         "lam": lam,
         "forward_plan": {"design": FORWARD_DESIGNS[plan[0]], "rows_per_lane": plan[1],
                          "min_groups": plan[2]},
+        "back_tile": {"rows": g.back_tile[0], "angles_per_pass": g.back_tile[1]},
         "geometry_setup_s": round(t_geom, 3),
         "forward_ms": round(ms_f, 3),
         "backward_ms": round(ms_b, 3),

@@ -103,11 +103,19 @@ int arec_geom_get_plan(const arec_geom *geom, int32_t *rows_per_lane, int32_t *m
  * selected, refused above 32 GiB, and released when design 1 is selected),
  * 3 = depth strips walked by angle pairs (each wave carries two angles, one
  * table entry per column of either, rows per lane 1 or 2; table ~1.13 x the
- * sweep table, same limits).  Results agree to FP32 summation order.
- * arec_geom_get_plan reports the selected design's rows per lane. */
+ * sweep table, same limits), 4 = staged strips (groups of 8 angles walk
+ * chunks of 32 columns staged in LDS; rows per lane 1, 2 or 4; table about
+ * 1.2 x the sweep table, same limits).  Results agree to FP32 summation
+ * order.  arec_geom_get_plan reports the selected design's rows per lane. */
 int arec_geom_set_forward(arec_ctx *ctx, arec_geom *geom, int32_t design, int32_t rows_per_lane,
                           int32_t min_groups);
 int32_t arec_geom_forward_design(const arec_geom *geom);
+/* Back projector tile shape (no counterpart in the reference): detector rows
+ * per workgroup x angles staged per LDS pass, one of 32 x 2 (the default),
+ * 16 x 4 and 64 x 2; other pairs return AREC_EINVAL.  Every shape sums each
+ * voxel's angles in the same order, so results are the same bits. */
+int arec_geom_set_back(arec_ctx *ctx, arec_geom *geom, int32_t rows, int32_t angles);
+int arec_geom_get_back(const arec_geom *geom, int32_t *rows, int32_t *angles);
 
 /* ---- device-resident operators --------------------------------------- */
 /* p = A x   (arecProject2D / _KB / _SQ, arecproject.cpp:19-99,351-437,540-648) */

@@ -7,7 +7,8 @@ of the forward projector's launch plans.
 * Batched SIRT over problems of uneven width, height and angle count gives
   each problem exactly (bit for bit) the history and volume of its own
   single-problem SIRT, including problems in different rows-per-lane
-  classes, on either forward design, and one that restarts.
+  classes, on any forward design (design-4 problems sharing one forward
+  launch per rows-per-lane class), and one that restarts.
 * Each batched problem's history agrees with the FP32 oracle under the
   solver tests' tolerance (1e-5 relative, or 4x the FP32/FP64 oracle spread).
 """
@@ -105,6 +106,64 @@ def test_batched_sirt_with_depth_strips_equals_single(ctx):
         assert res[k][0] == h1, k
         assert res[k][1] == l1
         assert np.array_equal(xs[k].download(), xk.download()), k
+
+
+STAGED = [  # (nx, ny, na, design, rows per lane): two rows-per-lane classes of design 4
+    (64, 140, 45, 4, 2),   # 140 rows: one whole 128-row chunk and a partial one
+    (48, 12, 30, 4, 2),
+    (97, 40, 37, 4, 2),    # angle count leaves a partial last group of 8
+    (80, 260, 90, 4, 4),   # one whole 256-row chunk and a partial one
+    (41, 20, 20, 4, 4),
+    (56, 16, 24, 4, 1),    # the only problem of its class: launched on its own
+    (33, 8, 30, 2, 2),     # depth strips: launched on its own
+    (45, 9, 17, 1, 0),     # detector sweep: launched on its own
+]
+
+
+BACK_TILES = [(32, 2), (16, 4), (64, 2)]
+
+
+@pytest.mark.parametrize("batched_forward", [True, False])
+def test_batched_sirt_staged_forward_equals_single(ctx, monkeypatch, batched_forward):
+    """Problems on forward design 4 share one forward launch per rows-per-lane
+    class (classes of 2 and 3 problems here, whole and partial row chunks);
+    a class of one, and problems on designs 1 and 2, get launches of their
+    own; the back projections run as one persistent launch per back tile
+    shape.  Each batched problem equals its single-problem SIRT bit for bit,
+    with the shared forward launches and with one launch per problem
+    (AREC_NO_FWD_BATCH)."""
+    import arec3d_amd as A
+
+    print(f"\n[case] batched sirt staged forward, shared launch {batched_forward}", flush=True)
+    if not batched_forward:
+        monkeypatch.setenv("AREC_NO_FWD_BATCH", "1")
+    probs = []
+    for k, (nx, ny, na, design, rpl) in enumerate(STAGED):
+        b, ang = problem("sq", nx, ny, na, 200 + k)
+        g = A.Geometry(ctx, "sq", nx, ny, ang)
+        if design != 1:
+            g.set_forward(design, rpl)
+        assert g.forward_design == design
+        g.set_back(*BACK_TILES[k % 3])  # three back classes: one persistent launch each
+        probs.append((g, b, ang, 1.0 / (na * nx)))
+    maxit, tol = 12, 1e-6
+    bs = [ctx.to_device(b) for _, b, _, _ in probs]
+    xs = [ctx.empty(g.vol_shape) for g, _, _, _ in probs]
+    res = A.sirt_batch(ctx, [p[0] for p in probs], bs, xs, [p[3] for p in probs], maxit, tol)
+    for k, (g, b, ang, lam) in enumerate(probs):
+        xk = ctx.empty(g.vol_shape)
+        h1, l1 = g.sirt(bs[k], xk, lam, maxit, tol)
+        assert res[k][0] == h1, k
+        assert res[k][1] == l1
+        assert np.array_equal(xs[k].download(), xk.download()), k
+    # and against the oracle on one staged problem of each shared class
+    for k in (0, 4):
+        g, b, ang, lam = probs[k]
+        _, xo, ho, _ = O.sirt("sq", b, ang, lam, maxit, tol)
+        _, _, ho64, _ = O.sirt("sq", b, ang, lam, maxit, tol, dtype=np.float64)
+        check_hist(res[k][0], ho, ho64)
+        x = xs[k].download().astype(np.float64)
+        assert np.linalg.norm(x - xo) <= 1e-4 * np.linalg.norm(xo)
 
 
 def test_batched_sirt_vs_oracle(ctx):

@@ -216,3 +216,103 @@ def test_forward_depth_strips_vs_oracle(ctx, design, method, nx, ny, na, rpl):
     assert rel_views(dp.download(), strips).max() <= TOL
     g.set_forward(1)
     assert g.forward_design == 1
+
+
+@pytest.mark.parametrize("method", ["sq", "kb", "linear"])
+def test_back_tile_shapes_same_bits(ctx, method):
+    """Every back-projector tile shape (rows per workgroup x angles per LDS
+    pass) sums each voxel's angles in the same order, so all give the same
+    bits, in every epilogue mode; 70 rows leave a partial tile for each
+    height and 37 angles a partial last angle block.  The default shape is
+    the oracle-checked one (test_projector_vs_oracle); here each shape is
+    also checked against the oracle directly."""
+    import arec3d_amd as A
+
+    print(f"\n[case] back tile shapes {method}", flush=True)
+    nx, ny, na = 97, 70, 37
+    rng = np.random.default_rng(70)
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    g = A.Geometry(ctx, method, nx, ny, ang)
+    assert g.back_tile == (32, 2)
+    ph = rng.random(g.proj_shape, dtype=np.float32)
+    ox = O.backproject(method, ph, ang, nx, ny)
+    p = ctx.to_device(ph)
+    x = ctx.empty(g.vol_shape)
+    first = None
+    for tile in [(32, 2), (16, 4), (64, 2)]:
+        g.set_back(*tile)
+        assert g.back_tile == tile
+        g.backproject(p, x)
+        gx = x.download()
+        if first is None:
+            first = gx
+        assert np.array_equal(gx, first), tile
+        gc, oc = (gx[:-2], ox[:-2]) if method == "linear" else (gx, ox)
+        nb = (len(gc) + 63) // 64
+        pad = np.zeros((nb * 64 - len(gc), ny), np.float32)
+        assert rel_views(np.concatenate([gc, pad]).reshape(nb, -1),
+                         np.concatenate([oc, pad]).reshape(nb, -1)).max() <= TOL
+    with pytest.raises(A.ArecError):
+        g.set_back(8, 4)
+    assert g.back_tile == (64, 2)
+
+
+def test_back_tile_shapes_sirt_and_cgls(ctx):
+    """SIRT (gradient epilogue) and CGLS (norm epilogue) with the 16 x 4 and
+    64 x 2 tiles give the default tile's volume bit for bit and its
+    history to FP64 summation order of the per-tile partial norms."""
+    import arec3d_amd as A
+    from test_gpu_solver import problem
+
+    print("\n[case] back tile shapes in SIRT and CGLS", flush=True)
+    nx, ny, na = 64, 40, 45
+    b, ang = problem("sq", nx, ny, na, 404)
+    g = A.Geometry(ctx, "sq", nx, ny, ang)
+    db = ctx.to_device(b)
+    lam = 1.0 / (na * nx)
+    out = {}
+    for tile in [(32, 2), (16, 4), (64, 2)]:
+        g.set_back(*tile)
+        xs, xc = ctx.empty(g.vol_shape), ctx.empty(g.vol_shape)
+        hs, _ = g.sirt(db, xs, lam, 15, 0.0)
+        hc = g.cgls(db, xc, 10, 0.0)
+        out[tile] = (hs, xs.download(), hc, xc.download())
+    hs0, xs0, hc0, xc0 = out[(32, 2)]
+    for tile in [(16, 4), (64, 2)]:
+        hs, xs, hc, xc = out[tile]
+        assert np.array_equal(xs, xs0), tile
+        assert [h[:2] for h in hs] == [h[:2] for h in hs0]
+        for h, h0 in zip(hs, hs0):
+            assert abs(h[2] - h0[2]) <= 1e-12 * abs(h0[2]) + 1e-30
+        assert len(hc) == len(hc0)
+        for h, h0 in zip(hc, hc0):
+            assert abs(h[2] - h0[2]) <= 1e-5 * abs(h0[2]) + 1e-30
+        assert np.linalg.norm(xc - xc0) <= 1e-5 * np.linalg.norm(xc0)
+
+
+@pytest.mark.parametrize("method", ["sq", "kb", "linear"])
+@pytest.mark.parametrize("nx,ny,na,rpl", [(64, 71, 45, 2), (97, 131, 37, 2), (129, 301, 90, 4), (48, 263, 20, 4),
+                                          (40, 9, 13, 4)])
+def test_staged_forward_rows_not_multiple_of_rpl(ctx, method, nx, ny, na, rpl):
+    """Design 4 accepts any row count: where the rows per lane do not divide
+    it, whole chunks (here 131 rows at 2 per lane, 301 and 263 at 4) load
+    each row on its own instead of one vector per lane; results match the
+    oracle and the sweep to FP32 summation order."""
+    import arec3d_amd as A
+
+    print(f"\n[case] staged forward, unaligned rows {method} {nx}x{ny}x{na} rpl={rpl}", flush=True)
+    rng = np.random.default_rng(ny * 13 + na)
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    g = A.Geometry(ctx, method, nx, ny, ang)
+    x = rng.random(g.vol_shape).astype(np.float32)
+    if method == "linear":
+        x[-2:] = 0.0
+    dx, dp = ctx.to_device(x), ctx.empty(g.proj_shape)
+    g.project(dx, dp)
+    sweep = dp.download()
+    g.set_forward(4, rpl)
+    assert g.forward_design == 4 and g.get_plan()[0] == rpl
+    g.project(dx, dp)
+    staged = dp.download()
+    assert rel_views(staged, O.project(method, x, ang, nx, ny)).max() <= TOL
+    assert rel_views(staged, sweep).max() <= TOL

@@ -248,9 +248,10 @@ def test_solvers_with_depth_strip_forward_vs_oracle(ctx, method, design):
 
 
 def test_autotune_designs(ctx, tmp_path):
-    """The autotuner times both forward designs, sets the fastest, caches it
-    and applies the cached plan on the next geometry of the same shape;
-    selecting design 1 releases the depth-strip table."""
+    """The autotuner times every forward design and back-projector tile,
+    sets the fastest, caches them and applies the cached plan on the next
+    geometry of the same shape; selecting design 1 releases the depth-strip
+    table."""
     import arec3d_amd as A
     from arec3d_amd.autotune import autotune, candidates
 
@@ -262,11 +263,13 @@ def test_autotune_designs(ctx, tmp_path):
     base = g.device_bytes()
     assert {c[0] for c in candidates(g)} == {1, 2, 3, 4}
     cache = str(tmp_path / "plans.json")
-    d, r, m, ms = autotune(ctx, g, cache=cache, reps=1)
+    d, r, m, ms, tile, bms = autotune(ctx, g, cache=cache, reps=1)
     assert g.forward_design == d and g.get_plan()[0] == r and ms > 0
+    assert g.back_tile == tile and bms > 0
     g2 = A.Geometry(ctx, "sq", nx, ny, ang)
-    assert autotune(ctx, g2, cache=cache)[:3] == (d, r, m)
-    assert g2.forward_design == d
+    assert g2.back_tile == (32, 2)
+    assert autotune(ctx, g2, cache=cache)[:5] == (d, r, m, ms, tile)
+    assert g2.forward_design == d and g2.back_tile == tile
     g2.set_forward(2, 4)
     assert g2.device_bytes() > base
     g2.set_forward(1, 4, 0)
