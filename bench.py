@@ -519,16 +519,19 @@ def cgls_workload(args):
     gs = A.Geometry(ctx, "sq", nx, nr, ang)
     xs = ctx.empty(gs.vol_shape)
     hs = gs.cgls(ctx.to_device(bsub), xs, iters, 0.0)
-    log(f"GPU CGLS on rows {r0}..{r0 + nr} done; FP64 and FP32 oracle solves of those rows follow "
-        f"(one thread each, about {45 * nr} s)")
+    log(f"GPU CGLS on rows {r0}..{r0 + nr} done; FP64, FP32 and FP32 device-arithmetic oracle solves of "
+        f"those rows follow (one thread each, about {45 * nr} s)")
     import concurrent.futures as cf
 
     t1 = time.perf_counter()
-    with cf.ThreadPoolExecutor(max_workers=2) as ex:
+    with cf.ThreadPoolExecutor(max_workers=3) as ex:
         f64 = ex.submit(O.cgls, "sq", bsub, ang, iters, 0.0, dtype=np.float64)
         f32 = ex.submit(O.cgls, "sq", bsub, ang, iters, 0.0, dtype=np.float32)
+        fdv = ex.submit(O.cgls, "sq", bsub, ang, iters, 0.0, dtype=np.float32, arith="device")
         _, x32, h32 = f32.result()
         log(f"FP32 oracle on rows {r0}..{r0 + nr}: {time.perf_counter() - t1:.1f} s")
+        _, xdv, hdv = fdv.result()
+        log(f"FP32 device-arithmetic oracle on rows {r0}..{r0 + nr}: {time.perf_counter() - t1:.1f} s")
         _, x64, h64 = f64.result()
     t_cpu = time.perf_counter() - t1
     log(f"FP64 oracle on rows {r0}..{r0 + nr}: {t_cpu:.1f} s")
@@ -564,6 +567,11 @@ def cgls_workload(args):
                 "max_rel_relnrm_deviation_vs_fp64": hist_err,
                 "max_rel_relnrm_deviation_fp32_oracle_vs_fp64": float(
                     max(abs(a[2] - c[2]) / c[2] for a, c in zip(h32, h64))),
+                "fp32_device_arithmetic_x_rel_error_vs_fp64": rel(xdv, x64),
+                "max_rel_relnrm_deviation_device_arithmetic_vs_fp64": float(
+                    max(abs(a[2] - c[2]) / c[2] for a, c in zip(hdv, h64))),
+                "device_arithmetic": "the reference's loop order with FP32 weights and fused multiply-add "
+                                     "per tap, as the GPU kernels (their summation order aside)",
                 "oracle_seconds": round(t_cpu, 1),
             },
         },

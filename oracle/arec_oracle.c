@@ -182,7 +182,7 @@ static int orc_column_taps(int method, const int *cord, int nrays, double radius
 }
 
 /* ------------------------------------------------------------------ */
-/* Projectors and solvers, FP32 and FP64 instantiations                */
+/* Projectors and solvers: FP32, FP64 and FP32 device-arithmetic       */
 /* ------------------------------------------------------------------ */
 
 #define REAL float
@@ -194,6 +194,14 @@ static int orc_column_taps(int method, const int *cord, int nrays, double radius
 #define REAL double
 #define SFX(n) n##_d
 #include "arec_oracle_tmpl.inc"
+#undef REAL
+#undef SFX
+
+#define REAL float
+#define SFX(n) n##_g
+#define ORC_DEVICE_ARITH
+#include "arec_oracle_tmpl.inc"
+#undef ORC_DEVICE_ARITH
 #undef REAL
 #undef SFX
 

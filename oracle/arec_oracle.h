@@ -73,6 +73,20 @@ int orc_cgls_d(int method, const double *b, int nx, int ny, int na, const float 
                int maxit, double tol, int warm, double *x, orc_hist *hist, int hist_cap,
                int *nhist);
 
+/* The same projectors and solvers with the GPU library's per-tap arithmetic
+ * (weights rounded to FP32, y = fmaf(w, v, y)) in the reference's loop order:
+ * an accuracy probe, not a restatement of the reference. */
+int orc_project_g(int method, const int *cord, int nrays, double radius, int nx, int ny,
+                  const float *x, const float *angles, int na, float *p);
+int orc_backproject_g(int method, const int *cord, int nrays, double radius, int nx, int ny,
+                      const float *p, const float *angles, int na, float *x);
+int orc_sirt_g(int method, const float *b, int nx, int ny, int na, const float *angles,
+               float lam, int maxit, float tol, float *x, orc_hist *hist, int hist_cap,
+               int *nhist, float *lam_out);
+int orc_cgls_g(int method, const float *b, int nx, int ny, int na, const float *angles,
+               int maxit, float tol, int warm, float *x, orc_hist *hist, int hist_cap,
+               int *nhist);
+
 /* Weighted CGLS, AREC3D/AREC3D/areccgls_stat.cpp:12-731, weights from
  * init_weights (:12-47) with tukey_filter_x_inplace (imagetools.cpp:369-391).
  * Integer radius nx/2 as in the reference: even nx fails with its status. */

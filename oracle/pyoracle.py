@@ -41,7 +41,7 @@ def _bind(path):
     L = C.CDLL(path)
     L.orc_cyl_geometry.argtypes = [C.c_double, C.c_int, C.c_void_p]
     L.orc_cyl_geometry.restype = C.c_int
-    for sfx, arr in (("f", _F), ("d", _D)):
+    for sfx, arr in (("f", _F), ("d", _D), ("g", _F)):
         getattr(L, f"orc_project_{sfx}").argtypes = [C.c_int, _I, C.c_int, C.c_double, C.c_int, C.c_int, arr, _F, C.c_int, arr]
         getattr(L, f"orc_backproject_{sfx}").argtypes = [C.c_int, _I, C.c_int, C.c_double, C.c_int, C.c_int, arr, _F, C.c_int, arr]
     L.orc_sirt_f.argtypes = [C.c_int, _F, C.c_int, C.c_int, C.c_int, _F, C.c_float, C.c_int, C.c_float, _F,
@@ -52,6 +52,8 @@ def _bind(path):
                              C.POINTER(Hist), C.c_int, C.POINTER(C.c_int)]
     L.orc_cgls_d.argtypes = [C.c_int, _D, C.c_int, C.c_int, C.c_int, _F, C.c_int, C.c_double, C.c_int, _D,
                              C.POINTER(Hist), C.c_int, C.POINTER(C.c_int)]
+    L.orc_sirt_g.argtypes = L.orc_sirt_f.argtypes
+    L.orc_cgls_g.argtypes = L.orc_cgls_f.argtypes
     L.orc_cgls_stat_f.argtypes = L.orc_cgls_f.argtypes
     L.orc_cgls_stat_d.argtypes = L.orc_cgls_d.argtypes
     L.orc_init_weights_f.argtypes = [_F, C.c_int, C.c_int, C.c_int, _F]
@@ -138,27 +140,39 @@ def cyl_geometry(radius, height=1):
     return cord.reshape(n, 2)
 
 
-def project(method, x, angles, nx, ny, dtype=np.float32):
+def _sfx(dtype, arith):
+    """Instantiation suffix: f (FP32, the reference's arithmetic), d (FP64) or
+    g (FP32 with the GPU library's per-tap arithmetic: FP32 weights, FMA)."""
+    if arith == "device":
+        if dtype != np.float32:
+            raise ValueError("device arithmetic is FP32")
+        return "g"
+    if arith != "reference":
+        raise ValueError(f"unknown arithmetic {arith!r}")
+    return "f" if dtype == np.float32 else "d"
+
+
+def project(method, x, angles, nx, ny, dtype=np.float32, arith="reference"):
     """x: [nrays, ny] -> p: [na, ny, nx]."""
     radius = 0.5 * (nx - 1)
     cord = np.ascontiguousarray(cyl_geometry(radius).reshape(-1))
     angles = np.ascontiguousarray(angles, np.float32)
     x = np.ascontiguousarray(x, dtype)
     p = np.zeros((len(angles), ny, nx), dtype)
-    f = lib().orc_project_f if dtype == np.float32 else lib().orc_project_d
+    f = getattr(lib(), "orc_project_" + _sfx(dtype, arith))
     st = f(_m(method), cord, len(cord) // 2, radius, nx, ny, x, angles, len(angles), p)
     if st:
         raise RuntimeError(f"orc_project status {st}")
     return p
 
 
-def backproject(method, p, angles, nx, ny, dtype=np.float32):
+def backproject(method, p, angles, nx, ny, dtype=np.float32, arith="reference"):
     radius = 0.5 * (nx - 1)
     cord = np.ascontiguousarray(cyl_geometry(radius).reshape(-1))
     angles = np.ascontiguousarray(angles, np.float32)
     p = np.ascontiguousarray(p, dtype)
     x = np.zeros((len(cord) // 2, ny), dtype)
-    f = lib().orc_backproject_f if dtype == np.float32 else lib().orc_backproject_d
+    f = getattr(lib(), "orc_backproject_" + _sfx(dtype, arith))
     st = f(_m(method), cord, len(cord) // 2, radius, nx, ny, p, angles, len(angles), x)
     if st:
         raise RuntimeError(f"orc_backproject status {st}")
@@ -169,16 +183,17 @@ def _hist(h, n):
     return [(h[i].iter, h[i].event, h[i].relnrm, h[i].rnorm) for i in range(min(n, len(h)))]
 
 
-def sirt(method, b, angles, lam, maxit, tol, dtype=np.float32):
+def sirt(method, b, angles, lam, maxit, tol, dtype=np.float32, arith="reference"):
     """Returns (status, x [nrays, ny], history, final lam)."""
     na, ny, nx = b.shape
+    _sfx(dtype, arith)
     nrays = len(cyl_geometry(0.5 * (nx - 1)))
     x = np.zeros((nrays, ny), dtype)
     H = (Hist * (maxit + 64))()
     nh = C.c_int(0)
     if dtype == np.float32:
         lo = C.c_float(0)
-        st = lib().orc_sirt_f(_m(method), np.ascontiguousarray(b, np.float32), nx, ny, na,
+        st = getattr(lib(), "orc_sirt_" + _sfx(dtype, arith))(_m(method), np.ascontiguousarray(b, np.float32), nx, ny, na,
                               np.ascontiguousarray(angles, np.float32), lam, maxit, tol, x, H, len(H),
                               C.byref(nh), C.byref(lo))
     else:
@@ -189,8 +204,11 @@ def sirt(method, b, angles, lam, maxit, tol, dtype=np.float32):
     return st, x, _hist(H, nh.value), lo.value
 
 
-def cgls(method, b, angles, maxit, tol, x0=None, dtype=np.float32, weighted=False):
+def cgls(method, b, angles, maxit, tol, x0=None, dtype=np.float32, weighted=False, arith="reference"):
     na, ny, nx = b.shape
+    if arith != "reference" and weighted:
+        raise ValueError("the weighted solver has the reference's arithmetic only")
+    _sfx(dtype, arith)
     radius = (nx // 2) if (method in (2, "kb") or weighted) else 0.5 * (nx - 1)
     nrays = len(cyl_geometry(radius))
     x = np.zeros((nrays, ny), dtype) if x0 is None else np.array(x0, dtype, copy=True)
@@ -201,7 +219,7 @@ def cgls(method, b, angles, maxit, tol, x0=None, dtype=np.float32, weighted=Fals
     elif weighted:
         f, bb = lib().orc_cgls_stat_d, np.ascontiguousarray(b, np.float64)
     elif dtype == np.float32:
-        f, bb = lib().orc_cgls_f, np.ascontiguousarray(b, np.float32)
+        f, bb = getattr(lib(), "orc_cgls_" + _sfx(dtype, arith)), np.ascontiguousarray(b, np.float32)
     else:
         f, bb = lib().orc_cgls_d, np.ascontiguousarray(b, np.float64)
     st = f(_m(method), bb, nx, ny, na, np.ascontiguousarray(angles, np.float32), maxit, tol,

@@ -517,3 +517,77 @@ def test_periodic_decomposition_window_and_periodic_input():
     for z in range(2):
         s = O.periodic_smooth(u[z]).astype(np.float64)
         np.testing.assert_allclose(u[z].astype(np.float64) - out[z], w * s, atol=3e-7)
+
+
+def _fmaf(a, b, c):
+    """fmaf(a, b, c) of FP32 inputs, exactly: a*b + c as a rational, rounded
+    once to the nearest FP32 (ties to even)."""
+    from fractions import Fraction
+
+    a, b, c = np.float32(a), np.float32(b), np.float32(c)
+    exact = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    f = np.float32(float(exact))  # within one FP32 step of the exact value
+    lo, hi = (np.nextafter(f, np.float32(-np.inf)), f) if Fraction(float(f)) > exact else \
+        (f, np.nextafter(f, np.float32(np.inf)))
+    dlo, dhi = exact - Fraction(float(lo)), Fraction(float(hi)) - exact
+    if dlo != dhi:
+        return lo if dlo < dhi else hi
+    return lo if (lo.view(np.uint32) & 1) == 0 else hi
+
+
+@pytest.mark.parametrize("method", ["sq", "kb", "linear"])
+def test_device_arithmetic_oracle_is_fp32_weights_with_fma(method):
+    """The device-arithmetic instantiation (an accuracy probe, not a
+    restatement of the reference): an impulse's projection is the FP64
+    projection rounded to FP32 (the weights are the FP64 weights rounded
+    once), and a whole projection is, pixel by pixel, the chain of exact
+    fmaf(w, x, y) over the columns in the reference's order (emulated here
+    with rational arithmetic)."""
+    nx, ny, na = 9, 1, 3
+    ang = np.array([0.3, 1.1, 2.5], np.float32)
+    nr = len(O.cyl_geometry(0.5 * (nx - 1)))
+    W = np.zeros((na, nx, nr), np.float32)  # FP32 weight of column j at (angle, pixel)
+    for j in range(nr):
+        e = np.zeros((nr, ny), np.float32)
+        e[j] = 1.0
+        pg = O.project(method, e, ang, nx, ny, arith="device")
+        pd = O.project(method, e.astype(np.float64), ang, nx, ny, dtype=np.float64)
+        assert np.array_equal(pg, pd.astype(np.float32)), j
+        W[:, :, j] = pg[:, 0, :]
+    rng = np.random.default_rng(17)
+    x = rng.random((nr, ny)).astype(np.float32) + np.float32(0.5)
+    got = O.project(method, x, ang, nx, ny, arith="device")
+    for a in range(na):
+        for u in range(nx):
+            y = np.float32(0.0)
+            for j in range(nr):  # columns in order; a column adds to a pixel once
+                if W[a, u, j] != 0:
+                    y = _fmaf(W[a, u, j], x[j, 0], y)
+            assert got[a, 0, u] == y, (a, u)
+
+
+def test_device_arithmetic_solvers_close_to_fp64():
+    """SIRT and CGLS with the device arithmetic share the reference's code
+    and differ from it only in per-tap rounding.  SIRT stays within FP32
+    rounding of the FP64 solution; CGLS in FP32 drifts from FP64 on this
+    ill-conditioned problem with either arithmetic (5% after 8 iterations
+    with the reference's), and the device arithmetic's drift is of the same
+    size."""
+    A, b, ang, nr = _small_problem(nx=11, ny=2, na=30)
+
+    def rel(x, xd):
+        return np.linalg.norm(x - xd) / np.linalg.norm(xd)
+
+    lam = 1.0 / np.linalg.norm(A, 2) ** 2
+    _, xg, hg, _ = O.sirt("sq", b, ang, lam, 20, 0.0, arith="device")
+    _, xd, hd, _ = O.sirt("sq", b, ang, lam, 20, 0.0, dtype=np.float64)
+    assert len(hg) == len(hd) and rel(xg, xd) <= 1e-5
+    _, xg, hg = O.cgls("sq", b, ang, 8, 0.0, arith="device")
+    _, xf, hf = O.cgls("sq", b, ang, 8, 0.0)
+    _, xd, hd = O.cgls("sq", b, ang, 8, 0.0, dtype=np.float64)
+    assert len(hg) == len(hd) == len(hf)
+    assert rel(xg, xd) <= max(1e-5, 4 * rel(xf, xd))
+    with pytest.raises(ValueError):
+        O.cgls("sq", b, ang, 3, 0.0, weighted=True, arith="device")
+    with pytest.raises(ValueError):
+        O.project("sq", np.zeros((nr, 2)), ang, 11, 2, dtype=np.float64, arith="device")
