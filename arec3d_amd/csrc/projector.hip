@@ -624,6 +624,55 @@ __device__ __forceinline__ void strip_walk(uint4 *ring, const float *__restrict_
     }
 }
 
+// Pixels st*kStripW + q of angle a (rows y0 .. y0 + RPL - 1) from acc[q + 2];
+// the FP64 sum of their FP32 squares into partials (optional).
+template <int RPL>
+__device__ __forceinline__ void strip_store(const float (&acc)[kStripW + 4][RPL], float *__restrict__ p,
+                                            double *__restrict__ partials, const int a, const int st,
+                                            const int chunk, const int nchunks, const int nstrips, const int nx,
+                                            const int ny, const int y0, const int lane) {
+    double sq = 0.0;
+    const int u0 = st * kStripW;
+    float *__restrict__ pa = p + (size_t)a * nx * ny;
+    // a whole strip with 16-byte aligned detector rows: four 16-byte stores
+    // per row (the same values and FP64 sum order as the pixel loop)
+    const bool vec = u0 + kStripW <= nx && (nx & 3) == 0;
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+        const int y = y0 + r;
+        if (y >= ny) continue;
+        if (vec) {
+            float4 *row = reinterpret_cast<float4 *>(pa + (size_t)y * nx + u0);
+#pragma unroll
+            for (int q4 = 0; q4 < kStripW / 4; q4++) {
+                const float v0 = acc[4 * q4 + 2][r], v1 = acc[4 * q4 + 3][r], v2 = acc[4 * q4 + 4][r],
+                            v3 = acc[4 * q4 + 5][r];
+                row[q4] = make_float4(v0, v1, v2, v3);
+                const float s0 = v0 * v0, s1 = v1 * v1, s2 = v2 * v2, s3 = v3 * v3;
+                sq += (double)s0;
+                sq += (double)s1;
+                sq += (double)s2;
+                sq += (double)s3;
+            }
+            continue;
+        }
+#pragma unroll
+        for (int q = 0; q < kStripW; q++) {
+            if (u0 + q < nx) {
+                const float val = acc[q + 2][r];
+                pa[(size_t)y * nx + u0 + q] = val;
+                const float s2 = val * val;
+                sq += (double)s2;
+            }
+        }
+    }
+    if (partials) {
+#pragma unroll
+        for (int off2 = 32; off2 > 0; off2 >>= 1) sq += __shfl_xor(sq, off2, 64);
+        if (lane == 0) partials[((size_t)a * nchunks + chunk) * nstrips + st] = sq;
+    }
+}
+
 // Forward projection, depth-strip design.  Workgroup b: angle group b % nag
 // (kStripAngles waves, one angle each), then detector strip, then row
 // chunk.  Each wave walks its angle's entries of the strip bin by bin in
@@ -665,29 +714,7 @@ __device__ __forceinline__ void forward_strip_body(const float *__restrict__ x, 
     else
         strip_walk<RPL, false>(s_ring[wave], x, t, o, nbins, active, ny, y0, lane, acc);
     if (!active) return;
-    // pixels st*kStripW + q, q < kStripW, are acc[q + 2]
-    double sq = 0.0;
-    const int u0 = st * kStripW;
-    float *__restrict__ pa = p + (size_t)a * nx * ny;
-#pragma unroll
-    for (int r = 0; r < RPL; r++) {
-        const int y = y0 + r;
-        if (y >= ny) continue;
-#pragma unroll
-        for (int q = 0; q < kStripW; q++) {
-            if (u0 + q < nx) {
-                const float val = acc[q + 2][r];
-                pa[(size_t)y * nx + u0 + q] = val;
-                const float s2 = val * val;
-                sq += (double)s2;
-            }
-        }
-    }
-    if (partials) {
-#pragma unroll
-        for (int off2 = 32; off2 > 0; off2 >>= 1) sq += __shfl_xor(sq, off2, 64);
-        if (lane == 0) partials[((size_t)a * nchunks + chunk) * nstrips + st] = sq;
-    }
+    strip_store<RPL>(acc, p, partials, a, st, chunk, nchunks, nstrips, nx, ny, y0, lane);
 }
 
 // At least 5 waves per SIMD (93 VGPRs at 4 rows per lane instead of 108 at 4
@@ -800,37 +827,6 @@ __device__ __forceinline__ void pair_walk(uint4 *ring, const float *__restrict__
             for (; i < bend; i += kStripBatch) pair_batch<RPL, FULL>(R, i, x, ny, y0, lane, acc0, acc1);
         }
         __builtin_amdgcn_s_barrier();
-    }
-}
-
-// Pixels st*kStripW + q of angle a (rows y0 .. y0 + RPL - 1) from acc[q + 2];
-// the FP64 sum of their FP32 squares into partials (optional).
-template <int RPL>
-__device__ __forceinline__ void strip_store(const float (&acc)[kStripW + 4][RPL], float *__restrict__ p,
-                                            double *__restrict__ partials, const int a, const int st,
-                                            const int chunk, const int nchunks, const int nstrips, const int nx,
-                                            const int ny, const int y0, const int lane) {
-    double sq = 0.0;
-    const int u0 = st * kStripW;
-    float *__restrict__ pa = p + (size_t)a * nx * ny;
-#pragma unroll
-    for (int r = 0; r < RPL; r++) {
-        const int y = y0 + r;
-        if (y >= ny) continue;
-#pragma unroll
-        for (int q = 0; q < kStripW; q++) {
-            if (u0 + q < nx) {
-                const float val = acc[q + 2][r];
-                pa[(size_t)y * nx + u0 + q] = val;
-                const float s2 = val * val;
-                sq += (double)s2;
-            }
-        }
-    }
-    if (partials) {
-#pragma unroll
-        for (int off2 = 32; off2 > 0; off2 >>= 1) sq += __shfl_xor(sq, off2, 64);
-        if (lane == 0) partials[((size_t)a * nchunks + chunk) * nstrips + st] = sq;
     }
 }
 
@@ -1318,7 +1314,32 @@ __device__ __forceinline__ void backward_body(const float *__restrict__ p, float
     }
 
     double part = 0.0;
-    if (j >= 0) {
+    if (j >= 0 && ybase + kBpRows <= ny && (ny & 3) == 0) {
+        // whole tile with 16-byte aligned column rows: vector loads and
+        // stores; the same FP32 values and FP64 sum order as below
+        float4 *o = reinterpret_cast<float4 *>(out + (size_t)j * ny + ybase);
+        const float4 *bv = bvec ? reinterpret_cast<const float4 *>(bvec + (size_t)j * ny + ybase) : nullptr;
+#pragma unroll
+        for (int q = 0; q < kBpRows / 4; q++) {
+            float v[4] = {acc[q].x, acc[q].y, acc[q].z, acc[q].w};
+            if (MODE == kBpSirtGrad) {
+                const float4 b4 = bv[q];
+                const float bb[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    v[r] = bb[r] - v[r];
+                    part += (double)v[r] * (double)v[r];
+                }
+            } else if (MODE == kBpStoreNorm) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    float s2 = v[r] * v[r];
+                    part += (double)s2;
+                }
+            }
+            o[q] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    } else if (j >= 0) {
         float *o = out + (size_t)j * ny + ybase;
         const float *bv = bvec ? bvec + (size_t)j * ny + ybase : nullptr;
 #pragma unroll
