@@ -162,7 +162,7 @@ int64_t arec_geom_device_bytes(const arec_geom *geom) {
                                            g.sg_chunk_base.back() * (int64_t)(sizeof(int) * kSgChunk +
                                                                               sizeof(int64_t) * kSgAngles)
                                      : 0);
-    return (int64_t)sizeof(uint4) * g.na * g.nvalid + btab + (int64_t)sizeof(AngleConst) * g.na +
+    return (g.d_fwd ? (int64_t)sizeof(uint4) * g.na * g.nvalid : 0) + btab + (int64_t)sizeof(AngleConst) * g.na +
            (int64_t)sizeof(int) * (g.coord.size() + g.tile_cols.size() + g.tile_org.size() + g.seg.size() +
                                    g.pixstart.size());
 }

@@ -1513,21 +1513,14 @@ void geometry_free_device(Geometry &g) {
     g.device_ready = false;
 }
 
-int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
-    int rc;
-    if ((rc = upload(&g.d_ang, g.ang, st))) return rc;
-    if ((rc = upload(&g.d_coord, g.coord, st))) return rc;
-    if ((rc = upload(&g.d_tile_cols, g.tile_cols, st))) return rc;
-    if ((rc = upload(&g.d_tile_org, g.tile_org, st))) return rc;
-    AREC_HIP(hipMalloc((void **)&g.d_err, sizeof(int)));
-    AREC_HIP(hipMemsetAsync(g.d_err, 0, sizeof(int), st));
+// Design 1's table (the detector sweep's entries of every angle): the order
+// on the host in batches of angles (a stable counting sort by first tap, which
+// also fills pixstart), the weights on the device.  It is released while
+// another design is selected (their tables replace it) and rebuilt, with the
+// same contents, when design 1 is selected again.
+static int build_sweep_table(Geometry &g, hipStream_t st) {
     size_t entries = (size_t)g.na * std::max(1, g.nvalid);
     AREC_HIP(hipMalloc((void **)&g.d_fwd, sizeof(uint4) * entries));
-    geometry_plan_forward(g, 0, 0);
-    g.npix = g.nx + 8;
-    g.pixstart.assign((size_t)g.na * g.npix, 0);
-    // Build the forward table in batches of angles: order on the host (stable
-    // counting sort by first tap), weights on the device.
     int batch = (int)std::max<int64_t>(1, std::min<int64_t>(g.na, (int64_t)(64 << 20) / std::max(1, g.nvalid)));
     std::vector<int> order;
     int *d_order = nullptr;
@@ -1549,6 +1542,45 @@ int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
     }
     AREC_HIP(hipStreamSynchronize(st));
     hipFree(d_order);
+    return 0;
+}
+
+static void release_sweep_table(Geometry &g) {
+    hipFree(g.d_fwd);
+    g.d_fwd = nullptr;
+}
+
+// Design 1's table back after another design's was released (or failed to
+// build); the device's self-check flag reports disagreeing taps.
+static int ensure_sweep_table(Geometry &g, hipStream_t st) {
+    if (g.d_fwd) return 0;
+    AREC_HIP(hipMemsetAsync(g.d_err, 0, sizeof(int), st));
+    int rc = build_sweep_table(g, st);
+    if (rc) {
+        release_sweep_table(g);
+        return rc;
+    }
+    int herr = 0;
+    AREC_HIP(hipMemcpy(&herr, g.d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) {
+        set_error("forward table: device taps disagree with the host ordering");
+        return AREC_EGEOM;
+    }
+    return 0;
+}
+
+int geometry_upload(Geometry &g, hipStream_t st, std::string *err) {
+    int rc;
+    if ((rc = upload(&g.d_ang, g.ang, st))) return rc;
+    if ((rc = upload(&g.d_coord, g.coord, st))) return rc;
+    if ((rc = upload(&g.d_tile_cols, g.tile_cols, st))) return rc;
+    if ((rc = upload(&g.d_tile_org, g.tile_org, st))) return rc;
+    AREC_HIP(hipMalloc((void **)&g.d_err, sizeof(int)));
+    AREC_HIP(hipMemsetAsync(g.d_err, 0, sizeof(int), st));
+    geometry_plan_forward(g, 0, 0);
+    g.npix = g.nx + 8;
+    g.pixstart.assign((size_t)g.na * g.npix, 0);
+    if ((rc = build_sweep_table(g, st))) return rc;
     // AREC_NO_BACK_TABLE forces the on-the-fly taps (same results; tests use it)
     const int64_t btab_entries = (int64_t)g.ntiles * g.na * kTileCols;
     if (g.ntiles > 0 && btab_entries * (int64_t)sizeof(uint4) <= kMaxBackTableBytes &&
@@ -1716,9 +1748,17 @@ static int build_staged_tables(Geometry &g, hipStream_t st) {
     }
     const int ns = g.nstrips, ngroups = (g.na + kSgAngles - 1) / kSgAngles;
     const int64_t nchunk = g.sg_chunk_base.back(), total = g.sg_group_ent.back();
-    if ((total + kStripBatch) * (int64_t)sizeof(uint4) + nchunk * (int64_t)(kSgChunk * 4 + kSgAngles * 8) >
-        kMaxStripTableBytes) {
-        set_error("staged-strip forward table too large for this geometry");
+    // Limited by the device memory free now (design 1's table is released
+    // before this build), leaving room for a solve: a SIRT's work vectors (two
+    // volumes and a projection stack) and 1 GiB.
+    const int64_t bytes =
+        (total + kStripBatch) * (int64_t)sizeof(uint4) + nchunk * (int64_t)(kSgChunk * 4 + kSgAngles * 8);
+    size_t free_b = 0, total_b = 0;
+    AREC_HIP(hipMemGetInfo(&free_b, &total_b));
+    const int64_t reserve = (int64_t)sizeof(float) * (2 * g.nnz + g.nproj) + (1ll << 30);
+    if (bytes + reserve > (int64_t)free_b) {
+        set_error("staged-strip forward table (" + std::to_string(bytes >> 20) + " MiB) does not fit in the " +
+                  std::to_string((int64_t)free_b >> 20) + " MiB of free device memory with room for a solve");
         return AREC_ESIZE;
     }
     int rc;
@@ -1802,15 +1842,32 @@ static void release_strip_tables(Geometry &g) {
     std::vector<int>().swap(g.strip_off);
 }
 
-// Selecting design 1 releases design 2's or 3's table (rebuilt when that
-// design is selected again); switching between designs 2 and 3 rebuilds it.
-// Design 4's tables exist only while it is selected.
+// Designs 2-4 replace design 1's table while selected (it is released before
+// their tables are built, and rebuilt with the same contents when design 1
+// is selected again or their build fails); design 1's segment plan and the
+// back projector's data stay.  Only one of the tables of designs 2-4 exists
+// at a time; switching between designs 2 and 3 rebuilds theirs.
 int geometry_set_forward(Geometry &g, int design, int rpl, int min_groups, hipStream_t st) {
+    if (!g.device_ready) {
+        set_error("forward design: the geometry is not uploaded");
+        return AREC_EINVAL;
+    }
+    if (design < 1 || design > 4) {
+        set_error("forward design must be 1 (sweep), 2 (depth strips), 3 (depth strips, angle pairs) or 4 "
+                  "(staged strips)");
+        return AREC_EINVAL;
+    }
+    if (design == 1) {
+        int rc = geometry_replan_forward(g, rpl, min_groups, st);
+        if (rc) return rc;
+        AREC_HIP(hipStreamSynchronize(st));
+        release_staged_tables(g);
+        release_strip_tables(g);
+        if ((rc = ensure_sweep_table(g, st))) return rc;
+        g.fwd_design = 1;
+        return 0;
+    }
     if (design == 4) {
-        if (!g.device_ready) {
-            set_error("forward design 4 needs an uploaded geometry");
-            return AREC_EINVAL;
-        }
         // any row count: whole row chunks take vector loads only where the
         // rows per lane divide the row count (column starts aligned)
         if (rpl == 0) rpl = g.ny >= 256 ? 4 : g.ny >= 128 ? 2 : 1;
@@ -1818,68 +1875,41 @@ int geometry_set_forward(Geometry &g, int design, int rpl, int min_groups, hipSt
             set_error("forward plan: rows per lane must be 1, 2 or 4");
             return AREC_EINVAL;
         }
-        if (g.d_strip) {
-            AREC_HIP(hipStreamSynchronize(st));
-            release_strip_tables(g);
+    } else {
+        if (rpl == 0)
+            rpl = design == 3 ? ((g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1)
+                              : (g.ny >= 256 && g.ny % 4 == 0) ? 4 : (g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1;
+        if (!fwd_rpl_allowed(g.ny, rpl) || (design == 3 && rpl == 4)) {
+            set_error("forward plan: rows per lane must be 1, 2 or 4 (design 3: 1 or 2) and divide the row count");
+            return AREC_EINVAL;
         }
-        if (!g.d_sg_tab) {
-            int rc = build_staged_tables(g, st);
-            if (rc) {
-                hipStreamSynchronize(st);
-                release_staged_tables(g);
-                g.fwd_design = 1;
-                return rc;
-            }
-        }
-        g.strip_rpl = rpl;
-        g.fwd_design = 4;
-        return 0;
-    }
-    if (g.d_sg_tab) {
-        AREC_HIP(hipStreamSynchronize(st));
-        release_staged_tables(g);
-        if (g.fwd_design == 4) g.fwd_design = 1;
-    }
-    if (design == 1) {
-        int rc = geometry_replan_forward(g, rpl, min_groups, st);
-        if (!rc) {
-            g.fwd_design = 1;
-            if (g.d_strip) {
-                AREC_HIP(hipStreamSynchronize(st));
-                release_strip_tables(g);
-            }
-        }
-        return rc;
-    }
-    if ((design != 2 && design != 3) || !g.device_ready) {
-        set_error("forward design must be 1 (sweep), 2 (depth strips) or 3 (depth strips, angle pairs) on an "
-                  "uploaded geometry");
-        return AREC_EINVAL;
-    }
-    if (rpl == 0)
-        rpl = design == 3 ? ((g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1)
-                          : (g.ny >= 256 && g.ny % 4 == 0) ? 4 : (g.ny >= 128 && g.ny % 2 == 0) ? 2 : 1;
-    if (!fwd_rpl_allowed(g.ny, rpl) || (design == 3 && rpl == 4)) {
-        set_error("forward plan: rows per lane must be 1, 2 or 4 (design 3: 1 or 2) and divide the row count");
-        return AREC_EINVAL;
     }
     const int unit = design == 3 ? 2 : 1;
-    if (g.d_strip && g.strip_unit != unit) {
-        AREC_HIP(hipStreamSynchronize(st));
+    AREC_HIP(hipStreamSynchronize(st));
+    if (design == 4) {
         release_strip_tables(g);
+    } else {
+        release_staged_tables(g);
+        if (g.d_strip && g.strip_unit != unit) release_strip_tables(g);
     }
-    g.strip_unit = unit;
-    if (!g.d_strip) {
-        int rc = build_strip_tables(g, st);
-        if (rc) {
-            hipStreamSynchronize(st);
-            release_strip_tables(g);
-            if (g.fwd_design != 1) g.fwd_design = 1;  // its table is gone; the sweep's tables always exist
-            return rc;
-        }
+    release_sweep_table(g);
+    int rc = 0;
+    if (design == 4 && !g.d_sg_tab) {
+        rc = build_staged_tables(g, st);
+    } else if (design != 4 && !g.d_strip) {
+        g.strip_unit = unit;
+        rc = build_strip_tables(g, st);
     }
-    // design 1's plan keeps its own rows per lane and chunks
-    g.strip_rpl = rpl;
+    if (rc) {
+        hipStreamSynchronize(st);
+        release_staged_tables(g);
+        release_strip_tables(g);
+        g.fwd_design = 1;  // design 1 again: its table comes back
+        const int rc1 = ensure_sweep_table(g, st);
+        if (rc1) return rc1;
+        return rc;
+    }
+    g.strip_rpl = rpl;  // design 1's plan keeps its own rows per lane and chunks
     g.fwd_design = design;
     return 0;
 }
@@ -1887,6 +1917,10 @@ int geometry_set_forward(Geometry &g, int design, int rpl, int min_groups, hipSt
 int launch_forward(const Geometry &g, const float *x, float *p, double *partials, hipStream_t st,
                    const int *skip) {
     if (g.nvalid == 0) return fill(p, 0.0f, g.nproj, st, skip);
+    if (g.fwd_design == 1 && !g.d_fwd) {
+        set_error("forward projection: design 1's table is missing");
+        return AREC_EINVAL;
+    }
     if (g.fwd_design == 4) {
         const int rpl = g.strip_rpl, nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
         const int ngroups = (g.na + kSgAngles - 1) / kSgAngles;

@@ -105,8 +105,11 @@ int arec_geom_get_plan(const arec_geom *geom, int32_t *rows_per_lane, int32_t *m
  * table entry per column of either, rows per lane 1 or 2; table ~1.13 x the
  * sweep table, same limits), 4 = staged strips (groups of 8 angles walk
  * chunks of 32 columns staged in LDS; rows per lane 1, 2 or 4; table about
- * 1.2 x the sweep table, same limits).  Results agree to FP32 summation
- * order.  arec_geom_get_plan reports the selected design's rows per lane. */
+ * 1.2 x the sweep table, refused where it would not leave room in free
+ * device memory for a SIRT's work vectors).  Designs 2-4 replace the sweep
+ * table while selected: it is released and rebuilt when design 1 is
+ * selected again.  Results agree to FP32 summation order.
+ * arec_geom_get_plan reports the selected design's rows per lane. */
 int arec_geom_set_forward(arec_ctx *ctx, arec_geom *geom, int32_t design, int32_t rows_per_lane,
                           int32_t min_groups);
 int32_t arec_geom_forward_design(const arec_geom *geom);
