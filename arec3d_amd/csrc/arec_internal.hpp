@@ -58,6 +58,10 @@ constexpr int kStripBatch = 4;   // depth-strip table entries whose loads are in
 constexpr int kSgAngles = 8;
 constexpr int kSgChunk = 32;
 constexpr int kSgEnt = 48;  // most entries of one (chunk, angle), padding included
+// Entry word of the staged table: the slot's byte offset in an LDS column
+// buffer (slot * 64 * RPL * 4 < 1 << kSgOffShift) plus the first tap's offset
+// d << kSgOffShift.
+constexpr int kSgOffShift = 16;
 // The back projector's tap table is built when it takes at most this much.
 // Back-projector tile shapes (rows per workgroup x angles staged per LDS
 // pass) a geometry may select; shape 0 is the default (12.5 ms at 512^3/720
@@ -138,6 +142,7 @@ struct Geometry {
     int64_t *d_sg_chunk_base = nullptr;
     int64_t *d_sg_off = nullptr;
     uint4 *d_sg_tab = nullptr;
+    int sg_rows4 = 0;  // column stride (bytes) the staged entry words are encoded for
     int bp_shape = 0;  // back-projector tile shape (index into kBpShapeRows)
     bool device_ready = false;
 };

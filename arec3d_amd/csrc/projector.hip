@@ -247,7 +247,7 @@ __global__ void build_pair_table(const uint32_t *__restrict__ order, const int64
 template <int M>
 __global__ void build_staged_table(const uint32_t *__restrict__ col, const uint32_t *__restrict__ angi,
                                    const uint32_t *__restrict__ meta, int64_t n, const int *__restrict__ coord,
-                                   const AngleConst *__restrict__ ang, double radius, int xcent,
+                                   const AngleConst *__restrict__ ang, double radius, int xcent, int rows4,
                                    uint4 *__restrict__ tab, int *__restrict__ err) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t m = meta[e];
@@ -261,7 +261,17 @@ __global__ void build_staged_table(const uint32_t *__restrict__ col, const uint3
                 t.w0 = t.w1 = t.w2 = 0.0f;
             }
         }
-        tab[e] = make_uint4(m & 0xffffu, __float_as_uint(t.w0), __float_as_uint(t.w1), __float_as_uint(t.w2));
+        tab[e] = make_uint4((m & 0xffu) * (uint32_t)rows4 | (uint32_t)d << kSgOffShift, __float_as_uint(t.w0),
+                            __float_as_uint(t.w1), __float_as_uint(t.w2));
+    }
+}
+
+// Re-encode the staged table's entry words for another column stride (rows
+// per lane): slot * from4 + (d << kSgOffShift) -> slot * to4 + (d << ...).
+__global__ void restride_staged_table(uint4 *__restrict__ tab, int64_t n, int from4, int to4) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = tab[e].x, lo = x & ((1u << kSgOffShift) - 1u);
+        tab[e].x = lo / (uint32_t)from4 * (uint32_t)to4 | (x & ~((1u << kSgOffShift) - 1u));
     }
 }
 
@@ -922,6 +932,101 @@ __device__ __forceinline__ void strip_add_single(float (&acc)[kStripW + 4][RPL],
     }
 }
 
+// Lane l gets lane (l & ~3) + k's value (quad_perm [k,k,k,k]); the compiler
+// folds it into the consuming add as a DPP operand.
+// A wave-uniform pointer to global memory, read back from the first lane
+// into scalar registers and typed as global: an address built from it plus
+// a lane offset uses the scalar-base form of the load, and a pointer read
+// from a struct in memory (generic to the compiler, so accessed with flat
+// instructions, which also count against the LDS wait counter) is accessed
+// with global ones.  For uniform pointers only.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *scalar_ptr(T *p) {
+    const uint64_t u = (uint64_t)p;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return (__attribute__((address_space(1))) T *)(((uint64_t)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T *as_global(T *p) {
+    return (T *)scalar_ptr(p);
+}
+
+template <int K>
+__device__ __forceinline__ unsigned quad_bcast_u32(const unsigned v) {
+    static_assert(K >= 0 && K < 4, "quad lane");
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, K * 0x55, 0xf, 0xf, false);
+}
+
+// a_t[r] += w_t(entry K) * v[r] for the three taps t: the weights are lane
+// (l & ~3) + K's, read through a quad_perm DPP operand of v_fmac_f32 (the
+// compiler keeps such a broadcast as a separate v_mov_b32_dpp).  A DPP
+// operand must not be written by a VALU instruction in the two preceding
+// cycles (the weights come from an LDS read; s_nop 1 covers any copy).
+#define AREC_QP(K) " quad_perm:[" #K "," #K "," #K "," #K "] row_mask:0xf bank_mask:0xf"
+#define AREC_FMA3(K, R)                                                                                   \
+    "v_fmac_f32_dpp %[a0" #R "], %[w0], %[v" #R "]" AREC_QP(K) "\n\t"                                    \
+    "v_fmac_f32_dpp %[a1" #R "], %[w1], %[v" #R "]" AREC_QP(K) "\n\t"                                    \
+    "v_fmac_f32_dpp %[a2" #R "], %[w2], %[v" #R "]" AREC_QP(K) "\n\t"
+template <int K, int RPL>
+__device__ __forceinline__ void fma3_quad(float (&a0)[RPL], float (&a1)[RPL], float (&a2)[RPL], const float w0,
+                                          const float w1, const float w2, const float (&v)[RPL]) {
+    static_assert(K >= 0 && K < 4 && (RPL == 1 || RPL == 2 || RPL == 4), "quad entry and rows per lane");
+#define AREC_FMA_ASM(K)                                                                                   \
+    if constexpr (RPL == 1) {                                                                             \
+        asm volatile("s_nop 1\n\t" AREC_FMA3(K, 0)                                                        \
+                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0]), [a20] "+v"(a2[0])                            \
+                     : [w0] "v"(w0), [w1] "v"(w1), [w2] "v"(w2), [v0] "v"(v[0]));                        \
+    } else if constexpr (RPL == 2) {                                                                      \
+        asm volatile("s_nop 1\n\t" AREC_FMA3(K, 0) AREC_FMA3(K, 1)                                        \
+                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0]), [a20] "+v"(a2[0]), [a01] "+v"(a0[RPL > 1]),  \
+                       [a11] "+v"(a1[RPL > 1]), [a21] "+v"(a2[RPL > 1])                                   \
+                     : [w0] "v"(w0), [w1] "v"(w1), [w2] "v"(w2), [v0] "v"(v[0]), [v1] "v"(v[RPL > 1]));  \
+    } else {                                                                                              \
+        constexpr int R1 = RPL > 1 ? 1 : 0, R2 = RPL > 2 ? 2 : 0, R3 = RPL > 3 ? 3 : 0;                   \
+        asm volatile("s_nop 1\n\t" AREC_FMA3(K, 0) AREC_FMA3(K, 1) AREC_FMA3(K, 2) AREC_FMA3(K, 3)          \
+                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0]), [a20] "+v"(a2[0]), [a01] "+v"(a0[R1]),       \
+                       [a11] "+v"(a1[R1]), [a21] "+v"(a2[R1]), [a02] "+v"(a0[R2]), [a12] "+v"(a1[R2]),    \
+                       [a22] "+v"(a2[R2]), [a03] "+v"(a0[R3]), [a13] "+v"(a1[R3]), [a23] "+v"(a2[R3])     \
+                     : [w0] "v"(w0), [w1] "v"(w1), [w2] "v"(w2), [v0] "v"(v[0]), [v1] "v"(v[R1]),         \
+                       [v2] "v"(v[R2]), [v3] "v"(v[R3]));                                                 \
+    }
+    if constexpr (K == 0) {
+        AREC_FMA_ASM(0)
+    } else if constexpr (K == 1) {
+        AREC_FMA_ASM(1)
+    } else if constexpr (K == 2) {
+        AREC_FMA_ASM(2)
+    } else {
+        AREC_FMA_ASM(3)
+    }
+#undef AREC_FMA_ASM
+}
+#undef AREC_FMA3
+#undef AREC_QP
+
+// acc[D..D+2] += w(entry k) * v[k] for the batch's four entries (lane group
+// l & ~3 holds them), for the one offset D whose bit is set in present;
+// entry by entry, so every accumulator sums in entry order, as strip_add.
+template <int LO, int HI, int RPL>
+__device__ __forceinline__ void strip_add_quad(float (&acc)[kStripW + 4][RPL], const unsigned present,
+                                               const float w0, const float w1, const float w2,
+                                               const float (&v)[kStripBatch][RPL]) {
+    static_assert(kStripBatch == 4, "one entry per lane of a quad");
+    if constexpr (HI - LO == 1) {
+        fma3_quad<0, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[0]);
+        fma3_quad<1, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[1]);
+        fma3_quad<2, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[2]);
+        fma3_quad<3, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[3]);
+    } else {
+        constexpr int MID = (LO + HI) / 2;
+        constexpr unsigned ML = ((1u << MID) - 1u) & ~((1u << LO) - 1u);
+        constexpr unsigned MH = ((1u << HI) - 1u) & ~((1u << MID) - 1u);
+        if (present & ML) strip_add_quad<LO, MID, RPL>(acc, present, w0, w1, w2, v);
+        if (present & MH) strip_add_quad<MID, HI, RPL>(acc, present, w0, w1, w2, v);
+    }
+}
+
 // The walk over chunks [K0, K1) of one (group, strip) for the wave's rows;
 // FULL: all 64*RPL rows lie inside the volume and the row count is a
 // multiple of RPL (so every lane's RPL rows of a column are aligned for one
@@ -938,24 +1043,55 @@ __device__ __forceinline__ void strip_add_single(float (&acc)[kStripW + 4][RPL],
 // entries from LDS (the same address in every lane), takes its offset from
 // the first, and adds each entry's column rows with the entry's weights.
 template <int RPL, bool FULL>
-__device__ __forceinline__ void staged_walk(float (*s_col)[kSgChunk][64 * RPL], uint4 (*s_ent)[kSgAngles][kSgEnt],
+__device__ __forceinline__ void staged_walk(float (*__restrict__ col0)[64 * RPL], float (*__restrict__ col1)[64 * RPL],
+                                            uint4 *__restrict__ ent0, uint4 *__restrict__ ent1,
                                             const float *__restrict__ x, const uint4 *__restrict__ tab,
                                             const int *__restrict__ stage, const int64_t *__restrict__ off,
                                             const int64_t K0, const int64_t K1, const int wave, const int lane,
                                             const int ny, const int y0, float (&acc)[kStripW + 4][RPL]) {
     constexpr int kPer = kSgChunk / kSgAngles;  // columns each wave stages per chunk
+    static_assert((kPer & (kPer - 1)) == 0 && kPer <= 64, "slot lanes");
     static_assert(kPer * kSgAngles == kSgChunk, "chunk staging pattern");
     static_assert(kSgEnt <= 64 && kSgEnt % kStripBatch == 0, "one entry per lane, whole batches");
     constexpr bool kDma = FULL && RPL == 4;
     float stg[kDma ? 1 : kPer][RPL];
     int nent_next;  // its entry count (a multiple of kStripBatch)
-    auto load_stage = [&](const int64_t K, const int buf) __attribute__((always_inline)) {
+    // Chunk K's stage-list slots and entry bounds: lane c < kPer holds the
+    // index of the column this wave stages into slot wave + c * kSgAngles,
+    // lanes 0 and 1 the entry range.  Below 4 rows per lane they are fetched
+    // one chunk before the columns, so that they have landed when the
+    // columns and entries are requested (no dependent round trip); at 4 rows
+    // per lane the registers to hold them ahead are not there (they spill,
+    // and a spill reload waits for the direct loads in flight), so they are
+    // fetched just before.
+    constexpr bool kAhead = RPL < 4;
+    int jv = 0;
+    int64_t bv = 0;
+    auto fetch_idx = [&](const int64_t K) __attribute__((always_inline)) {
+        // scalar bases plus small lane offsets: no 64-bit lane pointers
+        // kept across the chunk loop (at 4 rows per lane they spill)
+        const auto sb = scalar_ptr(stage + (K * kSgChunk + wave));
+        const auto ob = scalar_ptr(off + (K * kSgAngles + wave));
+        jv = sb[(unsigned)(lane & (kPer - 1)) * kSgAngles];
+        bv = ob[(unsigned)(lane & 1)];
+    };
+    auto load_stage = [&](float (*__restrict__ cb)[64 * RPL], uint4 *__restrict__ eb) __attribute__((always_inline)) {
+        int64_t e0 = 0;
+        auto entry_range = [&]() __attribute__((always_inline)) {
+            e0 = (int64_t)(unsigned)__builtin_amdgcn_readlane((int)bv, 0) |
+                 ((int64_t)__builtin_amdgcn_readlane((int)(bv >> 32), 0) << 32);
+            nent_next = __builtin_amdgcn_readlane((int)bv, 1) - (int)e0;
+        };
+        // fetched just before (4 rows per lane): the entry range first, so
+        // that waiting for it waits for no column load
+        if constexpr (!kAhead) entry_range();
 #pragma unroll
         for (int c = 0; c < kPer; c++) {
-            const int j = max(0, stage[K * kSgChunk + wave + c * kSgAngles]);
-            const float *__restrict__ xj = x + (size_t)j * ny + y0;
+            const int j = max(0, __builtin_amdgcn_readlane(jv, c));
+            // the column's scalar base plus the lane's row offset
+            const float *__restrict__ xj = (const float *)scalar_ptr(x + (size_t)j * ny) + (unsigned)y0;
             if constexpr (kDma) {
-                __builtin_amdgcn_global_load_lds(xj, &s_col[buf][wave + c * kSgAngles][0], 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(xj, &cb[wave + c * kSgAngles][0], 16, 0, 0);
             } else if constexpr (FULL && RPL == 2) {
                 const float2 q = *reinterpret_cast<const float2 *>(xj);
                 stg[c][0] = q.x;
@@ -967,19 +1103,14 @@ __device__ __forceinline__ void staged_walk(float (*s_col)[kSgChunk][64 * RPL], 
                 for (int r = 0; r < RPL; r++) stg[c][r] = y0 + r < ny ? xj[r] : 0.0f;
             }
         }
-        // this wave's entry range of chunk K: lanes 0 and 1 fetch its bounds
-        const int64_t bnd = off[K * kSgAngles + wave + (lane & 1)];
-        const int64_t e0 = (int64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)bnd) |
-                           ((int64_t)__builtin_amdgcn_readfirstlane((int)(bnd >> 32)) << 32);
-        const int64_t e1l = __shfl(bnd, 1, 64);
-        nent_next = __builtin_amdgcn_readfirstlane((int)(e1l - e0));
-        if (lane < nent_next) __builtin_amdgcn_global_load_lds((void *)(tab + e0 + lane), &s_ent[buf][wave][0], 16, 0, 0);
+        if constexpr (kAhead) entry_range();
+        if (lane < nent_next) __builtin_amdgcn_global_load_lds((void *)(tab + e0 + lane), eb, 16, 0, 0);
     };
-    auto store_stage = [&](const int buf) __attribute__((always_inline)) {
+    auto store_stage = [&](float (*__restrict__ cb)[64 * RPL]) __attribute__((always_inline)) {
         if constexpr (!kDma) {
 #pragma unroll
             for (int c = 0; c < kPer; c++) {
-                float *d = &s_col[buf][wave + c * kSgAngles][lane * RPL];
+                float *d = &cb[wave + c * kSgAngles][lane * RPL];
                 if constexpr (RPL == 4) {
                     *reinterpret_cast<float4 *>(d) = make_float4(stg[c][0], stg[c][RPL > 1 ? 1 : 0],
                                                                  stg[c][RPL > 2 ? 2 : 0], stg[c][RPL > 3 ? 3 : 0]);
@@ -993,60 +1124,75 @@ __device__ __forceinline__ void staged_walk(float (*s_col)[kSgChunk][64 * RPL], 
         __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the direct loads land before the barrier
     };
 
+    // chunk K from buffers (cb, eb) while chunk K + 1 goes to (nb, ne).  The
+    // two buffers are separate restrict pointers and alternate statically
+    // (the chunk loop is unrolled by two), so the compiler can tell that the
+    // reads of one do not wait for the direct loads into the other.
     int nent = 0;
-    if (K0 < K1) {
-        load_stage(K0, 0);
-        store_stage(0);
-        nent = nent_next;
-    }
-    __syncthreads();
-    for (int64_t K = K0; K < K1; K++) {
-        const int buf = (int)((K - K0) & 1);
+    auto step = [&](const int64_t K, const float (*__restrict__ cb)[64 * RPL], const uint4 *__restrict__ eb,
+                    float (*__restrict__ nb)[64 * RPL], uint4 *__restrict__ ne) __attribute__((always_inline)) {
         const bool more = K + 1 < K1;
-        if (more) load_stage(K + 1, buf ^ 1);
-        const uint4 *E = s_ent[buf][wave];
+        if (more) {
+            if (!kAhead) fetch_idx(K + 1);
+            load_stage(nb, ne);
+            if (kAhead && K + 2 < K1) fetch_idx(K + 2);
+        }
+        // lane l's byte address of its rows in slot 0 of this buffer
+        const char *lb = reinterpret_cast<const char *>(&cb[0][lane * RPL]);
         for (int i = 0; i < nent; i += kStripBatch) {
-            uint4 ec[kStripBatch];
-#pragma unroll
-            for (int k = 0; k < kStripBatch; k++) ec[k] = E[i + k];
+            // lane l holds entry i + l % 4; entry k reaches every lane of a
+            // quad through a quad_perm [k,k,k,k] operand
+            const uint4 q = eb[i + (lane & (kStripBatch - 1))];
+            const unsigned xd = (unsigned)__builtin_amdgcn_readfirstlane((int)q.x);
+            const unsigned d = (xd >> kSgOffShift) & 31u;
+            const char *lbd = lb - (d << kSgOffShift);  // entry words hold slot offset + (d << kSgOffShift)
             float v[kStripBatch][RPL];
-#pragma unroll
-            for (int k = 0; k < kStripBatch; k++) {
-                const float *src = &s_col[buf][ec[k].x & (kSgChunk - 1)][lane * RPL];
+            auto col_rows = [&](const float *src, float(&vk)[RPL]) __attribute__((always_inline)) {
                 if constexpr (RPL == 4) {
-                    const float4 q = *reinterpret_cast<const float4 *>(src);
-                    v[k][0] = q.x;
-                    v[k][RPL > 1 ? 1 : 0] = q.y;
-                    v[k][RPL > 2 ? 2 : 0] = q.z;
-                    v[k][RPL > 3 ? 3 : 0] = q.w;
+                    const float4 c = *reinterpret_cast<const float4 *>(src);
+                    vk[0] = c.x;
+                    vk[RPL > 1 ? 1 : 0] = c.y;
+                    vk[RPL > 2 ? 2 : 0] = c.z;
+                    vk[RPL > 3 ? 3 : 0] = c.w;
                 } else if constexpr (RPL == 2) {
-                    const float2 q = *reinterpret_cast<const float2 *>(src);
-                    v[k][0] = q.x;
-                    v[k][RPL > 1 ? 1 : 0] = q.y;
+                    const float2 c = *reinterpret_cast<const float2 *>(src);
+                    vk[0] = c.x;
+                    vk[RPL > 1 ? 1 : 0] = c.y;
                 } else {
-                    v[k][0] = src[0];
+                    vk[0] = src[0];
                 }
-            }
-            float w[kStripBatch][3];
-#pragma unroll
-            for (int k = 0; k < kStripBatch; k++) {
-                w[k][0] = __uint_as_float(ec[k].y);
-                w[k][1] = __uint_as_float(ec[k].z);
-                w[k][2] = __uint_as_float(ec[k].w);
-            }
-            const unsigned d = ((unsigned)__builtin_amdgcn_readfirstlane(ec[0].x) >> 8) & 31u;
+            };
+            static_assert(kStripBatch == 4, "one entry per lane of a quad");
+            col_rows(reinterpret_cast<const float *>(lbd + quad_bcast_u32<0>(q.x)), v[0]);
+            col_rows(reinterpret_cast<const float *>(lbd + quad_bcast_u32<1>(q.x)), v[1]);
+            col_rows(reinterpret_cast<const float *>(lbd + quad_bcast_u32<2>(q.x)), v[2]);
+            col_rows(reinterpret_cast<const float *>(lbd + quad_bcast_u32<3>(q.x)), v[3]);
             unsigned present = 1u << d;
             // opaque to the optimiser: knowing it is one bit, it turns the
             // bit tests into a switch whose leaves copy every accumulator
             // (at 1 row per lane past 128 VGPRs, into scratch)
             asm volatile("" : "+s"(present));
-            strip_add_single<0, kStripGroups, RPL>(acc, present, w, v);
+            strip_add_quad<0, kStripGroups, RPL>(acc, present, __uint_as_float(q.y), __uint_as_float(q.z),
+                                                 __uint_as_float(q.w), v);
         }
         if (more) {
-            store_stage(buf ^ 1);
+            store_stage(nb);
             nent = nent_next;
         }
         __syncthreads();
+    };
+
+    if (K0 < K1) {
+        fetch_idx(K0);
+        load_stage(col0, ent0);
+        if (kAhead && K0 + 1 < K1) fetch_idx(K0 + 1);
+        store_stage(col0);
+        nent = nent_next;
+    }
+    __syncthreads();
+    for (int64_t K = K0; K < K1; K += 2) {
+        step(K, col0, ent0, col1, ent1);
+        if (K + 1 < K1) step(K + 1, col1, ent1, col0, ent0);
     }
 }
 
@@ -1076,9 +1222,11 @@ __device__ __forceinline__ void forward_staged_body(const float *__restrict__ x,
         for (int r = 0; r < RPL; r++) acc[i][r] = 0.0f;
     // whole chunk with every column's rows aligned to RPL floats: vector loads
     if (ybase + ROWS <= ny && (ny & (RPL - 1)) == 0)
-        staged_walk<RPL, true>(s_col, s_ent, x, tab, stage, off, K0, K1, wave, lane, ny, y0, acc);
+        staged_walk<RPL, true>(s_col[0], s_col[1], s_ent[0][wave], s_ent[1][wave], x, tab, stage, off, K0, K1, wave,
+                               lane, ny, y0, acc);
     else
-        staged_walk<RPL, false>(s_col, s_ent, x, tab, stage, off, K0, K1, wave, lane, ny, y0, acc);
+        staged_walk<RPL, false>(s_col[0], s_col[1], s_ent[0][wave], s_ent[1][wave], x, tab, stage, off, K0, K1, wave,
+                                lane, ny, y0, acc);
     if (!active) return;
     strip_store<RPL>(acc, p, partials, a, st, chunk, nchunks, nstrips, nx, ny, y0, lane);
 }
@@ -1087,7 +1235,7 @@ __device__ __forceinline__ void forward_staged_body(const float *__restrict__ x,
 // multiple of 8 workgroups each XCD takes a contiguous range of them
 // (neighbouring angle groups, which read the same columns).
 template <int RPL>
-__global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void forward_staged_kernel(
+__global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(RPL == 2 ? 6 : 4, 8))) void forward_staged_kernel(
     const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ tab, const int *__restrict__ stage,
     const int64_t *__restrict__ chunk_base, const int64_t *__restrict__ off, double *__restrict__ partials,
     int nstrips, int nx, int ny, int na, int nchunks, const int *__restrict__ skip) {
@@ -1110,12 +1258,13 @@ __global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(RPL 
     const BatchFwd *__restrict__ probs, const int *__restrict__ owner) {
     const int k = owner[blockIdx.x >> 3];
     const BatchFwd q = probs[k];
-    if (q.skip && *q.skip) return;
+    if (q.skip && *as_global(q.skip)) return;
     const int l = (int)((int64_t)blockIdx.x - 8 * q.blk8);
     const int m = ((q.nblk + 7) >> 3);  // workgroups per XCD
     const int b = (l & 7) * m + (l >> 3);
     if (b >= q.nblk) return;
-    forward_staged_body<RPL>(q.x, q.p, q.tab, q.stage, q.chunk_base, q.off, nullptr, q.nstrips, q.nx, q.ny, q.na,
+    forward_staged_body<RPL>(as_global(q.x), as_global(q.p), as_global(q.tab), as_global(q.stage),
+                             as_global(q.chunk_base), as_global(q.off), nullptr, q.nstrips, q.nx, q.ny, q.na,
                              q.nchunks, b);
 }
 
@@ -1396,7 +1545,7 @@ __global__ __launch_bounds__(kBpThreads) void backward_kernel(BpArgs B, const in
 // (one back-projection workgroup's job of one problem) from a queue until it
 // is empty.  *counter must be 0 at launch.
 template <int M, int MODE, int S>
-__global__ __launch_bounds__(kBpThreads) void backward_batch_kernel(const BatchBp *__restrict__ probs, int nprob,
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(S == 2 ? 2 : 4, 8))) void backward_batch_kernel(const BatchBp *__restrict__ probs, int nprob,
                                                                    int total, int *__restrict__ counter) {
     __shared__ int s_item;
     __shared__ int s_item0[kBatchLdsProbs];
@@ -1409,8 +1558,19 @@ __global__ __launch_bounds__(kBpThreads) void backward_batch_kernel(const BatchB
         if (it >= total) break;
         const int k = __builtin_amdgcn_readfirstlane(owner_of(probs, s_item0, nprob, it));
         const int *skip = probs[k].skip;
-        if (skip && *skip) continue;
-        backward_item<M, MODE, S>(probs[k].a, it - probs[k].item0);
+        if (skip && *as_global(skip)) continue;
+        BpArgs a = probs[k].a;
+        a.p = as_global(a.p);
+        a.out = as_global(a.out);
+        a.bvec = as_global(a.bvec);
+        a.btab = as_global(a.btab);
+        a.tile_cols = as_global(a.tile_cols);
+        a.tile_org = as_global(a.tile_org);
+        a.coord = as_global(a.coord);
+        a.ang = as_global(a.ang);
+        a.partials = as_global(a.partials);
+        a.err = as_global(a.err);
+        backward_item<M, MODE, S>(a, it - probs[k].item0);
     }
 }
 
@@ -1737,7 +1897,7 @@ static int build_pair_tables(Geometry &g, hipStream_t st) {
 
 // Design 4's tables: the staged ordering on the host, in batches of angle
 // groups of at most 32 M entries, and the weights on the device.
-static int build_staged_tables(Geometry &g, hipStream_t st) {
+static int build_staged_tables(Geometry &g, const int rpl, hipStream_t st) {
     if (g.na > 65535 || (g.nx + kStripW - 1) / kStripW > 65535) {
         set_error("staged-strip forward table: at most 65535 angles and strips");
         return AREC_ESIZE;
@@ -1795,11 +1955,11 @@ static int build_staged_tables(Geometry &g, hipStream_t st) {
             AREC_HIP(hipMemcpyAsync(d_tmp + 2 * cap, meta.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
             dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 65536));
             if (g.method == kSQ)
-                build_staged_table<kSQ><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_sg_tab + E0, g.d_err);
+                build_staged_table<kSQ><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
             else if (g.method == kKB)
-                build_staged_table<kKB><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_sg_tab + E0, g.d_err);
+                build_staged_table<kKB><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
             else
-                build_staged_table<kLinear><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, g.d_sg_tab + E0, g.d_err);
+                build_staged_table<kLinear><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
             AREC_HIP(hipGetLastError());
         }
         G0 = G1;
@@ -1826,6 +1986,7 @@ static void release_staged_tables(Geometry &g) {
     g.d_sg_stage = nullptr;
     g.d_sg_off = nullptr;
     g.d_sg_chunk_base = nullptr;
+    g.sg_rows4 = 0;
     std::vector<int64_t>().swap(g.sg_chunk_base);
     std::vector<int64_t>().swap(g.sg_group_ent);
 }
@@ -1895,7 +2056,15 @@ int geometry_set_forward(Geometry &g, int design, int rpl, int min_groups, hipSt
     release_sweep_table(g);
     int rc = 0;
     if (design == 4 && !g.d_sg_tab) {
-        rc = build_staged_tables(g, st);
+        rc = build_staged_tables(g, rpl, st);
+        if (!rc) g.sg_rows4 = 256 * rpl;
+    } else if (design == 4 && g.sg_rows4 != 256 * rpl) {
+        // entry words hold the slot's byte offset for the current rows per lane
+        const int64_t n = g.sg_group_ent.back() + kStripBatch;
+        dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 65536));
+        restride_staged_table<<<grid, 256, 0, st>>>(g.d_sg_tab, n, g.sg_rows4, 256 * rpl);
+        AREC_HIP(hipGetLastError());
+        g.sg_rows4 = 256 * rpl;
     } else if (design != 4 && !g.d_strip) {
         g.strip_unit = unit;
         rc = build_strip_tables(g, st);
