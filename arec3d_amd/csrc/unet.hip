@@ -188,6 +188,222 @@ __global__ __launch_bounds__(kCvThreads) void conv3_mfma_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// The full-resolution 16-output convolutions (16 skip channels, plus 32
+// nearest-upsampled channels in the decoder), with the input staged in LDS.
+// The generic kernel above loads every A operand from global memory: each
+// input value is fetched for 27 taps x several workgroups through the
+// vector memory path, which bounds it far below the MFMA rate.  Here every
+// A operand is read from LDS, where the input tile was copied with 16-byte
+// loads (zeros outside the volume).  A staged slice is 6 H rows x 66 W
+// positions (64 + halo) of the 16 skip channels, and 4 x 34 half-resolution
+// positions of the 32 upsampled ones.  Channel quads are the slower LDS
+// index, so the 16 lanes of a quad read 16 consecutive 8-byte words.  The
+// MFMAs of an output run in the order (tap row, dw tap, channel block).  The
+// upsampled part needs even D, H and W: then every position outside the
+// volume maps to a half-resolution position outside it too, staged as zero.
+// ---------------------------------------------------------------------------
+constexpr int kC16Rows = 4;                 // H rows per workgroup (conv_rows(16))
+constexpr int kC16Pos = 66;                 // staged W positions: 64 + halo
+constexpr int kC16Hs = kC16Rows + 2;        // staged H rows
+constexpr int kC16Pos2 = 34;                // half-resolution W positions
+constexpr int kC16Hs2 = kC16Rows / 2 + 2;   // half-resolution H rows
+
+// The kernel walks D: a workgroup owns 64 positions along W x 4 H rows and
+// a run of kC16Zrun consecutive D slices.  Its LDS holds a ring of three
+// staged input slices (the 16 skip channels) and, with upsampled channels, a
+// ring of two half-resolution slices; per output slice one new input slice
+// (and every other slice a new half-resolution one) is loaded -- issued into
+// registers before the slice's MFMAs and stored into the ring after them --
+// so each input slice row is fetched once per run instead of three times,
+// and the fetch overlaps the arithmetic.  Without upsampled channels the 27
+// taps' weights stay in registers for the whole run.
+constexpr int kC16Zrun = 16;
+
+template <int C2>
+__global__ __launch_bounds__(kCvThreads) void conv3_c16z_kernel(const bf16_t *__restrict__ in1,
+                                                                  const bf16_t *__restrict__ in2,
+                                                                  const bf16_t *__restrict__ w,
+                                                                  const float *__restrict__ bias,
+                                                                  bf16_t *__restrict__ out, Dims d, int relu) {
+    static_assert(C2 == 0 || C2 == 32, "upsampled channels: none or 32");
+    constexpr int Ct = 16 + C2;
+    constexpr int kSlice = kC16Hs * kC16Pos * 2;                  // 16-byte halves per staged slice
+    constexpr int kPer = (kSlice + kCvThreads - 1) / kCvThreads;  // per thread
+    constexpr int kSlice2 = kC16Hs2 * kC16Pos2 * (C2 / 8);        // 16-byte quarters per half-resolution slice
+    constexpr int kPer2 = C2 ? (kSlice2 + kCvThreads - 1) / kCvThreads : 1;
+    __shared__ uint2 s1[3][kC16Hs][4][kC16Pos];
+    // half-resolution ring (the upsampled channels): slot of half slice zh is zh & 1
+    __shared__ uint2 s2[C2 ? 2 : 1][kC16Hs2][C2 ? C2 / 4 : 1][kC16Pos2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wtiles = (d.W + 63) / 64, hblocks = (d.H + kC16Rows - 1) / kC16Rows;
+    const int wt = blockIdx.x % wtiles;
+    const int h0 = ((blockIdx.x / wtiles) % hblocks) * kC16Rows;
+    const int zA = (blockIdx.x / (wtiles * hblocks)) * kC16Zrun, zE = min(zA + kC16Zrun, d.D);
+    const int w0 = wt * 64;
+    const int p = lane & 15, q = lane >> 4;
+
+    uint4 pre[kPer];
+    auto fetch = [&](const int zz) __attribute__((always_inline)) {  // slice zz into registers
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int idx = tid + k * kCvThreads;
+            const int half = idx & 1, pi = idx >> 1;
+            const int ws = pi % kC16Pos, hs = pi / kC16Pos;
+            const int hh = h0 - 1 + hs, ww = w0 - 1 + ws;
+            pre[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (idx < kSlice && zz >= 0 && zz < d.D && hh >= 0 && hh < d.H && ww >= 0 && ww < d.W)
+                pre[k] = *reinterpret_cast<const uint4 *>(in1 + (((size_t)zz * d.H + hh) * d.W + ww) * 16 + half * 8);
+        }
+    };
+    auto put = [&](const int slot) __attribute__((always_inline)) {  // registers into ring slot
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int idx = tid + k * kCvThreads;
+            if (idx < kSlice) {
+                const int half = idx & 1, pi = idx >> 1;
+                const int ws = pi % kC16Pos, hs = pi / kC16Pos;
+                s1[slot][hs][2 * half][ws] = make_uint2(pre[k].x, pre[k].y);
+                s1[slot][hs][2 * half + 1][ws] = make_uint2(pre[k].z, pre[k].w);
+            }
+        }
+    };
+    // half-resolution slice zh (upsampled channels, 8 per 16-byte quarter)
+    const int D2 = d.D >> 1, H2 = d.H >> 1, W2 = d.W >> 1;
+    const int hb2 = (h0 - 1) >> 1, wb2 = (w0 - 1) >> 1;  // arithmetic shifts
+    uint4 pre2[kPer2];
+    auto fetch2 = [&](const int zh) __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < kPer2; k++) {
+            const int idx = tid + k * kCvThreads;
+            const int qq = idx % (C2 / 8 ? C2 / 8 : 1), pi = idx / (C2 / 8 ? C2 / 8 : 1);
+            const int ws = pi % kC16Pos2, hs = pi / kC16Pos2;
+            const int hh = hb2 + hs, ww = wb2 + ws;
+            pre2[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (C2 && idx < kSlice2 && zh >= 0 && zh < D2 && hh >= 0 && hh < H2 && ww >= 0 && ww < W2)
+                pre2[k] = *reinterpret_cast<const uint4 *>(in2 + (((size_t)zh * H2 + hh) * W2 + ww) * C2 + qq * 8);
+        }
+    };
+    auto put2 = [&](const int zh) __attribute__((always_inline)) {
+        if constexpr (C2 > 0) {
+            const int slot = zh & 1;
+#pragma unroll
+            for (int k = 0; k < kPer2; k++) {
+                const int idx = tid + k * kCvThreads;
+                if (idx < kSlice2) {
+                    const int qq = idx % (C2 / 8), pi = idx / (C2 / 8);
+                    const int ws = pi % kC16Pos2, hs = pi / kC16Pos2;
+                    s2[slot][hs][2 * qq][ws] = make_uint2(pre2[k].x, pre2[k].y);
+                    s2[slot][hs][2 * qq + 1][ws] = make_uint2(pre2[k].z, pre2[k].w);
+                }
+            }
+        }
+    };
+    // C2 = 0: the 27 taps' B operands (channel quad q of output channel p)
+    // stay in registers; with upsampled channels they are loaded per tap
+    s4v b[C2 ? 1 : 27];
+    if constexpr (C2 == 0) {
+#pragma unroll
+        for (int tap = 0; tap < 27; tap++) b[tap] = *reinterpret_cast<const s4v *>(w + (size_t)tap * 256 + p * 16 + q * 4);
+    }
+    const float bv = bias ? bias[lane & 15] : 0.f;
+
+    // ring slot of slice zz: (zz - zA + 1) mod 3; of half slice zh: zh & 1.
+    // Output slice z reads half slices (z - 1) >> 1 and (z + 1) >> 1; the
+    // next one, (z + 2) >> 1, is new when z is even.
+    fetch(zA - 1);
+    put(0);
+    fetch(zA);
+    put(1);
+    if constexpr (C2 > 0) {
+        fetch2((zA - 1) >> 1);
+        put2((zA - 1) >> 1);
+        if (((zA + 1) >> 1) != ((zA - 1) >> 1)) {
+            fetch2((zA + 1) >> 1);
+            put2((zA + 1) >> 1);
+        }
+    }
+    fetch(zA + 1);
+    bool have2 = false;  // pre2 holds half slice (z + 2) >> 1
+    for (int z = zA; z < zE; z++) {
+        const int k = z - zA;
+        put((k + 2) % 3);  // slice z + 1
+        if constexpr (C2 > 0) {
+            if (have2) put2((z + 1) >> 1);  // its slot held (z - 3) >> 1, read last by slice z - 1
+        }
+        __syncthreads();
+        fetch(z + 2);      // in flight during the MFMAs (zeros past the volume)
+        if constexpr (C2 > 0) {
+            have2 = !(z & 1);  // (z + 2) >> 1 differs from (z + 1) >> 1 for even z
+            if (have2) fetch2((z + 2) >> 1);
+        }
+        f4v acc[kC16Rows];
+#pragma unroll
+        for (int r = 0; r < kC16Rows; r++) acc[r] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int dz = 0; dz < 3; dz++) {
+            const int zz = z - 1 + dz;
+            if (zz < 0 || zz >= d.D) continue;  // uniform
+            const int slot = (k + dz) % 3, slot2 = (zz >> 1) & 1;
+            auto tap_row = [&](const int dh) __attribute__((always_inline)) {
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    const int tap = (dz * 3 + dh) * 3 + t;
+                    const int ws = wave * 16 + p + t;
+                    s4v bt = b[0], bu[C2 ? C2 / 16 : 1];
+                    if constexpr (C2 == 0) {
+                        bt = b[tap];
+                    } else {
+                        const bf16_t *wtap = w + (size_t)tap * 16 * Ct + (size_t)p * Ct + q * 4;
+                        bt = *reinterpret_cast<const s4v *>(wtap);
+#pragma unroll
+                        for (int j = 0; j < C2 / 16; j++) bu[j] = *reinterpret_cast<const s4v *>(wtap + 16 + j * 16);
+                    }
+#pragma unroll
+                    for (int r = 0; r < kC16Rows; r++) {
+                        const s4v a1 = __builtin_bit_cast(s4v, s1[slot][r + dh][q][ws]);
+                        acc[r] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, bt, acc[r], 0, 0, 0);
+                        if constexpr (C2 > 0) {
+                            const int hs2 = ((h0 + r + dh - 1) >> 1) - hb2;
+                            const int ws2 = ((w0 + ws - 1) >> 1) - wb2;
+#pragma unroll
+                            for (int j = 0; j < C2 / 16; j++) {
+                                const s4v a2 = __builtin_bit_cast(s4v, s2[slot2][hs2][j * 4 + q][ws2]);
+                                acc[r] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, bu[j], acc[r], 0, 0, 0);
+                            }
+                        }
+                    }
+                }
+            };
+            if constexpr (C2 == 0) {
+                tap_row(0);
+                tap_row(1);
+                tap_row(2);
+            } else {
+                // one tap row at a time (unrolled, the operands of all nine
+                // taps would need over 256 VGPRs)
+#pragma unroll 1
+                for (int dh = 0; dh < 3; dh++) tap_row(dh);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kC16Rows; r++) {
+            const int h = h0 + r;
+            if (h >= d.H) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int pw = w0 + wave * 16 + (lane >> 4) * 4 + i;
+                if (pw < d.W) {
+                    float v = acc[r][i] + bv;
+                    if (relu) v = fmaxf(v, 0.f);
+                    out[(((size_t)z * d.H + h) * d.W + pw) * 16 + (lane & 15)] = f2bf(v);
+                }
+            }
+        }
+        __syncthreads();  // every wave is done with slice z - 1 before its slot takes z + 2
+    }
+}
+
 // W'[t][ci][co] = W[26-t][co][ci]: the backward-data convolution's weights.
 __global__ void weight_transpose_kernel(const bf16_t *__restrict__ w, bf16_t *__restrict__ wt, int Cout,
                                         int Cin) {
@@ -546,6 +762,17 @@ static int launch_conv3(const bf16_t *in1, int C1, const bf16_t *in2, int C2, co
     }
     const int R = conv_rows(Cout);
     dim3 grid((unsigned)(((d.W + 63) / 64) * ((d.H + R - 1) / R) * d.D));
+    // 16 full-resolution channels to 16 outputs (with 32 upsampled channels
+    // only where D, H and W are even): the LDS-staged kernel
+    if (Cout == 16 && C1 == 16 && (C2 == 0 || (C2 == 32 && !(d.D & 1) && !(d.H & 1) && !(d.W & 1)))) {
+        dim3 gz((unsigned)(((d.W + 63) / 64) * ((d.H + kC16Rows - 1) / kC16Rows) * ((d.D + kC16Zrun - 1) / kC16Zrun)));
+        if (C2 == 0)
+            conv3_c16z_kernel<0><<<gz, kCvThreads, 0, st>>>(in1, nullptr, w, bias, out, d, relu);
+        else
+            conv3_c16z_kernel<32><<<gz, kCvThreads, 0, st>>>(in1, in2, w, bias, out, d, relu);
+        AREC_HIP(hipGetLastError());
+        return 0;
+    }
     switch (Cout) {
     case 16: conv3_mfma_kernel<16><<<grid, kCvThreads, 0, st>>>(in1, C1, in2, C2, w, bias, out, d, relu); break;
     case 32: conv3_mfma_kernel<32><<<grid, kCvThreads, 0, st>>>(in1, C1, in2, C2, w, bias, out, d, relu); break;
