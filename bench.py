@@ -579,6 +579,52 @@ def cgls_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def unet_workload(args):
+    """The learned warm start's network alone: one inference of the trained
+    3-level U-Net on an n^3 cube (default 512^3, random input from a fixed
+    seed), timed over --steps inferences after --warmup.  Prints one JSON
+    line with the time per inference and the MFMA arithmetic rate
+    (2 x 27 x Cin x Cout flop per voxel per 3x3x3 layer, summed over the
+    layers at their resolutions, against the 2.5 PFLOP/s BF16 dense peak)."""
+    import arec3d_amd as A
+    from arec3d_amd import unet as U
+
+    n = args.n or 512
+    ctx = A.Context(0)
+    params, scale, meta = U.load_weights()
+    net = U.UNet(ctx, n, n, n, params=params, scale=scale)
+    x = ctx.to_device(np.random.default_rng(1).random((n, n, n), dtype=np.float32))
+    y = ctx.empty((n, n, n))
+    for _ in range(max(1, args.warmup)):
+        net.forward(x, y)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.forward(x, y)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # 3x3x3 MFMA layers {cin, cout, level} (the first 1 -> 16 and last 16 -> 1 layers run on FP32 lanes)
+    layers = [(16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2), (96, 32, 1), (32, 32, 1),
+              (48, 16, 0), (16, 16, 0)]
+    flop = sum(2 * 27 * ci * co * (n >> lv) ** 3 for ci, co, lv in layers)
+    out = {
+        "metric": "U-Net inference time (learned warm start's network)",
+        "value": round(dt, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic (uniform random cube, seed 1; trained weights)",
+        "config": {
+            "workload": f"3-level 3-D U-Net 16/32/64 on a {n}^3 cube, {args.steps} inferences",
+            "mfma_flop_per_inference": flop,
+            "mfma_tflops": round(flop / dt / 1e12, 1),
+            "frac_of_bf16_dense_peak": round(flop / dt / 2.5e15, 4),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -600,10 +646,11 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: 3 forward + 3 back launches and a 1 GiB copy kernel, no solve")
-    ap.add_argument("--workload", choices=["warm", "align", "batch", "cgls", "rotate"], default="warm",
+    ap.add_argument("--workload", choices=["warm", "align", "batch", "cgls", "rotate", "unet"], default="warm",
                     help="warm: the headline warm-start solve; align: projection matching with known shifts; "
                          "batch: 64 uneven SIRT problems in one batch; cgls: CGLS 512x512x256/720 with an "
-                         "FP64 check; rotate: the rotational alignment step at the full slab's view shape")
+                         "FP64 check; rotate: the rotational alignment step at the full slab's view shape; "
+                         "unet: the warm start's network alone at 512^3")
     ap.add_argument("--samples", type=int, default=64, help="batch workload: problems")
     ap.add_argument("--cpu-rows-cgls", type=int, default=1, help="cgls workload: rows of the FP64 check")
     ap.add_argument("--outer", type=int, default=5)
@@ -637,6 +684,8 @@ def main():
         return batch_workload(args)
     if args.workload == "cgls":
         return cgls_workload(args)
+    if args.workload == "unet":
+        return unet_workload(args)
 
     import arec3d_amd as A
     from arec3d_amd import phantom as P

@@ -18,7 +18,7 @@ def test_bench_help_lists_workloads():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], cwd=ROOT,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0
-    for w in ("warm", "align", "batch", "cgls", "rotate"):
+    for w in ("warm", "align", "batch", "cgls", "rotate", "unet"):
         assert w in r.stdout
 
 
