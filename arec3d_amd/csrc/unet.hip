@@ -596,9 +596,13 @@ __global__ void conv_last_kernel(const bf16_t *__restrict__ t, const float *__re
                                  const float *__restrict__ xin, float *__restrict__ y, int64_t S,
                                  float inv_scale) {
     for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < S; s += (int64_t)gridDim.x * blockDim.x) {
+        // the voxel's 16 channels as two 16-byte loads
+        const uint4 *tv = reinterpret_cast<const uint4 *>(t + s * 16);
+        const uint4 lo = tv[0], hi = tv[1];
+        const unsigned u[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
         float a = b;
 #pragma unroll
-        for (int c = 0; c < 16; c++) a = fmaf(w[c], bf2f(t[s * 16 + c]), a);
+        for (int c = 0; c < 16; c++) a = fmaf(w[c], bf2f((bf16_t)((c & 1) ? u[c >> 1] >> 16 : u[c >> 1] & 0xffffu)), a);
         y[s] = xin[s] + a * inv_scale;
     }
 }
@@ -640,21 +644,37 @@ __global__ void relu_mask_kernel(const bf16_t *__restrict__ dout, const bf16_t *
     GS(i, n) dzv[i] = bf2f(act[i]) > 0.f ? dout[i] : (bf16_t)0;
 }
 
-// 2x2x2 average pooling
-__global__ void avgpool_kernel(const bf16_t *__restrict__ in, bf16_t *__restrict__ out, Dims d, int C) {
-    const int D2 = d.D / 2, H2 = d.H / 2, W2 = d.W / 2;
-    const int64_t n = (int64_t)D2 * H2 * W2 * C;
-    GS(e, n) {
-        int c = (int)(e % C);
-        int64_t s = e / C;
-        int x2 = (int)(s % W2), y2 = (int)((s / W2) % H2), z2 = (int)(s / ((int64_t)W2 * H2));
-        float a = 0.f;
-        for (int k = 0; k < 8; k++) {
-            int z = 2 * z2 + (k >> 2), y = 2 * y2 + ((k >> 1) & 1), x = 2 * x2 + (k & 1);
-            a += bf2f(in[(((int64_t)z * d.H + y) * d.W + x) * C + c]);
+// 2x2x2 average pooling, C a multiple of 8: a thread per (output position,
+// 8 channels) with 16-byte loads and stores, one workgroup per 256 such
+// items of one output (D, H) row, coordinates without 64-bit divisions.
+// Each channel's 8 values are summed in (dz, dy, dx) order.
+__global__ __launch_bounds__(256) void avgpool8_kernel(const bf16_t *__restrict__ in, bf16_t *__restrict__ out,
+                                                       Dims d, int C) {
+    const int H2 = d.H / 2, W2 = d.W / 2, G = C / 8;
+    const int per_row = W2 * G, rb = (per_row + 255) / 256;
+    const int row = (int)(blockIdx.x / (unsigned)rb);
+    const int t = (int)(blockIdx.x - (unsigned)row * rb) * 256 + (int)threadIdx.x;
+    if (t >= per_row) return;
+    const int x2 = t / G, g = t - x2 * G;
+    const int z2 = row / H2, y2 = row - z2 * H2;
+    float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int z = 2 * z2 + (k >> 2), y = 2 * y2 + ((k >> 1) & 1), x = 2 * x2 + (k & 1);
+        const uint4 v = *reinterpret_cast<const uint4 *>(in + (((int64_t)z * d.H + y) * d.W + x) * C + g * 8);
+        const unsigned u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            a[2 * j] += bf2f((bf16_t)(u[j] & 0xffffu));
+            a[2 * j + 1] += bf2f((bf16_t)(u[j] >> 16));
         }
-        out[e] = f2bf(0.125f * a);
     }
+    unsigned pk[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        pk[j] = (unsigned)f2bf(0.125f * a[2 * j]) | ((unsigned)f2bf(0.125f * a[2 * j + 1]) << 16);
+    *reinterpret_cast<uint4 *>(out + (((int64_t)z2 * H2 + y2) * W2 + x2) * C + g * 8) =
+        make_uint4(pk[0], pk[1], pk[2], pk[3]);
 }
 
 // dS = dpart + avgpool backward of dP (each fine voxel gets dP/8)
@@ -733,20 +753,38 @@ __global__ void to_bf16_kernel(const float *__restrict__ a, bf16_t *__restrict__
 }
 
 // cylinder [nrays][ny] <-> cube [ny][n][n] (outside the mask: 0)
+// Element e = j * ny + y of the cylinder volume <-> cube voxel (y, iz, ix)
+// of ray j; the index arithmetic in 32 bits where the volume allows (a
+// 64-bit division costs dozens of instructions per element).
+template <typename I>
+__device__ __forceinline__ void cyl_cube_pairs(const int *__restrict__ cord, const int nrays, const int ny,
+                                               const int n, const float *__restrict__ src, float *__restrict__ dst,
+                                               const bool to_cube) {
+    const I total = (I)nrays * ny;
+    for (I e = blockIdx.x * (I)blockDim.x + threadIdx.x; e < total; e += (I)gridDim.x * blockDim.x) {
+        const int j = (int)(e / (I)ny), y = (int)(e - (I)j * ny);
+        const int64_t c = ((int64_t)y * n + cord[2 * j + 1]) * n + cord[2 * j];
+        if (to_cube)
+            dst[c] = src[e];
+        else
+            dst[e] = src[c];
+    }
+}
+
 __global__ void cyl_to_cube_kernel(const float *__restrict__ x, const int *__restrict__ cord, int nrays,
                                    int ny, int n, float *__restrict__ cube) {
-    GS(e, (int64_t)nrays * ny) {
-        int j = (int)(e / ny), y = (int)(e % ny);
-        cube[((int64_t)y * n + cord[2 * j + 1]) * n + cord[2 * j]] = x[e];
-    }
+    if ((int64_t)nrays * ny < (int64_t)INT32_MAX - (int64_t)gridDim.x * blockDim.x)
+        cyl_cube_pairs<int>(cord, nrays, ny, n, x, cube, true);
+    else
+        cyl_cube_pairs<int64_t>(cord, nrays, ny, n, x, cube, true);
 }
 
 __global__ void cube_to_cyl_kernel(const float *__restrict__ cube, const int *__restrict__ cord, int nrays,
                                    int ny, int n, float *__restrict__ x) {
-    GS(e, (int64_t)nrays * ny) {
-        int j = (int)(e / ny), y = (int)(e % ny);
-        x[e] = cube[((int64_t)y * n + cord[2 * j + 1]) * n + cord[2 * j]];
-    }
+    if ((int64_t)nrays * ny < (int64_t)INT32_MAX - (int64_t)gridDim.x * blockDim.x)
+        cyl_cube_pairs<int>(cord, nrays, ny, n, cube, x, false);
+    else
+        cyl_cube_pairs<int64_t>(cord, nrays, ny, n, cube, x, false);
 }
 
 static dim3 ew(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384))); }
@@ -920,10 +958,12 @@ static int unet_forward(UNet &n, const float *x, float *y, hipStream_t st) {
     auto W = [&](int i) { return n.pb + n.L[i].woff; };
     auto B = [&](int i) { return n.p + n.L[i].boff; };
     if ((rc = launch_conv3(n.act[T1A], 16, nullptr, 0, W(0), B(0), n.act[S1], 16, d0, 1, st))) return rc;
-    avgpool_kernel<<<ew(n.vox(1) * 16), 256, 0, st>>>(n.act[S1], n.act[P1], d0, 16);
+    avgpool8_kernel<<<dim3((unsigned)((int64_t)d1.D * d1.H * ((d1.W * 2 + 255) / 256))), 256, 0, st>>>(
+        n.act[S1], n.act[P1], d0, 16);
     if ((rc = launch_conv3(n.act[P1], 16, nullptr, 0, W(1), B(1), n.act[T2A], 32, d1, 1, st))) return rc;
     if ((rc = launch_conv3(n.act[T2A], 32, nullptr, 0, W(2), B(2), n.act[S2], 32, d1, 1, st))) return rc;
-    avgpool_kernel<<<ew(n.vox(2) * 32), 256, 0, st>>>(n.act[S2], n.act[P2], d1, 32);
+    avgpool8_kernel<<<dim3((unsigned)((int64_t)d2.D * d2.H * ((d2.W * 4 + 255) / 256))), 256, 0, st>>>(
+        n.act[S2], n.act[P2], d1, 32);
     if ((rc = launch_conv3(n.act[P2], 32, nullptr, 0, W(3), B(3), n.act[T3A], 64, d2, 1, st))) return rc;
     if ((rc = launch_conv3(n.act[T3A], 64, nullptr, 0, W(4), B(4), n.act[B3], 64, d2, 1, st))) return rc;
     if ((rc = launch_conv3(n.act[S2], 32, n.act[B3], 64, W(5), B(5), n.act[T4A], 32, d1, 1, st))) return rc;
