@@ -398,6 +398,7 @@ struct StagedItem {
     int s, band;
     double t;
     int j;
+    int8_t d[kSgAngles];  // first-tap offset in strip s per angle of the group, -1 outside [0, kStripGroups)
 };
 
 // One angle group's chunks and entries, in table order: stage holds
@@ -405,22 +406,21 @@ struct StagedItem {
 // strip s; cnt kSgAngles entry counts per chunk; ent the entries {slot,
 // d, column (-1: padding)}, angle i's after angle i - 1's in each chunk.
 struct StagedGroup {
-    std::vector<int> p0;
-    std::vector<StagedItem> items;
+    std::vector<StagedItem> items, sorted;
+    std::vector<int64_t> bucket, pos;  // counting-sort offsets per (strip, band)
     std::vector<int> stage, nchunk, cnt;
     std::vector<int4> ent;  // x slot, y d, z column, w angle
 };
 
 void staged_group(const Geometry &g, int G, StagedGroup &o) {
     const int ns = g.nstrips, a0 = G * kSgAngles, na = std::min(kSgAngles, g.na - a0), ac = a0 + (na - 1) / 2;
-    o.p0.resize((size_t)g.nvalid * kSgAngles);
     o.items.clear();
-    int sa[2], un[2 * kSgAngles];
+    int sa[2], un[2 * kSgAngles], p0[kSgAngles];
     for (int j = 0; j < g.nvalid; j++) {
         int nu = 0;
         for (int i = 0; i < na; i++) {
             const int q = host_first_tap(g, a0 + i, j);
-            o.p0[(size_t)j * kSgAngles + i] = q;
+            p0[i] = q;
             const int n = strips_of(q, ns, sa);
             for (int k = 0; k < n; k++) {
                 bool seen = false;
@@ -431,14 +431,42 @@ void staged_group(const Geometry &g, int G, StagedGroup &o) {
         if (!nu) continue;
         const int band = host_first_tap(g, ac, j);
         const double t = host_depth(g, ac, j);
-        for (int u = 0; u < nu; u++) o.items.push_back({un[u], band, t, j});
+        for (int u = 0; u < nu; u++) {
+            StagedItem it{un[u], band, t, j, {}};
+            for (int i = 0; i < kSgAngles; i++) {
+                const int d = i < na ? p0[i] - (un[u] * kStripW - 2) : -1;
+                it.d[i] = (int8_t)(d >= 0 && d < kStripGroups ? d : -1);
+            }
+            o.items.push_back(it);
+        }
     }
-    std::sort(o.items.begin(), o.items.end(), [](const StagedItem &x, const StagedItem &y) {
-        if (x.s != y.s) return x.s < y.s;
-        if (x.band != y.band) return x.band < y.band;
-        if (x.t != y.t) return x.t < y.t;
-        return x.j < y.j;
-    });
+    // order by (strip, band, depth, column): a counting sort on (strip,
+    // band), then each bucket (the columns along one ray of the central
+    // angle) by depth -- the order of one full sort, in linear time but for
+    // the small buckets
+    if (!o.items.empty()) {
+        int bmin = o.items[0].band, bmax = bmin;
+        for (const StagedItem &it : o.items) {
+            bmin = std::min(bmin, it.band);
+            bmax = std::max(bmax, it.band);
+        }
+        const int nb = bmax - bmin + 1;
+        o.bucket.assign((size_t)ns * nb + 1, 0);
+        for (const StagedItem &it : o.items) o.bucket[(size_t)it.s * nb + (it.band - bmin) + 1]++;
+        for (size_t k = 1; k < o.bucket.size(); k++) o.bucket[k] += o.bucket[k - 1];
+        o.sorted.resize(o.items.size());
+        std::vector<int64_t> &pos = o.pos;
+        pos.assign(o.bucket.begin(), o.bucket.end() - 1);
+        for (const StagedItem &it : o.items) o.sorted[pos[(size_t)it.s * nb + (it.band - bmin)]++] = it;
+        for (size_t k = 0; k + 1 < o.bucket.size(); k++)
+            if (o.bucket[k + 1] - o.bucket[k] > 1)
+                std::sort(o.sorted.begin() + o.bucket[k], o.sorted.begin() + o.bucket[k + 1],
+                          [](const StagedItem &x, const StagedItem &y) {
+                              if (x.t != y.t) return x.t < y.t;
+                              return x.j < y.j;
+                          });
+        o.items.swap(o.sorted);
+    }
     o.stage.clear();
     o.nchunk.assign(ns, 0);
     o.cnt.clear();
@@ -455,23 +483,23 @@ void staged_group(const Geometry &g, int G, StagedGroup &o) {
             // offset runs are padded to whole batches (widely spaced angles)
             int nq = 0;
             std::fill(&fill[0][0], &fill[0][0] + kSgAngles * kStripGroups, 0);
+            int padded[kSgAngles] = {};  // per angle: its entries with every offset's run padded
+            auto up = [](int c) { return (c + kStripBatch - 1) / kStripBatch * kStripBatch; };
             while (nq < kSgChunk && q0 + nq < idx + n) {
-                const int j = o.items[q0 + nq].j;
+                const StagedItem &it = o.items[q0 + nq];
                 bool fits = true;
                 for (int i = 0; i < na && fits; i++) {
-                    const int d = o.p0[(size_t)j * kSgAngles + i] - (s * kStripW - 2);
-                    if (d < 0 || d >= kStripGroups) continue;
-                    int padded = 0;
-                    for (int e = 0; e < kStripGroups; e++) {
-                        const int c = fill[i][e] + (e == d);
-                        padded += (c + kStripBatch - 1) / kStripBatch * kStripBatch;
-                    }
-                    fits = padded <= kSgEnt;
+                    const int d = it.d[i];
+                    if (d < 0) continue;
+                    fits = padded[i] - up(fill[i][d]) + up(fill[i][d] + 1) <= kSgEnt;
                 }
                 if (!fits) break;
                 for (int i = 0; i < na; i++) {
-                    const int d = o.p0[(size_t)j * kSgAngles + i] - (s * kStripW - 2);
-                    if (d >= 0 && d < kStripGroups) fill[i][d]++;
+                    const int d = it.d[i];
+                    if (d >= 0) {
+                        padded[i] += up(fill[i][d] + 1) - up(fill[i][d]);
+                        fill[i][d]++;
+                    }
                 }
                 nq++;
             }
@@ -482,15 +510,14 @@ void staged_group(const Geometry &g, int G, StagedGroup &o) {
                 if (i < na) {
                     std::fill(runs, runs + kStripGroups + 1, 0);
                     for (int q = 0; q < nq; q++) {
-                        const int d = o.p0[(size_t)o.items[q0 + q].j * kSgAngles + i] - (s * kStripW - 2);
-                        if (d >= 0 && d < kStripGroups) runs[d]++;
+                        const int d = o.items[q0 + q].d[i];
+                        if (d >= 0) runs[d]++;
                     }
                     for (int d = 0; d < kStripGroups; d++) {
                         if (!runs[d]) continue;
                         for (int q = 0; q < nq; q++) {
-                            const int j = o.items[q0 + q].j;
-                            if (o.p0[(size_t)j * kSgAngles + i] - (s * kStripW - 2) == d)
-                                o.ent.push_back(make_int4(q, d, j, a0 + i));
+                            const StagedItem &it = o.items[q0 + q];
+                            if (it.d[i] == d) o.ent.push_back(make_int4(q, d, it.j, a0 + i));
                         }
                         for (int k = runs[d]; k % kStripBatch; k++) o.ent.push_back(make_int4(0, d, -1, a0 + i));
                     }

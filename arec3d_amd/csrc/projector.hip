@@ -24,7 +24,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
+#include <thread>
 
 #include "arec_internal.hpp"
 
@@ -1933,9 +1935,18 @@ static int build_staged_tables(Geometry &g, const int rpl, hipStream_t st) {
     std::vector<uint32_t> col, angi, meta;
     uint32_t *d_tmp = nullptr;
     int64_t cap = 0;
+    // progress lines on stderr when AREC_PROGRESS is set (the host ordering
+    // of the full slab's table takes minutes)
+    const bool progress = getenv("AREC_PROGRESS") != nullptr;
+    // batches of angle groups: up to 32 M entries, and at least one group per
+    // host thread (the groups of a batch are ordered in parallel) within
+    // 512 M entries (6 GB of staging on each side)
+    const int min_groups = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     for (int G0 = 0; G0 < ngroups;) {
         int G1 = G0 + 1;
-        while (G1 < ngroups && g.sg_group_ent[G1 + 1] - g.sg_group_ent[G0] <= (int64_t)(32 << 20)) G1++;
+        while (G1 < ngroups && (g.sg_group_ent[G1 + 1] - g.sg_group_ent[G0] <= (int64_t)(32 << 20) ||
+                                (G1 - G0 < min_groups && g.sg_group_ent[G1 + 1] - g.sg_group_ent[G0] <= (int64_t)(512 << 20))))
+            G1++;
         geometry_staged_groups(g, G0, G1, stage, off, col, angi, meta);
         AREC_HIP(hipStreamSynchronize(st));  // the previous batch is done with d_tmp
         const int64_t n = (int64_t)col.size();
@@ -1962,6 +1973,8 @@ static int build_staged_tables(Geometry &g, const int rpl, hipStream_t st) {
                 build_staged_table<kLinear><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
             AREC_HIP(hipGetLastError());
         }
+        if (progress)
+            fprintf(stderr, "[staged table] angle groups %d of %d ordered and uploaded\n", G1, ngroups);
         G0 = G1;
     }
     AREC_HIP(hipMemcpyAsync(g.d_sg_off + nchunk * kSgAngles, &g.sg_group_ent.back(), sizeof(int64_t),

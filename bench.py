@@ -237,8 +237,16 @@ def align_workload(args):
         g.set_forward(*args.plan)
         if args.back_tile:
             g.set_back(*args.back_tile)
+    else:
+        # the staged-strip forward (design 4) where its table fits in device
+        # memory beside a solve, else the detector sweep (design 1)
+        try:
+            g.set_forward(4, 2, 0)
+        except A.ArecError as e:
+            log(f"staged-strip forward not used ({e}); detector sweep")
     t_geom = time.perf_counter() - t0
-    log(f"geometry {nx}x{ny}x{na}: {t_geom:.1f} s, {g.device_bytes() / 1e9:.1f} GB of tables")
+    log(f"geometry {nx}x{ny}x{na}: {t_geom:.1f} s, {g.device_bytes() / 1e9:.1f} GB of tables, "
+        f"forward design {g.forward_design}")
     support = 1000.0 * f
     prm = P.ellipsoid_params(5, args.ellipsoids, support, ny, axes=(32.0 * f, 480.0 * f), dens=(0.05, 1.0))
     xt = ctx.empty(g.vol_shape)
@@ -297,6 +305,7 @@ def align_workload(args):
             "sirt_voxel_updates_per_s": round(vu / dt, 1),
             "geometry_setup_s": round(t_geom, 2),
             "geometry_device_bytes": g.device_bytes(),
+            "forward_design": g.forward_design,
             "last_relnrm_per_loop": [h[2] for h in hl],
             "rotational_step": bool(args.rotate),
         },
