@@ -12,9 +12,11 @@
 * The learned warm start at 512^3/720 on a phantom seed disjoint from the
   seeds its input iteration count was tuned on (49000-49002): the network is
   kept and reaches relnrm 1e-3 in at least 3x fewer passes than plain SIRT.
-* The 2048-wide slab (radius 1023.5, 1800 angles): forward design 1 at 4
-  rows per lane (the slab's plan) and the back projector's computed taps
-  match the oracle's rows; the whole 2048^2 x 1024 slab satisfies
+* The 2048-wide slab (radius 1023.5, 1800 angles): forward design 4 (the
+  full slab's plan: staged strips at 2 rows per lane) against the oracle's
+  rows with the adjoint identity on that geometry; forward design 1 at 4
+  rows per lane and the back projector's computed taps match the oracle's
+  rows; the whole 2048^2 x 1024 slab satisfies
   <Ax, y> = <x, A^T y> with element offsets past 2^31.
 """
 import concurrent.futures as cf
@@ -194,6 +196,40 @@ def test_slab_2048_rows_vs_oracle(ctx):
         op, ox = fo.result(), bo.result()
     assert rel_views(gp, op).max() <= 1e-5
     assert rel_views(column_blocks(gx), column_blocks(ox)).max() <= 1e-5
+
+
+def test_slab_2048_rows_vs_oracle_staged(ctx):
+    """2048 wide, 256 rows, 1800 angles on the forward the full slab's
+    projection matching runs: design 4 (staged strips) at 2 rows per lane,
+    its table of 7.2e9 entries (116 GB, the same at any row count) built by
+    the parallel host ordering.  Two rows of the forward against the oracle,
+    and the back projection's adjoint identity on that geometry."""
+    import arec3d_amd as A
+
+    nx, ny, na = 2048, 256, 1800
+    print(f"\n[case] slab rows, staged strips {nx}x{ny}x{na}", flush=True)
+    ang = (np.arange(na) * np.pi / na).astype(np.float32)
+    g = A.Geometry(ctx, "sq", nx, ny, ang)
+    g.set_forward(4, 2, 0)
+    assert g.forward_design == 4 and g.get_plan()[0] == 2
+    rows = [0, 129]
+    rng = np.random.default_rng(2049)
+    x = rng.random(g.vol_shape, dtype=np.float32)
+    with cf.ThreadPoolExecutor(max_workers=1) as ex:
+        fo = ex.submit(oracle_forward_rows, "sq", x, ang, nx, rows)
+        dx = ctx.to_device(x)
+        dp = ctx.empty(g.proj_shape)
+        g.project(dx, dp)
+        gp = dp.download()[:, rows, :]
+        y = ctx.empty(g.proj_shape)
+        y.upload(rng.random(g.proj_shape, dtype=np.float32))
+        aty = ctx.empty(g.vol_shape)
+        g.backproject(y, aty)
+        lhs, rhs = A.dot(ctx, dp, y), A.dot(ctx, dx, aty)
+        scale = np.sqrt(A.dot(ctx, aty, aty) * A.dot(ctx, dx, dx))
+        op = fo.result()
+    assert rel_views(gp, op).max() <= 1e-5
+    assert abs(lhs - rhs) <= 1e-5 * scale
 
 
 def test_slab_2048x2048x1024_1800_adjoint(ctx):
