@@ -39,11 +39,13 @@ def test_conv3_forward_vs_oracle(ctx, C1, C2, Cout):
 
 
 @pytest.mark.parametrize("C1,C2,Cout", [(16, 0, 16), (16, 32, 16), (64, 0, 64), (32, 0, 48)])
-@pytest.mark.parametrize("D,H,W", [(4, 10, 40), (2, 6, 70), (2, 2, 16)])
+@pytest.mark.parametrize("D,H,W", [(4, 10, 40), (2, 6, 70), (2, 2, 16), (36, 6, 70)])
 def test_conv3_forward_row_blocks(ctx, C1, C2, Cout, D, H, W):
     """Heights that are not a multiple of the kernel's rows per workgroup
     (4, or 2 at 64+ output channels): the last block's extra rows are not
-    written (arec_conv3_check fails if a guard past the output changes)."""
+    written (arec_conv3_check fails if a guard past the output changes).
+    D = 36 spans three of the 16-output kernel's runs of 16 D slices (its
+    input ring and half-resolution ring restart at slices 16 and 32)."""
     import arec3d_amd as A
 
     print(f"\n[case] conv3 row blocks {C1}+{C2}->{Cout} {D}x{H}x{W}", flush=True)
