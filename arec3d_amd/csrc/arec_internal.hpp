@@ -264,7 +264,8 @@ int64_t forward_batch_item(const Geometry &g, const float *x, float *p, const in
 // One launch of the staged-strip forward over several problems, all at rpl
 // rows per lane: nblk workgroups (a multiple of 8), owner[i] = the problem
 // of workgroups 8i .. 8i + 7 (device arrays).
-int launch_forward_batch(int rpl, const BatchFwd *d_probs, const int *d_owner, int64_t nblk, hipStream_t st);
+int launch_forward_batch(int method, int rpl, const BatchFwd *d_probs, const int *d_owner, int64_t nblk,
+                         hipStream_t st);
 
 // One problem of a (batched) SIRT solve (solver.hip).
 struct SirtProblem {

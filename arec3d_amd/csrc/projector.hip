@@ -970,6 +970,42 @@ __device__ __forceinline__ unsigned quad_bcast_u32(const unsigned v) {
     "v_fmac_f32_dpp %[a0" #R "], %[w0], %[v" #R "]" AREC_QP(K) "\n\t"                                    \
     "v_fmac_f32_dpp %[a1" #R "], %[w1], %[v" #R "]" AREC_QP(K) "\n\t"                                    \
     "v_fmac_f32_dpp %[a2" #R "], %[w2], %[v" #R "]" AREC_QP(K) "\n\t"
+#define AREC_FMA2(K, R)                                                                                   \
+    "v_fmac_f32_dpp %[a0" #R "], %[w0], %[v" #R "]" AREC_QP(K) "\n\t"                                    \
+    "v_fmac_f32_dpp %[a1" #R "], %[w1], %[v" #R "]" AREC_QP(K) "\n\t"
+// The linear pair's two taps (its third weight is zero): a_0, a_1 only.
+template <int K, int RPL>
+__device__ __forceinline__ void fma2_quad(float (&a0)[RPL], float (&a1)[RPL], const float w0, const float w1,
+                                          const float (&v)[RPL]) {
+    static_assert(K >= 0 && K < 4 && (RPL == 1 || RPL == 2 || RPL == 4), "quad entry and rows per lane");
+#define AREC_FMA_ASM(K)                                                                                   \
+    if constexpr (RPL == 1) {                                                                             \
+        asm volatile("s_nop 1\n\t" AREC_FMA2(K, 0)                                                        \
+                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0])                                               \
+                     : [w0] "v"(w0), [w1] "v"(w1), [v0] "v"(v[0]));                                       \
+    } else if constexpr (RPL == 2) {                                                                      \
+        asm volatile("s_nop 1\n\t" AREC_FMA2(K, 0) AREC_FMA2(K, 1)                                        \
+                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0]), [a01] "+v"(a0[RPL > 1]), [a11] "+v"(a1[RPL > 1]) \
+                     : [w0] "v"(w0), [w1] "v"(w1), [v0] "v"(v[0]), [v1] "v"(v[RPL > 1]));                 \
+    } else {                                                                                              \
+        constexpr int R1 = RPL > 1 ? 1 : 0, R2 = RPL > 2 ? 2 : 0, R3 = RPL > 3 ? 3 : 0;                   \
+        asm volatile("s_nop 1\n\t" AREC_FMA2(K, 0) AREC_FMA2(K, 1) AREC_FMA2(K, 2) AREC_FMA2(K, 3)          \
+                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0]), [a01] "+v"(a0[R1]), [a11] "+v"(a1[R1]),      \
+                       [a02] "+v"(a0[R2]), [a12] "+v"(a1[R2]), [a03] "+v"(a0[R3]), [a13] "+v"(a1[R3])     \
+                     : [w0] "v"(w0), [w1] "v"(w1), [v0] "v"(v[0]), [v1] "v"(v[R1]), [v2] "v"(v[R2]),      \
+                       [v3] "v"(v[R3]));                                                                  \
+    }
+    if constexpr (K == 0) {
+        AREC_FMA_ASM(0)
+    } else if constexpr (K == 1) {
+        AREC_FMA_ASM(1)
+    } else if constexpr (K == 2) {
+        AREC_FMA_ASM(2)
+    } else {
+        AREC_FMA_ASM(3)
+    }
+#undef AREC_FMA_ASM
+}
 template <int K, int RPL>
 __device__ __forceinline__ void fma3_quad(float (&a0)[RPL], float (&a1)[RPL], float (&a2)[RPL], const float w0,
                                           const float w1, const float w2, const float (&v)[RPL]) {
@@ -1005,27 +1041,36 @@ __device__ __forceinline__ void fma3_quad(float (&a0)[RPL], float (&a1)[RPL], fl
 #undef AREC_FMA_ASM
 }
 #undef AREC_FMA3
+#undef AREC_FMA2
 #undef AREC_QP
 
 // acc[D..D+2] += w(entry k) * v[k] for the batch's four entries (lane group
 // l & ~3 holds them), for the one offset D whose bit is set in present;
 // entry by entry, so every accumulator sums in entry order, as strip_add.
-template <int LO, int HI, int RPL>
+template <int LO, int HI, int RPL, int TAPS>
 __device__ __forceinline__ void strip_add_quad(float (&acc)[kStripW + 4][RPL], const unsigned present,
                                                const float w0, const float w1, const float w2,
                                                const float (&v)[kStripBatch][RPL]) {
     static_assert(kStripBatch == 4, "one entry per lane of a quad");
+    static_assert(TAPS == 2 || TAPS == 3, "taps per entry");
     if constexpr (HI - LO == 1) {
-        fma3_quad<0, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[0]);
-        fma3_quad<1, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[1]);
-        fma3_quad<2, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[2]);
-        fma3_quad<3, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[3]);
+        if constexpr (TAPS == 3) {
+            fma3_quad<0, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[0]);
+            fma3_quad<1, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[1]);
+            fma3_quad<2, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[2]);
+            fma3_quad<3, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[3]);
+        } else {
+            fma2_quad<0, RPL>(acc[LO], acc[LO + 1], w0, w1, v[0]);
+            fma2_quad<1, RPL>(acc[LO], acc[LO + 1], w0, w1, v[1]);
+            fma2_quad<2, RPL>(acc[LO], acc[LO + 1], w0, w1, v[2]);
+            fma2_quad<3, RPL>(acc[LO], acc[LO + 1], w0, w1, v[3]);
+        }
     } else {
         constexpr int MID = (LO + HI) / 2;
         constexpr unsigned ML = ((1u << MID) - 1u) & ~((1u << LO) - 1u);
         constexpr unsigned MH = ((1u << HI) - 1u) & ~((1u << MID) - 1u);
-        if (present & ML) strip_add_quad<LO, MID, RPL>(acc, present, w0, w1, w2, v);
-        if (present & MH) strip_add_quad<MID, HI, RPL>(acc, present, w0, w1, w2, v);
+        if (present & ML) strip_add_quad<LO, MID, RPL, TAPS>(acc, present, w0, w1, w2, v);
+        if (present & MH) strip_add_quad<MID, HI, RPL, TAPS>(acc, present, w0, w1, w2, v);
     }
 }
 
@@ -1044,7 +1089,7 @@ __device__ __forceinline__ void strip_add_quad(float (&acc)[kStripW + 4][RPL], c
 // than two direct 4-byte loads: 15.9 vs 17.0 ms at 512^3/720).  A batch reads its four
 // entries from LDS (the same address in every lane), takes its offset from
 // the first, and adds each entry's column rows with the entry's weights.
-template <int RPL, bool FULL>
+template <int RPL, bool FULL, int TAPS>
 __device__ __forceinline__ void staged_walk(float (*__restrict__ col0)[64 * RPL], float (*__restrict__ col1)[64 * RPL],
                                             uint4 *__restrict__ ent0, uint4 *__restrict__ ent1,
                                             const float *__restrict__ x, const uint4 *__restrict__ tab,
@@ -1174,7 +1219,7 @@ __device__ __forceinline__ void staged_walk(float (*__restrict__ col0)[64 * RPL]
             // bit tests into a switch whose leaves copy every accumulator
             // (at 1 row per lane past 128 VGPRs, into scratch)
             asm volatile("" : "+s"(present));
-            strip_add_quad<0, kStripGroups, RPL>(acc, present, __uint_as_float(q.y), __uint_as_float(q.z),
+            strip_add_quad<0, kStripGroups, RPL, TAPS>(acc, present, __uint_as_float(q.y), __uint_as_float(q.z),
                                                  __uint_as_float(q.w), v);
         }
         if (more) {
@@ -1200,7 +1245,7 @@ __device__ __forceinline__ void staged_walk(float (*__restrict__ col0)[64 * RPL]
 
 // Workgroup b of one staged-strip forward: angle group b % ngroups (one
 // angle per wave), then detector strip, then row chunk.
-template <int RPL>
+template <int RPL, int TAPS>
 __device__ __forceinline__ void forward_staged_body(const float *__restrict__ x, float *__restrict__ p,
                                                     const uint4 *__restrict__ tab, const int *__restrict__ stage,
                                                     const int64_t *__restrict__ chunk_base,
@@ -1224,10 +1269,10 @@ __device__ __forceinline__ void forward_staged_body(const float *__restrict__ x,
         for (int r = 0; r < RPL; r++) acc[i][r] = 0.0f;
     // whole chunk with every column's rows aligned to RPL floats: vector loads
     if (ybase + ROWS <= ny && (ny & (RPL - 1)) == 0)
-        staged_walk<RPL, true>(s_col[0], s_col[1], s_ent[0][wave], s_ent[1][wave], x, tab, stage, off, K0, K1, wave,
+        staged_walk<RPL, true, TAPS>(s_col[0], s_col[1], s_ent[0][wave], s_ent[1][wave], x, tab, stage, off, K0, K1, wave,
                                lane, ny, y0, acc);
     else
-        staged_walk<RPL, false>(s_col[0], s_col[1], s_ent[0][wave], s_ent[1][wave], x, tab, stage, off, K0, K1, wave,
+        staged_walk<RPL, false, TAPS>(s_col[0], s_col[1], s_ent[0][wave], s_ent[1][wave], x, tab, stage, off, K0, K1, wave,
                                 lane, ny, y0, acc);
     if (!active) return;
     strip_store<RPL>(acc, p, partials, a, st, chunk, nchunks, nstrips, nx, ny, y0, lane);
@@ -1236,7 +1281,7 @@ __device__ __forceinline__ void forward_staged_body(const float *__restrict__ x,
 // At least 4 waves per SIMD, so that two workgroups share a CU.  With a
 // multiple of 8 workgroups each XCD takes a contiguous range of them
 // (neighbouring angle groups, which read the same columns).
-template <int RPL>
+template <int RPL, int TAPS>
 __global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(RPL == 2 ? 6 : 4, 8))) void forward_staged_kernel(
     const float *__restrict__ x, float *__restrict__ p, const uint4 *__restrict__ tab, const int *__restrict__ stage,
     const int64_t *__restrict__ chunk_base, const int64_t *__restrict__ off, double *__restrict__ partials,
@@ -1245,7 +1290,7 @@ __global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(RPL 
     const int nblk = gridDim.x;
     int b = blockIdx.x;
     if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
-    forward_staged_body<RPL>(x, p, tab, stage, chunk_base, off, partials, nstrips, nx, ny, na, nchunks, b);
+    forward_staged_body<RPL, TAPS>(x, p, tab, stage, chunk_base, off, partials, nstrips, nx, ny, na, nchunks, b);
 }
 
 // The staged-strip forward of many problems in one launch (one grid, no
@@ -1255,7 +1300,7 @@ __global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(RPL 
 // local index l of the problem sits on XCD l & 7 and is remapped as in the
 // single launch: each XCD takes a contiguous eighth of every problem, and
 // all XCDs move through the problems together.  Padding workgroups return.
-template <int RPL>
+template <int RPL, int TAPS>
 __global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(RPL == 2 ? 6 : 4, 8))) void forward_staged_batch_kernel(
     const BatchFwd *__restrict__ probs, const int *__restrict__ owner) {
     const int k = owner[blockIdx.x >> 3];
@@ -1265,7 +1310,7 @@ __global__ __launch_bounds__(kSgThreads) __attribute__((amdgpu_waves_per_eu(RPL 
     const int m = ((q.nblk + 7) >> 3);  // workgroups per XCD
     const int b = (l & 7) * m + (l >> 3);
     if (b >= q.nblk) return;
-    forward_staged_body<RPL>(as_global(q.x), as_global(q.p), as_global(q.tab), as_global(q.stage),
+    forward_staged_body<RPL, TAPS>(as_global(q.x), as_global(q.p), as_global(q.tab), as_global(q.stage),
                              as_global(q.chunk_base), as_global(q.off), nullptr, q.nstrips, q.nx, q.ny, q.na,
                              q.nchunks, b);
 }
@@ -1448,6 +1493,20 @@ __device__ __forceinline__ void backward_body(const float *__restrict__ p, float
                 const float4 *c0 = reinterpret_cast<const float4 *>(&s_p[cur][ab][o][0]);
                 const float4 *c1 = reinterpret_cast<const float4 *>(&s_p[cur][ab][o + 1][0]);
                 const float4 *c2 = reinterpret_cast<const float4 *>(&s_p[cur][ab][o + 2][0]);
+                if constexpr (M == kLinear) {
+                    // two taps: the linear pair's third weight is zero
+#pragma unroll
+                    for (int q = 0; q < kBpRows / 4; q++) {
+                        float4 v0 = c0[q], v1 = c1[q];
+                        float4 a = acc[q];
+                        a.x = fmaf(w1, v1.x, fmaf(w0, v0.x, a.x));
+                        a.y = fmaf(w1, v1.y, fmaf(w0, v0.y, a.y));
+                        a.z = fmaf(w1, v1.z, fmaf(w0, v0.z, a.z));
+                        a.w = fmaf(w1, v1.w, fmaf(w0, v0.w, a.w));
+                        acc[q] = a;
+                    }
+                    continue;
+                }
 #pragma unroll
                 for (int q = 0; q < kBpRows / 4; q++) {
                     float4 v0 = c0[q], v1 = c1[q], v2 = c2[q];
@@ -2107,16 +2166,19 @@ int launch_forward(const Geometry &g, const float *x, float *p, double *partials
         const int rpl = g.strip_rpl, nchunks = (g.ny + 64 * rpl - 1) / (64 * rpl);
         const int ngroups = (g.na + kSgAngles - 1) / kSgAngles;
         dim3 grid((unsigned)((int64_t)ngroups * g.nstrips * nchunks));
-#define AREC_FWDG(R)                                                                                          \
-    forward_staged_kernel<R><<<grid, kSgThreads, 0, st>>>(x, p, g.d_sg_tab, g.d_sg_stage, g.d_sg_chunk_base,   \
-                                                          g.d_sg_off, partials, g.nstrips, g.nx, g.ny, g.na, \
-                                                          nchunks, skip)
-        if (rpl == 4)
-            AREC_FWDG(4);
-        else if (rpl == 2)
-            AREC_FWDG(2);
-        else
-            AREC_FWDG(1);
+        // the linear pair's entries have two taps (the third weight is zero)
+#define AREC_FWDG(R, T)                                                                                           \
+    forward_staged_kernel<R, T><<<grid, kSgThreads, 0, st>>>(x, p, g.d_sg_tab, g.d_sg_stage, g.d_sg_chunk_base,    \
+                                                             g.d_sg_off, partials, g.nstrips, g.nx, g.ny, g.na, \
+                                                             nchunks, skip)
+        const bool two = g.method == kLinear;
+        if (rpl == 4) {
+            if (two) AREC_FWDG(4, 2); else AREC_FWDG(4, 3);
+        } else if (rpl == 2) {
+            if (two) AREC_FWDG(2, 2); else AREC_FWDG(2, 3);
+        } else {
+            if (two) AREC_FWDG(1, 2); else AREC_FWDG(1, 3);
+        }
 #undef AREC_FWDG
         AREC_HIP(hipGetLastError());
         return 0;
@@ -2221,24 +2283,28 @@ int64_t forward_batch_item(const Geometry &g, const float *x, float *p, const in
     return (nblk + 7) / 8 * 8;
 }
 
-int launch_forward_batch(int rpl, const BatchFwd *d_probs, const int *d_owner, int64_t nblk, hipStream_t st) {
+int launch_forward_batch(int method, int rpl, const BatchFwd *d_probs, const int *d_owner, int64_t nblk,
+                         hipStream_t st) {
     if (nblk <= 0) return 0;
     if (nblk % 8 || nblk > INT32_MAX) {
         set_error("batched forward projection: workgroup count must be a multiple of 8 below 2^31");
         return AREC_ESIZE;
     }
     dim3 grid((unsigned)nblk);
-    if (rpl == 4)
-        forward_staged_batch_kernel<4><<<grid, kSgThreads, 0, st>>>(d_probs, d_owner);
-    else if (rpl == 2)
-        forward_staged_batch_kernel<2><<<grid, kSgThreads, 0, st>>>(d_probs, d_owner);
-    else if (rpl == 1)
-        forward_staged_batch_kernel<1><<<grid, kSgThreads, 0, st>>>(d_probs, d_owner);
-    else {
+    const bool two = method == kLinear;  // two taps per entry
+#define AREC_FWDB(R, T) forward_staged_batch_kernel<R, T><<<grid, kSgThreads, 0, st>>>(d_probs, d_owner)
+    if (rpl == 4) {
+        if (two) AREC_FWDB(4, 2); else AREC_FWDB(4, 3);
+    } else if (rpl == 2) {
+        if (two) AREC_FWDB(2, 2); else AREC_FWDB(2, 3);
+    } else if (rpl == 1) {
+        if (two) AREC_FWDB(1, 2); else AREC_FWDB(1, 3);
+    } else {
         set_error("batched forward projection: rows per lane must be 1, 2 or 4");
         return AREC_EINVAL;
     }
     AREC_HIP(hipGetLastError());
+#undef AREC_FWDB
     return 0;
 }
 

@@ -599,8 +599,8 @@ int sirt_batch_run(const std::vector<SirtProblem> &P, hipStream_t st, int maxit,
                 return r;
         } else {
             for (const FwdClass &c : fclass)
-                if ((r = launch_forward_batch(c.rpl, dfwd.as<BatchFwd>() + c.prob0, down.as<int>() + c.own0, c.nblk,
-                                              st)))
+                if ((r = launch_forward_batch(method, c.rpl, dfwd.as<BatchFwd>() + c.prob0, down.as<int>() + c.own0,
+                                              c.nblk, st)))
                     return r;
             for (int k : fsolo)
                 if ((r = launch_forward(*P[k].g, P[k].x, proj.as<float>() + o_proj[k], nullptr, st, &S[k].skip_proj)))
