@@ -960,111 +960,88 @@ __device__ __forceinline__ unsigned quad_bcast_u32(const unsigned v) {
     return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, K * 0x55, 0xf, 0xf, false);
 }
 
-// a_t[r] += w_t(entry K) * v[r] for the three taps t: the weights are lane
-// (l & ~3) + K's, read through a quad_perm DPP operand of v_fmac_f32 (the
-// compiler keeps such a broadcast as a separate v_mov_b32_dpp).  A DPP
-// operand must not be written by a VALU instruction in the two preceding
-// cycles (the weights come from an LDS read; s_nop 1 covers any copy).
+// a_t[r] += w_t(entry k) * v[k][r] for the batch's four entries k and the
+// taps t (3, or the linear pair's 2): entry k's weights are lane (l & ~3) +
+// k's, read through a quad_perm DPP operand of v_fmac_f32 (the compiler keeps
+// such a broadcast as a separate v_mov_b32_dpp).  Entry by entry, so every
+// accumulator sums in entry order.  One asm statement per batch (two at 4
+// rows per lane, for the operand count), each opening with the two wait
+// states a DPP operand needs after a VALU write (the weights come from an
+// LDS read; s_nop 1 covers any copy); inside a statement no DPP operand is
+// written.
 #define AREC_QP(K) " quad_perm:[" #K "," #K "," #K "," #K "] row_mask:0xf bank_mask:0xf"
-#define AREC_FMA3(K, R)                                                                                   \
-    "v_fmac_f32_dpp %[a0" #R "], %[w0], %[v" #R "]" AREC_QP(K) "\n\t"                                    \
-    "v_fmac_f32_dpp %[a1" #R "], %[w1], %[v" #R "]" AREC_QP(K) "\n\t"                                    \
-    "v_fmac_f32_dpp %[a2" #R "], %[w2], %[v" #R "]" AREC_QP(K) "\n\t"
-#define AREC_FMA2(K, R)                                                                                   \
-    "v_fmac_f32_dpp %[a0" #R "], %[w0], %[v" #R "]" AREC_QP(K) "\n\t"                                    \
-    "v_fmac_f32_dpp %[a1" #R "], %[w1], %[v" #R "]" AREC_QP(K) "\n\t"
-// The linear pair's two taps (its third weight is zero): a_0, a_1 only.
-template <int K, int RPL>
-__device__ __forceinline__ void fma2_quad(float (&a0)[RPL], float (&a1)[RPL], const float w0, const float w1,
-                                          const float (&v)[RPL]) {
-    static_assert(K >= 0 && K < 4 && (RPL == 1 || RPL == 2 || RPL == 4), "quad entry and rows per lane");
-#define AREC_FMA_ASM(K)                                                                                   \
-    if constexpr (RPL == 1) {                                                                             \
-        asm volatile("s_nop 1\n\t" AREC_FMA2(K, 0)                                                        \
-                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0])                                               \
-                     : [w0] "v"(w0), [w1] "v"(w1), [v0] "v"(v[0]));                                       \
-    } else if constexpr (RPL == 2) {                                                                      \
-        asm volatile("s_nop 1\n\t" AREC_FMA2(K, 0) AREC_FMA2(K, 1)                                        \
-                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0]), [a01] "+v"(a0[RPL > 1]), [a11] "+v"(a1[RPL > 1]) \
-                     : [w0] "v"(w0), [w1] "v"(w1), [v0] "v"(v[0]), [v1] "v"(v[RPL > 1]));                 \
-    } else {                                                                                              \
-        constexpr int R1 = RPL > 1 ? 1 : 0, R2 = RPL > 2 ? 2 : 0, R3 = RPL > 3 ? 3 : 0;                   \
-        asm volatile("s_nop 1\n\t" AREC_FMA2(K, 0) AREC_FMA2(K, 1) AREC_FMA2(K, 2) AREC_FMA2(K, 3)          \
-                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0]), [a01] "+v"(a0[R1]), [a11] "+v"(a1[R1]),      \
-                       [a02] "+v"(a0[R2]), [a12] "+v"(a1[R2]), [a03] "+v"(a0[R3]), [a13] "+v"(a1[R3])     \
-                     : [w0] "v"(w0), [w1] "v"(w1), [v0] "v"(v[0]), [v1] "v"(v[R1]), [v2] "v"(v[R2]),      \
-                       [v3] "v"(v[R3]));                                                                  \
-    }
-    if constexpr (K == 0) {
-        AREC_FMA_ASM(0)
-    } else if constexpr (K == 1) {
-        AREC_FMA_ASM(1)
-    } else if constexpr (K == 2) {
-        AREC_FMA_ASM(2)
+#define AREC_TAP(T, K, R) "v_fmac_f32_dpp %[a" #T #R "], %[w" #T "], %[v" #K #R "]" AREC_QP(K) "\n\t"
+#define AREC_E3(K, R) AREC_TAP(0, K, R) AREC_TAP(1, K, R) AREC_TAP(2, K, R)
+#define AREC_E2(K, R) AREC_TAP(0, K, R) AREC_TAP(1, K, R)
+#define AREC_R1(E, K) E(K, 0)
+#define AREC_R2(E, K) E(K, 0) E(K, 1)
+#define AREC_R4(E, K) E(K, 0) E(K, 1) E(K, 2) E(K, 3)
+#define AREC_A3(R) [a0##R] "+v"(a0[R]), [a1##R] "+v"(a1[R]), [a2##R] "+v"(a2[R])
+#define AREC_A2(R) [a0##R] "+v"(a0[R]), [a1##R] "+v"(a1[R])
+#define AREC_W3 [w0] "v"(w0), [w1] "v"(w1), [w2] "v"(w2)
+#define AREC_W2 [w0] "v"(w0), [w1] "v"(w1)
+#define AREC_V(K, R) [v##K##R] "v"(v[K][R])
+#define AREC_V2(K) AREC_V(K, 0), AREC_V(K, 1)
+#define AREC_V4(K) AREC_V(K, 0), AREC_V(K, 1), AREC_V(K, 2), AREC_V(K, 3)
+template <int RPL, int TAPS>
+__device__ __forceinline__ void fma_batch(float (&a0)[RPL], float (&a1)[RPL], float (&a2)[RPL], const float w0,
+                                          const float w1, const float w2, const float (&v)[kStripBatch][RPL]) {
+    static_assert(kStripBatch == 4 && (RPL == 1 || RPL == 2 || RPL == 4) && (TAPS == 2 || TAPS == 3),
+                  "quad entries, rows per lane, taps");
+    if constexpr (TAPS == 3 && RPL == 1) {
+        asm volatile("s_nop 1\n\t" AREC_R1(AREC_E3, 0) AREC_R1(AREC_E3, 1) AREC_R1(AREC_E3, 2) AREC_R1(AREC_E3, 3)
+                     : AREC_A3(0)
+                     : AREC_W3, AREC_V(0, 0), AREC_V(1, 0), AREC_V(2, 0), AREC_V(3, 0));
+    } else if constexpr (TAPS == 3 && RPL == 2) {
+        asm volatile("s_nop 1\n\t" AREC_R2(AREC_E3, 0) AREC_R2(AREC_E3, 1) AREC_R2(AREC_E3, 2) AREC_R2(AREC_E3, 3)
+                     : AREC_A3(0), AREC_A3(1)
+                     : AREC_W3, AREC_V2(0), AREC_V2(1), AREC_V2(2), AREC_V2(3));
+    } else if constexpr (TAPS == 3) {
+        asm volatile("s_nop 1\n\t" AREC_R4(AREC_E3, 0) AREC_R4(AREC_E3, 1)
+                     : AREC_A3(0), AREC_A3(1), AREC_A3(2), AREC_A3(3)
+                     : AREC_W3, AREC_V4(0), AREC_V4(1));
+        asm volatile("s_nop 1\n\t" AREC_R4(AREC_E3, 2) AREC_R4(AREC_E3, 3)
+                     : AREC_A3(0), AREC_A3(1), AREC_A3(2), AREC_A3(3)
+                     : AREC_W3, AREC_V4(2), AREC_V4(3));
+    } else if constexpr (RPL == 1) {
+        asm volatile("s_nop 1\n\t" AREC_R1(AREC_E2, 0) AREC_R1(AREC_E2, 1) AREC_R1(AREC_E2, 2) AREC_R1(AREC_E2, 3)
+                     : AREC_A2(0)
+                     : AREC_W2, AREC_V(0, 0), AREC_V(1, 0), AREC_V(2, 0), AREC_V(3, 0));
+    } else if constexpr (RPL == 2) {
+        asm volatile("s_nop 1\n\t" AREC_R2(AREC_E2, 0) AREC_R2(AREC_E2, 1) AREC_R2(AREC_E2, 2) AREC_R2(AREC_E2, 3)
+                     : AREC_A2(0), AREC_A2(1)
+                     : AREC_W2, AREC_V2(0), AREC_V2(1), AREC_V2(2), AREC_V2(3));
     } else {
-        AREC_FMA_ASM(3)
+        asm volatile("s_nop 1\n\t" AREC_R4(AREC_E2, 0) AREC_R4(AREC_E2, 1) AREC_R4(AREC_E2, 2) AREC_R4(AREC_E2, 3)
+                     : AREC_A2(0), AREC_A2(1), AREC_A2(2), AREC_A2(3)
+                     : AREC_W2, AREC_V4(0), AREC_V4(1), AREC_V4(2), AREC_V4(3));
     }
-#undef AREC_FMA_ASM
 }
-template <int K, int RPL>
-__device__ __forceinline__ void fma3_quad(float (&a0)[RPL], float (&a1)[RPL], float (&a2)[RPL], const float w0,
-                                          const float w1, const float w2, const float (&v)[RPL]) {
-    static_assert(K >= 0 && K < 4 && (RPL == 1 || RPL == 2 || RPL == 4), "quad entry and rows per lane");
-#define AREC_FMA_ASM(K)                                                                                   \
-    if constexpr (RPL == 1) {                                                                             \
-        asm volatile("s_nop 1\n\t" AREC_FMA3(K, 0)                                                        \
-                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0]), [a20] "+v"(a2[0])                            \
-                     : [w0] "v"(w0), [w1] "v"(w1), [w2] "v"(w2), [v0] "v"(v[0]));                        \
-    } else if constexpr (RPL == 2) {                                                                      \
-        asm volatile("s_nop 1\n\t" AREC_FMA3(K, 0) AREC_FMA3(K, 1)                                        \
-                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0]), [a20] "+v"(a2[0]), [a01] "+v"(a0[RPL > 1]),  \
-                       [a11] "+v"(a1[RPL > 1]), [a21] "+v"(a2[RPL > 1])                                   \
-                     : [w0] "v"(w0), [w1] "v"(w1), [w2] "v"(w2), [v0] "v"(v[0]), [v1] "v"(v[RPL > 1]));  \
-    } else {                                                                                              \
-        constexpr int R1 = RPL > 1 ? 1 : 0, R2 = RPL > 2 ? 2 : 0, R3 = RPL > 3 ? 3 : 0;                   \
-        asm volatile("s_nop 1\n\t" AREC_FMA3(K, 0) AREC_FMA3(K, 1) AREC_FMA3(K, 2) AREC_FMA3(K, 3)          \
-                     : [a00] "+v"(a0[0]), [a10] "+v"(a1[0]), [a20] "+v"(a2[0]), [a01] "+v"(a0[R1]),       \
-                       [a11] "+v"(a1[R1]), [a21] "+v"(a2[R1]), [a02] "+v"(a0[R2]), [a12] "+v"(a1[R2]),    \
-                       [a22] "+v"(a2[R2]), [a03] "+v"(a0[R3]), [a13] "+v"(a1[R3]), [a23] "+v"(a2[R3])     \
-                     : [w0] "v"(w0), [w1] "v"(w1), [w2] "v"(w2), [v0] "v"(v[0]), [v1] "v"(v[R1]),         \
-                       [v2] "v"(v[R2]), [v3] "v"(v[R3]));                                                 \
-    }
-    if constexpr (K == 0) {
-        AREC_FMA_ASM(0)
-    } else if constexpr (K == 1) {
-        AREC_FMA_ASM(1)
-    } else if constexpr (K == 2) {
-        AREC_FMA_ASM(2)
-    } else {
-        AREC_FMA_ASM(3)
-    }
-#undef AREC_FMA_ASM
-}
-#undef AREC_FMA3
-#undef AREC_FMA2
+#undef AREC_V4
+#undef AREC_V2
+#undef AREC_V
+#undef AREC_W2
+#undef AREC_W3
+#undef AREC_A2
+#undef AREC_A3
+#undef AREC_R4
+#undef AREC_R2
+#undef AREC_R1
+#undef AREC_E2
+#undef AREC_E3
+#undef AREC_TAP
 #undef AREC_QP
 
 // acc[D..D+2] += w(entry k) * v[k] for the batch's four entries (lane group
-// l & ~3 holds them), for the one offset D whose bit is set in present;
-// entry by entry, so every accumulator sums in entry order, as strip_add.
+// l & ~3 holds them), for the one offset D whose bit is set in present: a
+// binary search over [LO, HI), one path through it (present has exactly one
+// bit, inside the range).
 template <int LO, int HI, int RPL, int TAPS>
 __device__ __forceinline__ void strip_add_quad(float (&acc)[kStripW + 4][RPL], const unsigned present,
                                                const float w0, const float w1, const float w2,
                                                const float (&v)[kStripBatch][RPL]) {
-    static_assert(kStripBatch == 4, "one entry per lane of a quad");
-    static_assert(TAPS == 2 || TAPS == 3, "taps per entry");
     if constexpr (HI - LO == 1) {
-        if constexpr (TAPS == 3) {
-            fma3_quad<0, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[0]);
-            fma3_quad<1, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[1]);
-            fma3_quad<2, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[2]);
-            fma3_quad<3, RPL>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v[3]);
-        } else {
-            fma2_quad<0, RPL>(acc[LO], acc[LO + 1], w0, w1, v[0]);
-            fma2_quad<1, RPL>(acc[LO], acc[LO + 1], w0, w1, v[1]);
-            fma2_quad<2, RPL>(acc[LO], acc[LO + 1], w0, w1, v[2]);
-            fma2_quad<3, RPL>(acc[LO], acc[LO + 1], w0, w1, v[3]);
-        }
+        fma_batch<RPL, TAPS>(acc[LO], acc[LO + 1], acc[LO + 2], w0, w1, w2, v);
     } else {
         constexpr int MID = (LO + HI) / 2;
         constexpr unsigned ML = ((1u << MID) - 1u) & ~((1u << LO) - 1u);
