@@ -175,9 +175,11 @@ void geometry_pair_units(Geometry &g, int u0, int u1, std::vector<uint32_t> &ord
 // slots each, starting at chunk sg_chunk_base[G0*nstrips]), off (the groups'
 // sg_off values, kSgAngles per chunk) and, per entry of the groups in table
 // order, col (column; 0xffffffff for a zero-weight padding entry), ang
-// (angle) and meta (slot | d << 8 | strip << 16).
+// (angle) and meta (slot | d << 8 | strip << 16); false if a group's
+// entries do not come out as the counting pass sized them (nothing is
+// written past a group's share then).
 int geometry_staged_count(Geometry &g);
-void geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage, std::vector<int64_t> &off,
+bool geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage, std::vector<int64_t> &off,
                             std::vector<uint32_t> &col, std::vector<uint32_t> &ang, std::vector<uint32_t> &meta);
 // Segment table of segments of width segw (nseg of them) for g's forward table.
 void segment_table(const Geometry &g, int segw, int nseg, std::vector<int> &seg);

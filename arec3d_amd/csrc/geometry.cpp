@@ -4,6 +4,7 @@
 // rounding, exactly like the device weight function that uses the same
 // formulas.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <map>
 #include <thread>
@@ -401,18 +402,31 @@ struct StagedItem {
     int8_t d[kSgAngles];  // first-tap offset in strip s per angle of the group, -1 outside [0, kStripGroups)
 };
 
-// One angle group's chunks and entries, in table order: stage holds
+// One angle group's chunks and entry counts, in table order: stage holds
 // kSgChunk slots per chunk of strips 0, 1, ...; nchunk[s] the chunks of
-// strip s; cnt kSgAngles entry counts per chunk; ent the entries {slot,
-// d, column (-1: padding)}, angle i's after angle i - 1's in each chunk.
-struct StagedGroup {
-    std::vector<StagedItem> items, sorted;
-    std::vector<int64_t> bucket, pos;  // counting-sort offsets per (strip, band)
-    std::vector<int> stage, nchunk, cnt;
-    std::vector<int4> ent;  // x slot, y d, z column, w angle
+// strip s; cnt kSgAngles entry counts per chunk (angle i's entries follow
+// angle i - 1's in each chunk); nent their sum.  With a sink the entries
+// themselves go straight to the upload arrays: column (-1: padding), angle,
+// and slot | d << 8 | strip << 16.
+struct StagedKey {
+    double t;
+    int j, q;
 };
 
-void staged_group(const Geometry &g, int G, StagedGroup &o) {
+struct StagedGroup {
+    std::vector<StagedItem> items, sorted, gather, band;
+    std::vector<StagedKey> keys;
+    std::vector<int64_t> sbase, bucket, pos;  // counting-sort offsets per strip and per band of a strip
+    std::vector<int> stage, nchunk, cnt;
+    int64_t nent = 0;
+};
+
+struct StagedSink {
+    uint32_t *col, *ang, *meta;  // the group's first entry in each array
+    int64_t cap;                 // the group's entries (from the counting pass)
+};
+
+void staged_group(const Geometry &g, int G, StagedGroup &o, const StagedSink *sink) {
     const int ns = g.nstrips, a0 = G * kSgAngles, na = std::min(kSgAngles, g.na - a0), ac = a0 + (na - 1) / 2;
     o.items.clear();
     int sa[2], un[2 * kSgAngles], p0[kSgAngles];
@@ -440,37 +454,64 @@ void staged_group(const Geometry &g, int G, StagedGroup &o) {
             o.items.push_back(it);
         }
     }
-    // order by (strip, band, depth, column): a counting sort on (strip,
-    // band), then each bucket (the columns along one ray of the central
-    // angle) by depth -- the order of one full sort, in linear time but for
-    // the small buckets
+    // order by (strip, band, depth, column): counting sorts by strip (one
+    // write stream per strip) and, inside each strip, by band; then each
+    // band (the columns along one ray of the central angle) by depth -- the
+    // order of one full sort, in linear time but for the small buckets
     if (!o.items.empty()) {
-        int bmin = o.items[0].band, bmax = bmin;
-        for (const StagedItem &it : o.items) {
-            bmin = std::min(bmin, it.band);
-            bmax = std::max(bmax, it.band);
-        }
-        const int nb = bmax - bmin + 1;
-        o.bucket.assign((size_t)ns * nb + 1, 0);
-        for (const StagedItem &it : o.items) o.bucket[(size_t)it.s * nb + (it.band - bmin) + 1]++;
-        for (size_t k = 1; k < o.bucket.size(); k++) o.bucket[k] += o.bucket[k - 1];
-        o.sorted.resize(o.items.size());
+        const size_t n = o.items.size();
         std::vector<int64_t> &pos = o.pos;
-        pos.assign(o.bucket.begin(), o.bucket.end() - 1);
-        for (const StagedItem &it : o.items) o.sorted[pos[(size_t)it.s * nb + (it.band - bmin)]++] = it;
-        for (size_t k = 0; k + 1 < o.bucket.size(); k++)
-            if (o.bucket[k + 1] - o.bucket[k] > 1)
-                std::sort(o.sorted.begin() + o.bucket[k], o.sorted.begin() + o.bucket[k + 1],
-                          [](const StagedItem &x, const StagedItem &y) {
-                              if (x.t != y.t) return x.t < y.t;
-                              return x.j < y.j;
-                          });
+        o.sbase.assign((size_t)ns + 1, 0);
+        for (const StagedItem &it : o.items) o.sbase[(size_t)it.s + 1]++;
+        for (int s = 0; s < ns; s++) o.sbase[(size_t)s + 1] += o.sbase[(size_t)s];
+        o.gather.resize(n);
+        pos.assign(o.sbase.begin(), o.sbase.end() - 1);
+        for (const StagedItem &it : o.items) o.gather[(size_t)pos[(size_t)it.s]++] = it;
+        o.sorted.resize(n);
+        for (int s = 0; s < ns; s++) {
+            const int64_t s0 = o.sbase[(size_t)s], s1 = o.sbase[(size_t)s + 1];
+            if (s0 == s1) continue;
+            int bmin = o.gather[(size_t)s0].band, bmax = bmin;
+            for (int64_t q = s0; q < s1; q++) {
+                bmin = std::min(bmin, o.gather[(size_t)q].band);
+                bmax = std::max(bmax, o.gather[(size_t)q].band);
+            }
+            const int nb = bmax - bmin + 1;
+            o.bucket.assign((size_t)nb + 1, 0);
+            o.bucket[0] = s0;
+            for (int64_t q = s0; q < s1; q++) o.bucket[(size_t)(o.gather[(size_t)q].band - bmin) + 1]++;
+            for (int k = 0; k < nb; k++) o.bucket[(size_t)k + 1] += o.bucket[(size_t)k];
+            pos.assign(o.bucket.begin(), o.bucket.end() - 1);
+            for (int64_t q = s0; q < s1; q++) {
+                const StagedItem &it = o.gather[(size_t)q];
+                o.sorted[(size_t)pos[(size_t)(it.band - bmin)]++] = it;
+            }
+            // each band by (depth, column) through 16-byte keys (the column
+            // is unique in a band of one strip), the items gathered back in
+            // that order
+            for (int k = 0; k < nb; k++) {
+                const int64_t b0 = o.bucket[(size_t)k], nbk = o.bucket[(size_t)k + 1] - b0;
+                if (nbk < 2) continue;
+                o.keys.resize((size_t)nbk);
+                for (int64_t q = 0; q < nbk; q++) {
+                    const StagedItem &it = o.sorted[(size_t)(b0 + q)];
+                    o.keys[(size_t)q] = StagedKey{it.t, it.j, (int)q};
+                }
+                std::sort(o.keys.begin(), o.keys.end(), [](const StagedKey &x, const StagedKey &y) {
+                    if (x.t != y.t) return x.t < y.t;
+                    return x.j < y.j;
+                });
+                o.band.resize((size_t)nbk);
+                for (int64_t q = 0; q < nbk; q++) o.band[(size_t)q] = o.sorted[(size_t)(b0 + o.keys[(size_t)q].q)];
+                std::copy(o.band.begin(), o.band.end(), o.sorted.begin() + b0);
+            }
+        }
         o.items.swap(o.sorted);
     }
     o.stage.clear();
     o.nchunk.assign(ns, 0);
     o.cnt.clear();
-    o.ent.clear();
+    o.nent = 0;
     int runs[kStripGroups + 1];
     int fill[kSgAngles][kStripGroups];  // per angle and offset: entries of the chunk being filled
     size_t idx = 0;
@@ -505,24 +546,46 @@ void staged_group(const Geometry &g, int G, StagedGroup &o) {
             }
             o.nchunk[s]++;
             for (int q = 0; q < kSgChunk; q++) o.stage.push_back(q < nq ? o.items[q0 + q].j : -1);
+            // (chunk, angle)'s entries by offset, each offset's columns in
+            // stage order (a stable counting sort), each run padded to whole
+            // batches with zero-weight entries
             for (int i = 0; i < kSgAngles; i++) {
-                const size_t e0 = o.ent.size();
+                int tot = 0;
                 if (i < na) {
                     std::fill(runs, runs + kStripGroups + 1, 0);
                     for (int q = 0; q < nq; q++) {
                         const int d = o.items[q0 + q].d[i];
                         if (d >= 0) runs[d]++;
                     }
+                    int at[kStripGroups];
                     for (int d = 0; d < kStripGroups; d++) {
-                        if (!runs[d]) continue;
+                        at[d] = tot;
+                        tot += up(runs[d]);
+                    }
+                    if (sink && o.nent + tot <= sink->cap) {
+                        uint32_t *col = sink->col + o.nent, *ang = sink->ang + o.nent, *meta = sink->meta + o.nent;
+                        const uint32_t sd = (uint32_t)s << 16;
                         for (int q = 0; q < nq; q++) {
                             const StagedItem &it = o.items[q0 + q];
-                            if (it.d[i] == d) o.ent.push_back(make_int4(q, d, it.j, a0 + i));
+                            const int d = it.d[i];
+                            if (d < 0) continue;
+                            const int k = at[d]++;
+                            col[k] = (uint32_t)it.j;
+                            ang[k] = (uint32_t)(a0 + i);
+                            meta[k] = (uint32_t)q | ((uint32_t)d << 8) | sd;
                         }
-                        for (int k = runs[d]; k % kStripBatch; k++) o.ent.push_back(make_int4(0, d, -1, a0 + i));
+                        for (int d = 0, end = 0; d < kStripGroups; d++) {
+                            end += up(runs[d]);
+                            for (int k = at[d]; k < end; k++) {
+                                col[k] = 0xffffffffu;
+                                ang[k] = (uint32_t)(a0 + i);
+                                meta[k] = ((uint32_t)d << 8) | sd;
+                            }
+                        }
                     }
                 }
-                o.cnt.push_back((int)(o.ent.size() - e0));
+                o.cnt.push_back(tot);
+                o.nent += tot;
             }
             q0 += nq;
         }
@@ -546,9 +609,9 @@ int geometry_staged_count(Geometry &g) {
     auto work = [&](unsigned t) {
         StagedGroup o;
         for (int G = (int)t; G < ngroups; G += (int)nth) {
-            staged_group(g, G, o);
+            staged_group(g, G, o, nullptr);
             for (int s = 0; s < ns; s++) chunks[(size_t)G * ns + s] = o.nchunk[s];
-            ent[G] = (int64_t)o.ent.size();
+            ent[G] = o.nent;
             for (int c : o.cnt) most[G] = std::max(most[G], c);
         }
     };
@@ -562,7 +625,7 @@ int geometry_staged_count(Geometry &g) {
     return *std::max_element(most.begin(), most.end());
 }
 
-void geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage, std::vector<int64_t> &off,
+bool geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage, std::vector<int64_t> &off,
                             std::vector<uint32_t> &col, std::vector<uint32_t> &ang, std::vector<uint32_t> &meta) {
     const int ns = g.nstrips;
     const int64_t K0 = g.sg_chunk_base[(size_t)G0 * ns], K1 = g.sg_chunk_base[(size_t)G1 * ns];
@@ -573,10 +636,15 @@ void geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage
     ang.resize((size_t)(E1 - E0));
     meta.resize((size_t)(E1 - E0));
     const unsigned nth = staged_threads(G1 - G0);
+    std::atomic<bool> bad{false};
     auto work = [&](unsigned t) {
         StagedGroup o;
         for (int G = G0 + (int)t; G < G1; G += (int)nth) {
-            staged_group(g, G, o);
+            const int64_t o0 = g.sg_group_ent[G] - E0;
+            const int64_t cap = g.sg_group_ent[G + 1] - g.sg_group_ent[G];
+            const StagedSink sink{col.data() + o0, ang.data() + o0, meta.data() + o0, cap};
+            staged_group(g, G, o, &sink);
+            if (o.nent != cap) bad = true;
             const int64_t Kg = g.sg_chunk_base[(size_t)G * ns] - K0;
             std::copy(o.stage.begin(), o.stage.end(), stage.begin() + Kg * kSgChunk);
             int64_t e = g.sg_group_ent[G];
@@ -584,24 +652,12 @@ void geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage
                 off[(size_t)Kg * kSgAngles + k] = e;
                 e += o.cnt[k];
             }
-            // the strip of each chunk, for the device's check of d
-            std::vector<int> strip_of;
-            for (int s = 0; s < ns; s++) strip_of.insert(strip_of.end(), (size_t)o.nchunk[s], s);
-            size_t k = 0;
-            int64_t o0 = g.sg_group_ent[G] - E0;
-            for (size_t c = 0; c < strip_of.size(); c++)
-                for (int i = 0; i < kSgAngles; i++)
-                    for (int m = 0; m < o.cnt[c * kSgAngles + i]; m++, k++) {
-                        const int4 v = o.ent[k];
-                        col[o0 + k] = (uint32_t)v.z;
-                        ang[o0 + k] = (uint32_t)v.w;
-                        meta[o0 + k] = (uint32_t)v.x | ((uint32_t)v.y << 8) | ((uint32_t)strip_of[c] << 16);
-                    }
         }
     };
     std::vector<std::thread> pool;
     for (unsigned t = 0; t < nth; t++) pool.emplace_back(work, t);
     for (auto &th : pool) th.join();
+    return !bad;
 }
 
 // Stable counting sort of the contributing columns of angles [a0, a1) by

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <thread>
@@ -1940,10 +1941,14 @@ static int build_staged_tables(Geometry &g, const int rpl, hipStream_t st) {
         set_error("staged-strip forward table: at most 65535 angles and strips");
         return AREC_ESIZE;
     }
+    const auto tc0 = std::chrono::steady_clock::now();
     if (geometry_staged_count(g) > kSgEnt) {
         set_error("staged-strip forward table: a chunk holds more entries for one angle than the kernel stages");
         return AREC_ESIZE;
     }
+    if (getenv("AREC_PROGRESS"))
+        fprintf(stderr, "[staged table] %lld entries counted in %.1f s\n", (long long)g.sg_group_ent.back(),
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - tc0).count());
     const int ns = g.nstrips, ngroups = (g.na + kSgAngles - 1) / kSgAngles;
     const int64_t nchunk = g.sg_chunk_base.back(), total = g.sg_group_ent.back();
     // Limited by the device memory free now (design 1's table is released
@@ -1978,12 +1983,21 @@ static int build_staged_tables(Geometry &g, const int rpl, hipStream_t st) {
     // host thread (the groups of a batch are ordered in parallel) within
     // 512 M entries (6 GB of staging on each side)
     const int min_groups = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    double t_host = 0.0, t_copy = 0.0;  // seconds: host ordering, waits for the uploads
+    auto now = []() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
     for (int G0 = 0; G0 < ngroups;) {
         int G1 = G0 + 1;
         while (G1 < ngroups && (g.sg_group_ent[G1 + 1] - g.sg_group_ent[G0] <= (int64_t)(32 << 20) ||
                                 (G1 - G0 < min_groups && g.sg_group_ent[G1 + 1] - g.sg_group_ent[G0] <= (int64_t)(512 << 20))))
             G1++;
-        geometry_staged_groups(g, G0, G1, stage, off, col, angi, meta);
+        double t0 = now();
+        if (!geometry_staged_groups(g, G0, G1, stage, off, col, angi, meta)) {
+            hipFree(d_tmp);
+            set_error("staged-strip table: the host ordering's two passes disagree");
+            return AREC_EGEOM;
+        }
+        double t1 = now();
+        t_host += t1 - t0;
         AREC_HIP(hipStreamSynchronize(st));  // the previous batch is done with d_tmp
         const int64_t n = (int64_t)col.size();
         if (n > cap) {
@@ -2009,8 +2023,10 @@ static int build_staged_tables(Geometry &g, const int rpl, hipStream_t st) {
                 build_staged_table<kLinear><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
             AREC_HIP(hipGetLastError());
         }
+        t_copy += now() - t1;
         if (progress)
-            fprintf(stderr, "[staged table] angle groups %d of %d ordered and uploaded\n", G1, ngroups);
+            fprintf(stderr, "[staged table] angle groups %d of %d ordered and uploaded (%.1f s ordering, %.1f s uploads)\n",
+                    G1, ngroups, t_host, t_copy);
         G0 = G1;
     }
     AREC_HIP(hipMemcpyAsync(g.d_sg_off + nchunk * kSgAngles, &g.sg_group_ent.back(), sizeof(int64_t),

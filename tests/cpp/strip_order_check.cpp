@@ -98,7 +98,7 @@ int main(int argc, char **argv) {
     std::vector<int> stage;
     std::vector<int64_t> off;
     std::vector<uint32_t> col, angi, meta;
-    geometry_staged_groups(q, 0, ngroups, stage, off, col, angi, meta);
+    if (!geometry_staged_groups(q, 0, ngroups, stage, off, col, angi, meta)) fails++;
     std::map<std::tuple<int, int, int>, int> d4;
     const int64_t nchunk = q.sg_chunk_base.back(), total = q.sg_group_ent.back();
     if ((int64_t)stage.size() != nchunk * kSgChunk || (int64_t)off.size() != nchunk * kSgAngles ||
