@@ -181,6 +181,10 @@ void geometry_pair_units(Geometry &g, int u0, int u1, std::vector<uint32_t> &ord
 int geometry_staged_count(Geometry &g);
 bool geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage, std::vector<int64_t> &off,
                             std::vector<uint32_t> &col, std::vector<uint32_t> &ang, std::vector<uint32_t> &meta);
+// The same into caller buffers of at least the groups' chunk and entry counts
+// (the stage slots of chunks without a column are written too).
+bool geometry_staged_groups(Geometry &g, int G0, int G1, int *stage, int64_t *off, uint32_t *col, uint32_t *ang,
+                            uint32_t *meta);
 // Segment table of segments of width segw (nseg of them) for g's forward table.
 void segment_table(const Geometry &g, int segw, int nseg, std::vector<int> &seg);
 

@@ -625,16 +625,11 @@ int geometry_staged_count(Geometry &g) {
     return *std::max_element(most.begin(), most.end());
 }
 
-bool geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage, std::vector<int64_t> &off,
-                            std::vector<uint32_t> &col, std::vector<uint32_t> &ang, std::vector<uint32_t> &meta) {
+bool geometry_staged_groups(Geometry &g, int G0, int G1, int *stage, int64_t *off, uint32_t *col, uint32_t *ang,
+                            uint32_t *meta) {
     const int ns = g.nstrips;
     const int64_t K0 = g.sg_chunk_base[(size_t)G0 * ns], K1 = g.sg_chunk_base[(size_t)G1 * ns];
-    const int64_t E0 = g.sg_group_ent[G0], E1 = g.sg_group_ent[G1];
-    stage.assign((size_t)(K1 - K0) * kSgChunk, -1);
-    off.assign((size_t)(K1 - K0) * kSgAngles, 0);
-    col.resize((size_t)(E1 - E0));
-    ang.resize((size_t)(E1 - E0));
-    meta.resize((size_t)(E1 - E0));
+    const int64_t E0 = g.sg_group_ent[G0];
     const unsigned nth = staged_threads(G1 - G0);
     std::atomic<bool> bad{false};
     auto work = [&](unsigned t) {
@@ -642,11 +637,16 @@ bool geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage
         for (int G = G0 + (int)t; G < G1; G += (int)nth) {
             const int64_t o0 = g.sg_group_ent[G] - E0;
             const int64_t cap = g.sg_group_ent[G + 1] - g.sg_group_ent[G];
-            const StagedSink sink{col.data() + o0, ang.data() + o0, meta.data() + o0, cap};
+            const StagedSink sink{col + o0, ang + o0, meta + o0, cap};
             staged_group(g, G, o, &sink);
             if (o.nent != cap) bad = true;
             const int64_t Kg = g.sg_chunk_base[(size_t)G * ns] - K0;
-            std::copy(o.stage.begin(), o.stage.end(), stage.begin() + Kg * kSgChunk);
+            const int64_t nk = g.sg_chunk_base[(size_t)(G + 1) * ns] - g.sg_chunk_base[(size_t)G * ns];
+            if ((int64_t)o.stage.size() != nk * kSgChunk || (int64_t)o.cnt.size() != nk * kSgAngles) {
+                bad = true;
+                continue;
+            }
+            std::copy(o.stage.begin(), o.stage.end(), stage + Kg * kSgChunk);
             int64_t e = g.sg_group_ent[G];
             for (size_t k = 0; k < o.cnt.size(); k++) {
                 off[(size_t)Kg * kSgAngles + k] = e;
@@ -657,7 +657,21 @@ bool geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage
     std::vector<std::thread> pool;
     for (unsigned t = 0; t < nth; t++) pool.emplace_back(work, t);
     for (auto &th : pool) th.join();
+    (void)K1;
     return !bad;
+}
+
+bool geometry_staged_groups(Geometry &g, int G0, int G1, std::vector<int> &stage, std::vector<int64_t> &off,
+                            std::vector<uint32_t> &col, std::vector<uint32_t> &ang, std::vector<uint32_t> &meta) {
+    const int ns = g.nstrips;
+    const int64_t K0 = g.sg_chunk_base[(size_t)G0 * ns], K1 = g.sg_chunk_base[(size_t)G1 * ns];
+    const int64_t n = g.sg_group_ent[G1] - g.sg_group_ent[G0];
+    stage.assign((size_t)(K1 - K0) * kSgChunk, -1);
+    off.assign((size_t)(K1 - K0) * kSgAngles, 0);
+    col.resize((size_t)n);
+    ang.resize((size_t)n);
+    meta.resize((size_t)n);
+    return geometry_staged_groups(g, G0, G1, stage.data(), off.data(), col.data(), ang.data(), meta.data());
 }
 
 // Stable counting sort of the contributing columns of angles [a0, a1) by

@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <memory>
 #include <thread>
 
 #include "arec_internal.hpp"
@@ -1971,9 +1972,6 @@ static int build_staged_tables(Geometry &g, const int rpl, hipStream_t st) {
     AREC_HIP(hipMalloc((void **)&g.d_sg_off, sizeof(int64_t) * (size_t)(nchunk * kSgAngles + 1)));
     if ((rc = upload(&g.d_sg_chunk_base, g.sg_chunk_base, st))) return rc;
     AREC_HIP(hipMemsetAsync(g.d_err, 0, sizeof(int), st));
-    std::vector<int> stage;
-    std::vector<int64_t> off;
-    std::vector<uint32_t> col, angi, meta;
     uint32_t *d_tmp = nullptr;
     int64_t cap = 0;
     // progress lines on stderr when AREC_PROGRESS is set (the host ordering
@@ -1983,37 +1981,57 @@ static int build_staged_tables(Geometry &g, const int rpl, hipStream_t st) {
     // host thread (the groups of a batch are ordered in parallel) within
     // 512 M entries (6 GB of staging on each side)
     const int min_groups = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    double t_host = 0.0, t_copy = 0.0;  // seconds: host ordering, waits for the uploads
-    auto now = []() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
-    for (int G0 = 0; G0 < ngroups;) {
+    std::vector<int> bounds{0};
+    int64_t most_ent = 0, most_chunks = 0;
+    while (bounds.back() < ngroups) {
+        const int G0 = bounds.back();
         int G1 = G0 + 1;
         while (G1 < ngroups && (g.sg_group_ent[G1 + 1] - g.sg_group_ent[G0] <= (int64_t)(32 << 20) ||
                                 (G1 - G0 < min_groups && g.sg_group_ent[G1 + 1] - g.sg_group_ent[G0] <= (int64_t)(512 << 20))))
             G1++;
+        most_ent = std::max(most_ent, g.sg_group_ent[G1] - g.sg_group_ent[G0]);
+        most_chunks = std::max(most_chunks, g.sg_chunk_base[(size_t)G1 * ns] - g.sg_chunk_base[(size_t)G0 * ns]);
+        bounds.push_back(G1);
+    }
+    // host staging for the largest batch, written once per batch by the
+    // ordering threads (no zero fill)
+    std::unique_ptr<uint32_t[]> h_ent(new uint32_t[(size_t)std::max<int64_t>(1, 3 * most_ent)]);
+    std::unique_ptr<int[]> h_stage(new int[(size_t)std::max<int64_t>(1, most_chunks * kSgChunk)]);
+    std::unique_ptr<int64_t[]> h_off(new int64_t[(size_t)std::max<int64_t>(1, most_chunks * kSgAngles)]);
+    uint32_t *col = h_ent.get(), *angi = col + most_ent, *meta = angi + most_ent;
+    double t_host = 0.0, t_copy = 0.0;  // seconds: host ordering, waits for the uploads
+    auto now = []() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+    for (size_t bi = 0; bi + 1 < bounds.size(); bi++) {
+        const int G0 = bounds[bi], G1 = bounds[bi + 1];
+        const int64_t nchunks_b = g.sg_chunk_base[(size_t)G1 * ns] - g.sg_chunk_base[(size_t)G0 * ns];
         double t0 = now();
-        if (!geometry_staged_groups(g, G0, G1, stage, off, col, angi, meta)) {
+        // the previous batch's uploads have read the host staging and its
+        // table kernel is done with d_tmp
+        AREC_HIP(hipStreamSynchronize(st));
+        t_copy += now() - t0;
+        t0 = now();
+        if (!geometry_staged_groups(g, G0, G1, h_stage.get(), h_off.get(), col, angi, meta)) {
             hipFree(d_tmp);
             set_error("staged-strip table: the host ordering's two passes disagree");
             return AREC_EGEOM;
         }
         double t1 = now();
         t_host += t1 - t0;
-        AREC_HIP(hipStreamSynchronize(st));  // the previous batch is done with d_tmp
-        const int64_t n = (int64_t)col.size();
+        const int64_t n = g.sg_group_ent[G1] - g.sg_group_ent[G0];
         if (n > cap) {
             hipFree(d_tmp);
             cap = n;
             AREC_HIP(hipMalloc((void **)&d_tmp, 3 * sizeof(uint32_t) * (size_t)std::max<int64_t>(1, cap)));
         }
         const int64_t K0 = g.sg_chunk_base[(size_t)G0 * ns], E0 = g.sg_group_ent[G0];
-        AREC_HIP(hipMemcpyAsync(g.d_sg_stage + K0 * kSgChunk, stage.data(), sizeof(int) * stage.size(),
+        AREC_HIP(hipMemcpyAsync(g.d_sg_stage + K0 * kSgChunk, h_stage.get(), sizeof(int) * (size_t)(nchunks_b * kSgChunk),
                                 hipMemcpyHostToDevice, st));
-        AREC_HIP(hipMemcpyAsync(g.d_sg_off + K0 * kSgAngles, off.data(), sizeof(int64_t) * off.size(),
+        AREC_HIP(hipMemcpyAsync(g.d_sg_off + K0 * kSgAngles, h_off.get(), sizeof(int64_t) * (size_t)(nchunks_b * kSgAngles),
                                 hipMemcpyHostToDevice, st));
         if (n > 0) {
-            AREC_HIP(hipMemcpyAsync(d_tmp, col.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
-            AREC_HIP(hipMemcpyAsync(d_tmp + cap, angi.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
-            AREC_HIP(hipMemcpyAsync(d_tmp + 2 * cap, meta.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+            AREC_HIP(hipMemcpyAsync(d_tmp, col, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+            AREC_HIP(hipMemcpyAsync(d_tmp + cap, angi, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+            AREC_HIP(hipMemcpyAsync(d_tmp + 2 * cap, meta, 4 * (size_t)n, hipMemcpyHostToDevice, st));
             dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 65536));
             if (g.method == kSQ)
                 build_staged_table<kSQ><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
@@ -2027,7 +2045,6 @@ static int build_staged_tables(Geometry &g, const int rpl, hipStream_t st) {
         if (progress)
             fprintf(stderr, "[staged table] angle groups %d of %d ordered and uploaded (%.1f s ordering, %.1f s uploads)\n",
                     G1, ngroups, t_host, t_copy);
-        G0 = G1;
     }
     AREC_HIP(hipMemcpyAsync(g.d_sg_off + nchunk * kSgAngles, &g.sg_group_ent.back(), sizeof(int64_t),
                             hipMemcpyHostToDevice, st));
