@@ -628,7 +628,7 @@ int geometry_staged_count(Geometry &g) {
 bool geometry_staged_groups(Geometry &g, int G0, int G1, int *stage, int64_t *off, uint32_t *col, uint32_t *ang,
                             uint32_t *meta) {
     const int ns = g.nstrips;
-    const int64_t K0 = g.sg_chunk_base[(size_t)G0 * ns], K1 = g.sg_chunk_base[(size_t)G1 * ns];
+    const int64_t K0 = g.sg_chunk_base[(size_t)G0 * ns];
     const int64_t E0 = g.sg_group_ent[G0];
     const unsigned nth = staged_threads(G1 - G0);
     std::atomic<bool> bad{false};
@@ -657,7 +657,6 @@ bool geometry_staged_groups(Geometry &g, int G0, int G1, int *stage, int64_t *of
     std::vector<std::thread> pool;
     for (unsigned t = 0; t < nth; t++) pool.emplace_back(work, t);
     for (auto &th : pool) th.join();
-    (void)K1;
     return !bad;
 }
 
