@@ -82,6 +82,10 @@ def roofline(kernel, ms, vu, g, plan):
             out["traffic"] = k["hbm_bytes_per_launch"]
             out["traffic_over_compulsory"] = round(k["hbm_bytes_per_launch"] / compulsory, 2)
             out["traffic_rate_frac_of_hbm"] = round(k["hbm_bytes_per_launch"] / t / (HBM_PEAK_GBS * 1e9), 4)
+            if "valu_busy_frac" in k:
+                # VALU issue slots used (counter run of the same plan): the
+                # kernel's limiter, next to its flop fraction above
+                out["valu_busy_frac_counters"] = round(k["valu_busy_frac"], 4)
             out["traffic_source"] = os.path.relpath(pmc, ROOT)
     return out
 
