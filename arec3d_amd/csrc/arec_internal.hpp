@@ -73,6 +73,19 @@ constexpr int64_t kMaxBackTableBytes = 16ll << 30;
 // The depth-strip forward table is built (on request) up to this size.
 constexpr int64_t kMaxStripTableBytes = 32ll << 30;
 
+// One angle group's design-4 host ordering, kept by the counting pass for the
+// filling pass (which then skips ordering the group again): its staged
+// columns in stage order with their per-angle offsets in the strip (int8
+// each, -1 outside, packed), the columns of each chunk and the chunks of each
+// strip.
+struct StagedOrder {
+    bool kept = false;
+    std::vector<int32_t> col;
+    std::vector<uint64_t> offs;
+    std::vector<uint8_t> nq;
+    std::vector<int> nchunk;
+};
+
 struct Geometry {
     int method = kSQ;
     int nx = 0, ny = 0, na = 0;  // detector width, rows (height), angles
@@ -138,6 +151,7 @@ struct Geometry {
     // w0, w1, w2}, d = first tap - (s*kStripW - 2).
     std::vector<int64_t> sg_chunk_base;  // ngroups * nstrips + 1
     std::vector<int64_t> sg_group_ent;   // ngroups + 1: first entry of each group
+    std::vector<StagedOrder> sg_order;   // per group, while the filling pass has not used it
     int *d_sg_stage = nullptr;
     int64_t *d_sg_chunk_base = nullptr;
     int64_t *d_sg_off = nullptr;

@@ -6,8 +6,11 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <thread>
+
+#include <unistd.h>
 
 #include "arec_internal.hpp"
 
@@ -426,9 +429,21 @@ struct StagedSink {
     int64_t cap;                 // the group's entries (from the counting pass)
 };
 
-void staged_group(const Geometry &g, int G, StagedGroup &o, const StagedSink *sink) {
+// With keep, the group's ordered columns and chunk sizes are also stored
+// there; with replay (from such a keep), they are taken from it instead of
+// being ordered again.
+void staged_group(const Geometry &g, int G, StagedGroup &o, const StagedSink *sink, StagedOrder *keep,
+                  const StagedOrder *replay) {
     const int ns = g.nstrips, a0 = G * kSgAngles, na = std::min(kSgAngles, g.na - a0), ac = a0 + (na - 1) / 2;
     o.items.clear();
+    if (replay) {
+        o.items.resize(replay->col.size());
+        for (size_t q = 0; q < o.items.size(); q++) {
+            StagedItem &it = o.items[q];
+            it.j = replay->col[q];
+            std::memcpy(it.d, &replay->offs[q], sizeof(it.d));
+        }
+    } else {
     int sa[2], un[2 * kSgAngles], p0[kSgAngles];
     for (int j = 0; j < g.nvalid; j++) {
         int nu = 0;
@@ -454,11 +469,12 @@ void staged_group(const Geometry &g, int G, StagedGroup &o, const StagedSink *si
             o.items.push_back(it);
         }
     }
+    }  // ordered below unless replayed
     // order by (strip, band, depth, column): counting sorts by strip (one
     // write stream per strip) and, inside each strip, by band; then each
     // band (the columns along one ray of the central angle) by depth -- the
     // order of one full sort, in linear time but for the small buckets
-    if (!o.items.empty()) {
+    if (!replay && !o.items.empty()) {
         const size_t n = o.items.size();
         std::vector<int64_t> &pos = o.pos;
         o.sbase.assign((size_t)ns + 1, 0);
@@ -514,19 +530,34 @@ void staged_group(const Geometry &g, int G, StagedGroup &o, const StagedSink *si
     o.nent = 0;
     int runs[kStripGroups + 1];
     int fill[kSgAngles][kStripGroups];  // per angle and offset: entries of the chunk being filled
-    size_t idx = 0;
+    if (keep) {
+        keep->col.resize(o.items.size());
+        keep->offs.resize(o.items.size());
+        for (size_t q = 0; q < o.items.size(); q++) {
+            keep->col[q] = o.items[q].j;
+            std::memcpy(&keep->offs[q], o.items[q].d, sizeof(o.items[q].d));
+        }
+        keep->nq.clear();
+        keep->nchunk.assign(ns, 0);
+    }
+    auto up = [](int c) { return (c + kStripBatch - 1) / kStripBatch * kStripBatch; };
+    size_t idx = 0, kc = 0;
     for (int s = 0; s < ns; s++) {
         size_t n = 0;
-        while (idx + n < o.items.size() && o.items[idx + n].s == s) n++;
+        if (replay) {
+            for (int c = 0; c < replay->nchunk[(size_t)s]; c++) n += replay->nq[kc + (size_t)c];
+        } else {
+            while (idx + n < o.items.size() && o.items[idx + n].s == s) n++;
+        }
         for (size_t q0 = idx; q0 < idx + n;) {
             // the chunk takes up to kSgChunk columns, fewer when one more
             // column would give an angle more than kSgEnt entries once its
             // offset runs are padded to whole batches (widely spaced angles)
             int nq = 0;
+            if (replay) nq = replay->nq[kc++];
             std::fill(&fill[0][0], &fill[0][0] + kSgAngles * kStripGroups, 0);
             int padded[kSgAngles] = {};  // per angle: its entries with every offset's run padded
-            auto up = [](int c) { return (c + kStripBatch - 1) / kStripBatch * kStripBatch; };
-            while (nq < kSgChunk && q0 + nq < idx + n) {
+            while (!replay && nq < kSgChunk && q0 + nq < idx + n) {
                 const StagedItem &it = o.items[q0 + nq];
                 bool fits = true;
                 for (int i = 0; i < na && fits; i++) {
@@ -545,6 +576,10 @@ void staged_group(const Geometry &g, int G, StagedGroup &o, const StagedSink *si
                 nq++;
             }
             o.nchunk[s]++;
+            if (keep) {
+                keep->nq.push_back((uint8_t)nq);
+                keep->nchunk[(size_t)s]++;
+            }
             for (int q = 0; q < kSgChunk; q++) o.stage.push_back(q < nq ? o.items[q0 + q].j : -1);
             // (chunk, angle)'s entries by offset, each offset's columns in
             // stage order (a stable counting sort), each run padded to whole
@@ -606,10 +641,24 @@ int geometry_staged_count(Geometry &g) {
     std::vector<int64_t> chunks((size_t)ngroups * ns, 0), ent(ngroups, 0);
     std::vector<int> most(ngroups, 0);
     const unsigned nth = staged_threads(ngroups);
+    g.sg_order.assign((size_t)ngroups, StagedOrder());
+    std::atomic<int64_t> kept_bytes{0};
+    const long pages = sysconf(_SC_PHYS_PAGES), page = sysconf(_SC_PAGE_SIZE);
+    const int64_t keep_budget = pages > 0 && page > 0 ? (int64_t)pages * page / 8 : 0;
     auto work = [&](unsigned t) {
         StagedGroup o;
         for (int G = (int)t; G < ngroups; G += (int)nth) {
-            staged_group(g, G, o, nullptr);
+            // keep the group's order for the filling pass while the kept
+            // orders stay within an eighth of the host's memory
+            StagedOrder &k = g.sg_order[(size_t)G];
+            staged_group(g, G, o, nullptr, &k, nullptr);
+            const int64_t kb = (int64_t)(k.col.size() * 12 + k.nq.size() + k.nchunk.size() * 4);
+            if (kept_bytes.fetch_add(kb) + kb <= keep_budget) {
+                k.kept = true;
+            } else {
+                kept_bytes.fetch_sub(kb);
+                k = StagedOrder();
+            }
             for (int s = 0; s < ns; s++) chunks[(size_t)G * ns + s] = o.nchunk[s];
             ent[G] = o.nent;
             for (int c : o.cnt) most[G] = std::max(most[G], c);
@@ -638,7 +687,9 @@ bool geometry_staged_groups(Geometry &g, int G0, int G1, int *stage, int64_t *of
             const int64_t o0 = g.sg_group_ent[G] - E0;
             const int64_t cap = g.sg_group_ent[G + 1] - g.sg_group_ent[G];
             const StagedSink sink{col + o0, ang + o0, meta + o0, cap};
-            staged_group(g, G, o, &sink);
+            StagedOrder *kept = (size_t)G < g.sg_order.size() && g.sg_order[(size_t)G].kept ? &g.sg_order[(size_t)G] : nullptr;
+            staged_group(g, G, o, &sink, nullptr, kept);
+            if (kept) *kept = StagedOrder();  // used once
             if (o.nent != cap) bad = true;
             const int64_t Kg = g.sg_chunk_base[(size_t)G * ns] - K0;
             const int64_t nk = g.sg_chunk_base[(size_t)(G + 1) * ns] - g.sg_chunk_base[(size_t)G * ns];

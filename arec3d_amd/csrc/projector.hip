@@ -2046,6 +2046,7 @@ static int build_staged_tables(Geometry &g, const int rpl, hipStream_t st) {
             fprintf(stderr, "[staged table] angle groups %d of %d ordered and uploaded (%.1f s ordering, %.1f s uploads)\n",
                     G1, ngroups, t_host, t_copy);
     }
+    std::vector<StagedOrder>().swap(g.sg_order);  // every kept order has been used
     AREC_HIP(hipMemcpyAsync(g.d_sg_off + nchunk * kSgAngles, &g.sg_group_ent.back(), sizeof(int64_t),
                             hipMemcpyHostToDevice, st));
     AREC_HIP(hipStreamSynchronize(st));
@@ -2071,6 +2072,7 @@ static void release_staged_tables(Geometry &g) {
     g.sg_rows4 = 0;
     std::vector<int64_t>().swap(g.sg_chunk_base);
     std::vector<int64_t>().swap(g.sg_group_ent);
+    std::vector<StagedOrder>().swap(g.sg_order);
 }
 
 // Frees the depth-strip tables (device and host).
