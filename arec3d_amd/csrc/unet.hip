@@ -752,39 +752,56 @@ __global__ void to_bf16_kernel(const float *__restrict__ a, bf16_t *__restrict__
     GS(i, n) b[i] = f2bf(a[i]);
 }
 
-// cylinder [nrays][ny] <-> cube [ny][n][n] (outside the mask: 0)
-// Element e = j * ny + y of the cylinder volume <-> cube voxel (y, iz, ix)
-// of ray j; the index arithmetic in 32 bits where the volume allows (a
-// 64-bit division costs dozens of instructions per element).
-template <typename I>
-__device__ __forceinline__ void cyl_cube_pairs(const int *__restrict__ cord, const int nrays, const int ny,
-                                               const int n, const float *__restrict__ src, float *__restrict__ dst,
-                                               const bool to_cube) {
-    const I total = (I)nrays * ny;
-    for (I e = blockIdx.x * (I)blockDim.x + threadIdx.x; e < total; e += (I)gridDim.x * blockDim.x) {
-        const int j = (int)(e / (I)ny), y = (int)(e - (I)j * ny);
-        const int64_t c = ((int64_t)y * n + cord[2 * j + 1]) * n + cord[2 * j];
-        if (to_cube)
-            dst[c] = src[e];
-        else
-            dst[e] = src[c];
+// cylinder [nrays][ny] <-> cube [ny][n][n] (outside the mask: 0): element
+// j * ny + y of the cylinder volume <-> cube voxel (y, iz, ix) of ray j.
+// Cylinder to cube through 64-ray x 64-row tiles in LDS: each ray's rows
+// are read as one contiguous run and each cube row is written by
+// neighbouring rays (a direct element-wise copy scatters one of the two
+// sides: 2.2 ms at 512^3).  256 threads; tiles with rays fastest.
+constexpr int kCubeTile = 64;
+__global__ __launch_bounds__(256) void cyl_to_cube_kernel(const float *__restrict__ x, const int *__restrict__ cord,
+                                                          int nrays, int ny, int n, float *__restrict__ cube) {
+    __shared__ float t[kCubeTile][kCubeTile + 1];
+    const int nrt = (nrays + kCubeTile - 1) / kCubeTile, nyt = (ny + kCubeTile - 1) / kCubeTile;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int tile = blockIdx.x; tile < nrt * nyt; tile += gridDim.x) {
+        const int j0 = (tile % nrt) * kCubeTile, y0 = (tile / nrt) * kCubeTile;
+        for (int r = w; r < kCubeTile; r += 4) {
+            const int j = j0 + r, y = y0 + lane;
+            t[r][lane] = (j < nrays && y < ny) ? x[(int64_t)j * ny + y] : 0.0f;
+        }
+        __syncthreads();
+        const int j = j0 + lane;
+        if (j < nrays) {
+            const int64_t c0 = (int64_t)cord[2 * j + 1] * n + cord[2 * j];
+            for (int yy = w; yy < kCubeTile && y0 + yy < ny; yy += 4)
+                cube[(int64_t)(y0 + yy) * n * n + c0] = t[lane][yy];
+        }
+        __syncthreads();
     }
 }
 
-__global__ void cyl_to_cube_kernel(const float *__restrict__ x, const int *__restrict__ cord, int nrays,
-                                   int ny, int n, float *__restrict__ cube) {
-    if ((int64_t)nrays * ny < (int64_t)INT32_MAX - (int64_t)gridDim.x * blockDim.x)
-        cyl_cube_pairs<int>(cord, nrays, ny, n, x, cube, true);
-    else
-        cyl_cube_pairs<int64_t>(cord, nrays, ny, n, x, cube, true);
-}
-
-__global__ void cube_to_cyl_kernel(const float *__restrict__ cube, const int *__restrict__ cord, int nrays,
-                                   int ny, int n, float *__restrict__ x) {
-    if ((int64_t)nrays * ny < (int64_t)INT32_MAX - (int64_t)gridDim.x * blockDim.x)
-        cyl_cube_pairs<int>(cord, nrays, ny, n, cube, x, false);
-    else
-        cyl_cube_pairs<int64_t>(cord, nrays, ny, n, cube, x, false);
+// Cube to cylinder through the same tiles: cube rows read by neighbouring
+// rays, each ray's rows written as one contiguous run.
+__global__ __launch_bounds__(256) void cube_to_cyl_kernel(const float *__restrict__ cube, const int *__restrict__ cord,
+                                                          int nrays, int ny, int n, float *__restrict__ x) {
+    __shared__ float t[kCubeTile][kCubeTile + 1];
+    const int nrt = (nrays + kCubeTile - 1) / kCubeTile, nyt = (ny + kCubeTile - 1) / kCubeTile;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int tile = blockIdx.x; tile < nrt * nyt; tile += gridDim.x) {
+        const int j0 = (tile % nrt) * kCubeTile, y0 = (tile / nrt) * kCubeTile;
+        const int j = j0 + lane;
+        if (j < nrays) {
+            const int64_t c0 = (int64_t)cord[2 * j + 1] * n + cord[2 * j];
+            for (int yy = w; yy < kCubeTile && y0 + yy < ny; yy += 4) t[lane][yy] = cube[(int64_t)(y0 + yy) * n * n + c0];
+        }
+        __syncthreads();
+        for (int r = w; r < kCubeTile; r += 4) {
+            const int jr = j0 + r, y = y0 + lane;
+            if (jr < nrays && y < ny) x[(int64_t)jr * ny + y] = t[r][lane];
+        }
+        __syncthreads();
+    }
 }
 
 static dim3 ew(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384))); }
@@ -1240,7 +1257,9 @@ int arec_cyl_to_cube(arec_ctx *ctx, const float *x_dev, double radius, int64_t n
     AREC_HIP(hipMalloc((void **)&dc, sizeof(int) * cord.size()));
     AREC_HIP(hipMemcpyAsync(dc, cord.data(), sizeof(int) * cord.size(), hipMemcpyHostToDevice, ctx->stream));
     AREC_HIP(hipMemsetAsync(cube_dev, 0, sizeof(float) * (size_t)ny * n * n, ctx->stream));
-    cyl_to_cube_kernel<<<ew((int64_t)nrays * ny), 256, 0, ctx->stream>>>(x_dev, dc, nrays, (int)ny, n, cube_dev);
+    const int64_t ntile = (int64_t)((nrays + kCubeTile - 1) / kCubeTile) * ((ny + kCubeTile - 1) / kCubeTile);
+    cyl_to_cube_kernel<<<(unsigned)std::max<int64_t>(1, std::min<int64_t>(ntile, 16384)), 256, 0, ctx->stream>>>(
+        x_dev, dc, nrays, (int)ny, n, cube_dev);
     AREC_HIP(hipGetLastError());
     AREC_HIP(hipStreamSynchronize(ctx->stream));
     hipFree(dc);
@@ -1255,7 +1274,9 @@ int arec_cube_to_cyl(arec_ctx *ctx, const float *cube_dev, double radius, int64_
     int *dc;
     AREC_HIP(hipMalloc((void **)&dc, sizeof(int) * cord.size()));
     AREC_HIP(hipMemcpyAsync(dc, cord.data(), sizeof(int) * cord.size(), hipMemcpyHostToDevice, ctx->stream));
-    cube_to_cyl_kernel<<<ew((int64_t)nrays * ny), 256, 0, ctx->stream>>>(cube_dev, dc, nrays, (int)ny, n, x_dev);
+    const int64_t ntile = (int64_t)((nrays + kCubeTile - 1) / kCubeTile) * ((ny + kCubeTile - 1) / kCubeTile);
+    cube_to_cyl_kernel<<<(unsigned)std::max<int64_t>(1, std::min<int64_t>(ntile, 16384)), 256, 0, ctx->stream>>>(
+        cube_dev, dc, nrays, (int)ny, n, x_dev);
     AREC_HIP(hipGetLastError());
     AREC_HIP(hipStreamSynchronize(ctx->stream));
     hipFree(dc);
