@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <chrono>
 #include <cstdio>
@@ -1993,58 +1994,93 @@ static int build_staged_tables(Geometry &g, const int rpl, hipStream_t st) {
         most_chunks = std::max(most_chunks, g.sg_chunk_base[(size_t)G1 * ns] - g.sg_chunk_base[(size_t)G0 * ns]);
         bounds.push_back(G1);
     }
-    // host staging for the largest batch, written once per batch by the
-    // ordering threads (no zero fill)
-    std::unique_ptr<uint32_t[]> h_ent(new uint32_t[(size_t)std::max<int64_t>(1, 3 * most_ent)]);
-    std::unique_ptr<int[]> h_stage(new int[(size_t)std::max<int64_t>(1, most_chunks * kSgChunk)]);
-    std::unique_ptr<int64_t[]> h_off(new int64_t[(size_t)std::max<int64_t>(1, most_chunks * kSgAngles)]);
-    uint32_t *col = h_ent.get(), *angi = col + most_ent, *meta = angi + most_ent;
+    // two host staging sets, each sized for the largest batch and written
+    // once per batch by the ordering threads (no zero fill): batch b is
+    // ordered into set b & 1 while a helper thread uploads batch b - 1 from
+    // the other set and runs its table kernel (one upload at a time, so one
+    // device buffer)
+    const int nset = bounds.size() > 2 ? 2 : 1;
+    std::unique_ptr<uint32_t[]> h_ent[2];
+    std::unique_ptr<int[]> h_stage[2];
+    std::unique_ptr<int64_t[]> h_off[2];
+    for (int k = 0; k < nset; k++) {
+        h_ent[k].reset(new uint32_t[(size_t)std::max<int64_t>(1, 3 * most_ent)]);
+        h_stage[k].reset(new int[(size_t)std::max<int64_t>(1, most_chunks * kSgChunk)]);
+        h_off[k].reset(new int64_t[(size_t)std::max<int64_t>(1, most_chunks * kSgAngles)]);
+    }
+    cap = std::max<int64_t>(1, most_ent);
+    AREC_HIP(hipMalloc((void **)&d_tmp, 3 * sizeof(uint32_t) * (size_t)cap));
+    int dev = 0;
+    AREC_HIP(hipGetDevice(&dev));
+    std::atomic<int> up_err{0};  // the first failing call's hipError_t
+    auto upload = [&](const size_t bi, const int k) {
+        hipError_t e = hipSetDevice(dev);
+        const int G0 = bounds[bi], G1 = bounds[bi + 1];
+        const int64_t nchunks_b = g.sg_chunk_base[(size_t)G1 * ns] - g.sg_chunk_base[(size_t)G0 * ns];
+        const int64_t n = g.sg_group_ent[G1] - g.sg_group_ent[G0];
+        const int64_t K0 = g.sg_chunk_base[(size_t)G0 * ns], E0 = g.sg_group_ent[G0];
+        const uint32_t *col = h_ent[k].get(), *angi = col + most_ent, *meta = angi + most_ent;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(g.d_sg_stage + K0 * kSgChunk, h_stage[k].get(),
+                               sizeof(int) * (size_t)(nchunks_b * kSgChunk), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(g.d_sg_off + K0 * kSgAngles, h_off[k].get(),
+                               sizeof(int64_t) * (size_t)(nchunks_b * kSgAngles), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && n > 0) {
+            e = hipMemcpyAsync(d_tmp, col, 4 * (size_t)n, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(d_tmp + cap, angi, 4 * (size_t)n, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(d_tmp + 2 * cap, meta, 4 * (size_t)n, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) {
+                dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 65536));
+                if (g.method == kSQ)
+                    build_staged_table<kSQ><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
+                else if (g.method == kKB)
+                    build_staged_table<kKB><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
+                else
+                    build_staged_table<kLinear><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
+                e = hipGetLastError();
+            }
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(st);  // the staging set and d_tmp are free again
+        if (e != hipSuccess) {
+            int z = 0;
+            up_err.compare_exchange_strong(z, (int)e);
+        }
+    };
+    std::thread up;
     double t_host = 0.0, t_copy = 0.0;  // seconds: host ordering, waits for the uploads
     auto now = []() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
     for (size_t bi = 0; bi + 1 < bounds.size(); bi++) {
         const int G0 = bounds[bi], G1 = bounds[bi + 1];
-        const int64_t nchunks_b = g.sg_chunk_base[(size_t)G1 * ns] - g.sg_chunk_base[(size_t)G0 * ns];
+        const int k = (int)(bi % (size_t)nset);
+        if (nset == 1 && up.joinable()) up.join();  // one set: its upload must be done first
         double t0 = now();
-        // the previous batch's uploads have read the host staging and its
-        // table kernel is done with d_tmp
-        AREC_HIP(hipStreamSynchronize(st));
-        t_copy += now() - t0;
-        t0 = now();
-        if (!geometry_staged_groups(g, G0, G1, h_stage.get(), h_off.get(), col, angi, meta)) {
-            hipFree(d_tmp);
-            set_error("staged-strip table: the host ordering's two passes disagree");
-            return AREC_EGEOM;
-        }
+        uint32_t *col = h_ent[k].get(), *angi = col + most_ent, *meta = angi + most_ent;
+        const bool ok = geometry_staged_groups(g, G0, G1, h_stage[k].get(), h_off[k].get(), col, angi, meta);
         double t1 = now();
         t_host += t1 - t0;
-        const int64_t n = g.sg_group_ent[G1] - g.sg_group_ent[G0];
-        if (n > cap) {
-            hipFree(d_tmp);
-            cap = n;
-            AREC_HIP(hipMalloc((void **)&d_tmp, 3 * sizeof(uint32_t) * (size_t)std::max<int64_t>(1, cap)));
-        }
-        const int64_t K0 = g.sg_chunk_base[(size_t)G0 * ns], E0 = g.sg_group_ent[G0];
-        AREC_HIP(hipMemcpyAsync(g.d_sg_stage + K0 * kSgChunk, h_stage.get(), sizeof(int) * (size_t)(nchunks_b * kSgChunk),
-                                hipMemcpyHostToDevice, st));
-        AREC_HIP(hipMemcpyAsync(g.d_sg_off + K0 * kSgAngles, h_off.get(), sizeof(int64_t) * (size_t)(nchunks_b * kSgAngles),
-                                hipMemcpyHostToDevice, st));
-        if (n > 0) {
-            AREC_HIP(hipMemcpyAsync(d_tmp, col, 4 * (size_t)n, hipMemcpyHostToDevice, st));
-            AREC_HIP(hipMemcpyAsync(d_tmp + cap, angi, 4 * (size_t)n, hipMemcpyHostToDevice, st));
-            AREC_HIP(hipMemcpyAsync(d_tmp + 2 * cap, meta, 4 * (size_t)n, hipMemcpyHostToDevice, st));
-            dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 65536));
-            if (g.method == kSQ)
-                build_staged_table<kSQ><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
-            else if (g.method == kKB)
-                build_staged_table<kKB><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
-            else
-                build_staged_table<kLinear><<<grid, 256, 0, st>>>(d_tmp, d_tmp + cap, d_tmp + 2 * cap, n, g.d_coord, g.d_ang, g.radius, g.xcent, 256 * rpl, g.d_sg_tab + E0, g.d_err);
-            AREC_HIP(hipGetLastError());
-        }
+        if (up.joinable()) up.join();  // the previous batch (the other set) is uploaded
         t_copy += now() - t1;
+        if (!ok || up_err.load()) {
+            hipFree(d_tmp);
+            if (!ok) set_error("staged-strip table: the host ordering's two passes disagree");
+            else set_error(std::string("staged-strip table upload: ") + hipGetErrorString((hipError_t)up_err.load()));
+            return ok ? AREC_EHIP : AREC_EGEOM;
+        }
+        up = std::thread(upload, bi, k);
         if (progress)
             fprintf(stderr, "[staged table] angle groups %d of %d ordered and uploaded (%.1f s ordering, %.1f s uploads)\n",
                     G1, ngroups, t_host, t_copy);
+    }
+    {
+        const double t1 = now();
+        if (up.joinable()) up.join();
+        t_copy += now() - t1;
+    }
+    if (up_err.load()) {
+        hipFree(d_tmp);
+        set_error(std::string("staged-strip table upload: ") + hipGetErrorString((hipError_t)up_err.load()));
+        return AREC_EHIP;
     }
     std::vector<StagedOrder>().swap(g.sg_order);  // every kept order has been used
     AREC_HIP(hipMemcpyAsync(g.d_sg_off + nchunk * kSgAngles, &g.sg_group_ent.back(), sizeof(int64_t),
